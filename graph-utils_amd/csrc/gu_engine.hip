@@ -1,0 +1,1170 @@
+// gu_engine.hip -- kernels and host orchestration of the batched multi-source
+// search (BFS distances, path counts, Brandes dependencies, diameter) and the
+// device-wide load-balanced search, behind the C ABI in graph_utils_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/graph_utils_amd.h"
+#include "gu_kernels.hpp"
+
+namespace gu {
+
+// ---- small utility kernels ------------------------------------------------
+
+__global__ void k_fill_u64(u64 *p, size_t count, u64 v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t st = (size_t)gridDim.x * blockDim.x;
+  for (; i < count; i += st) p[i] = v;
+}
+__global__ void k_fill_i32(int *p, size_t count, int v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t st = (size_t)gridDim.x * blockDim.x;
+  for (; i < count; i += st) p[i] = v;
+}
+
+// F[k] = row of entry k (device_graph's COO array when the caller has none).
+__global__ void k_build_F(const int *R, int n, int *F) {
+  int v = blockIdx.x * (blockDim.x / kWave) + wave_id();
+  int stride = gridDim.x * (blockDim.x / kWave);
+  for (; v < n; v += stride)
+    for (int k = R[v] + lane_id(); k < R[v + 1]; k += kWave) F[k] = v;
+}
+
+// Graph validation: R non-decreasing from 0 to m, C in [0, n).
+__global__ void k_validate(const int *R, const int *C, int n, long long m, int *bad) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t st = (size_t)gridDim.x * blockDim.x;
+  for (size_t v = i; v < (size_t)n; v += st)
+    if (R[v + 1] < R[v]) atomicOr(bad, 1);
+  for (size_t k = i; k < (size_t)m; k += st)
+    if (C[k] < 0 || C[k] >= n) atomicOr(bad, 2);
+  if (i == 0 && (R[0] != 0 || (long long)R[n] != m)) atomicOr(bad, 4);
+}
+
+// ---- batch seeding (shuffle_based.cuh:104-153 per-source initialisation) --
+template <int NW>
+__global__ void k_seed(Eng<NW> e) {
+  const int tid = threadIdx.x;
+  u64 *f0 = frs(e, 0);
+  if (tid < e.nb) {
+    int s = e.srcs[tid];
+    int j = tid / kWL;
+    u64 bit = 1ull << (tid % kWL);
+    atomicOr(&e.visit[(size_t)s * NW + j], bit);
+    atomicOr(&f0[(size_t)s * NW + j], bit);
+    e.dvm[(size_t)s * (kWL * NW) + tid] = 0;
+    e.svm[(size_t)s * (kWL * NW) + tid] = 1;
+  }
+  __syncthreads();
+  if (tid < e.nb) {
+    int s = e.srcs[tid];
+    if (atomicMax(&e.tag[s], 0) < 0) {  // first lane with this source pushes it
+      unsigned pos = atomicAdd(&e.ctrl->qlen[0], 1u);
+      e.q[pos] = s;  // slot 0 and stack offset 0 coincide
+      atomicAdd(&e.ctrl->fr_edges[0], (u64)(e.R[s + 1] - e.R[s]));
+    }
+  }
+  __syncthreads();
+  if (e.stack) {
+    unsigned q0 = atomicAdd(&e.ctrl->qlen[0], 0u);
+    for (unsigned i = tid; i < q0; i += blockDim.x) {
+      int s = e.q[i];
+      for (int j = 0; j < NW; j++) e.smask[(size_t)i * NW + j] = atomicOr(&f0[(size_t)s * NW + j], 0ull);
+    }
+    if (tid == 0) {
+      e.ep[0] = 0;
+      e.ep[1] = q0;
+      e.ctrl->S_len = q0;
+    }
+  }
+}
+
+// ---- one level ------------------------------------------------------------
+template <int NW, int VAR, bool SIGMA>
+__global__ __launch_bounds__(1024) void k_expand(Eng<NW> e) {
+  if (e.ctrl->done) return;
+  const int L = e.ctrl->level;
+  const int mode = e.ctrl->mode;
+  if (L >= 1) {  // clear the stale frontier slot (level L-1) for reuse at L+1
+    const int *Qs = qptr(e, L - 1);
+    const unsigned qs = e.ctrl->qlen[slot_of(L - 1)];
+    u64 *fs = frs(e, L - 1);
+    const unsigned nt = gridDim.x * blockDim.x;
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < qs; i += nt) {
+      int v = Qs[i];
+#pragma unroll
+      for (int j = 0; j < NW; j++) fs[(size_t)v * NW + j] = 0;
+    }
+  }
+  if (mode != 0) {
+    bu_level<NW, SIGMA>(e, L);
+    return;
+  }
+  if (VAR == V_SHUFFLE) td_shuffle<NW>(e, L);
+  else if (VAR == V_WARP) td_warp<NW>(e, L);
+  else if (VAR == V_CTA_WARP) td_cta_warp<NW>(e, L);
+  else if (VAR == V_SCAN) td_scan<NW>(e, L);
+  else if (VAR == V_EDGE_PARALLEL) td_edge_parallel<NW>(e, L);
+  else td_linear<NW>(e, L);
+}
+
+template <int NW, bool SIGMA>
+__global__ __launch_bounds__(1024) void k_finalize(Eng<NW> e) {
+  if (e.ctrl->done || e.ctrl->mode != 0) return;
+  td_finalize<NW, SIGMA>(e, e.ctrl->level);
+}
+
+template <int NW>
+__global__ void k_control(Eng<NW> e) {
+  if (threadIdx.x != 0) return;
+  Ctrl *c = e.ctrl;
+  if (c->done) return;
+  const int L = c->level;
+  const unsigned qn = c->qlen[slot_of(L + 1)];
+  c->unvis_edges -= c->unvis_dec;
+  c->unvis_dec = 0;
+  if (c->mode == 0)
+    c->td_levels++;
+  else
+    c->bu_levels++;
+  if (qn == 0) {
+    c->done = 1;
+    c->last_level = L;
+    return;
+  }
+  if (e.stack) {
+    e.ep[L + 2] = e.ep[L + 1] + qn;
+    c->S_len = e.ep[L + 2];
+  }
+  c->level = L + 1;
+  c->qlen[slot_of(L + 2)] = 0;
+  c->fr_edges[slot_of(L + 2)] = 0;
+  for (int i = 0; i < 4; i++) c->work[i] = 0;
+  const u64 mf = c->fr_edges[slot_of(L + 1)];
+  const u64 mu = c->unvis_edges;
+  int mode = c->mode;
+  if (c->direction == GU_DIR_TOPDOWN)
+    mode = 0;
+  else if (c->direction == GU_DIR_BOTTOMUP)
+    mode = 1;
+  else if (mode == 0) {
+    if (mf * (u64)c->alpha > mu) mode = 1;
+  } else {
+    if ((u64)qn * (u64)c->beta < (u64)c->n) mode = 0;
+  }
+  c->mode = mode;
+}
+
+// ---- device-wide exclusive scan ------------------------------------------
+// Loader: value(i) for i < count().  Tile = blockDim * kScanIPT items.
+constexpr int kScanIPT = 8;
+
+struct FrontierDeg {
+  const int *R;
+  const int *Q;
+  __device__ int value(long long i) const {
+    int v = Q[i];
+    return R[v + 1] - R[v];
+  }
+};
+struct Counts {
+  const int *c;
+  __device__ int value(long long i) const { return c[i]; }
+};
+
+// Phase 1: per-tile sums.
+template <class Ld>
+__device__ void scan_reduce(Ld ld, long long count, long long *tsum) {
+  __shared__ long long s_red[16];
+  const int T = blockDim.x;
+  const long long TS = (long long)T * kScanIPT;
+  for (long long tile = blockIdx.x; tile * TS < count; tile += gridDim.x) {
+    long long b = tile * TS + (long long)threadIdx.x * kScanIPT;
+    long long s = 0;
+    for (int k = 0; k < kScanIPT; k++)
+      if (b + k < count) s += ld.value(b + k);
+    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, kWave);
+    if (lane_id() == 0) s_red[wave_id()] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      long long t = 0;
+      for (int w = 0; w < T / kWave; w++) t += s_red[w];
+      tsum[tile] = t;
+    }
+    __syncthreads();
+  }
+}
+
+// Phase 2: one workgroup scans the tile sums in place; *total = sum.
+__device__ void scan_tiles(long long *tsum, long long ntiles, long long *total) {
+  __shared__ long long s_carry;
+  __shared__ long long s_w[17];
+  const int T = blockDim.x;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (long long b = 0; b < ntiles; b += T) {
+    long long i = b + threadIdx.x;
+    long long x = i < ntiles ? tsum[i] : 0;
+    // wave inclusive scan (64-bit, shuffles)
+    long long inc = x;
+    for (int o = 1; o < kWave; o <<= 1) {
+      long long y = __shfl_up(inc, o, kWave);
+      if (lane_id() >= o) inc += y;
+    }
+    if (lane_id() == kWave - 1) s_w[wave_id()] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      long long acc = 0;
+      for (int w = 0; w < T / kWave; w++) {
+        long long t = s_w[w];
+        s_w[w] = acc;
+        acc += t;
+      }
+      s_w[16] = acc;
+    }
+    __syncthreads();
+    if (i < ntiles) tsum[i] = s_carry + s_w[wave_id()] + inc - x;
+    __syncthreads();
+    if (threadIdx.x == 0) s_carry += s_w[16];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = s_carry;
+}
+
+// Phase 3: per-tile exclusive scan plus the tile's offset; with write_total,
+// out[count] = total as well.
+template <class Ld>
+__device__ void scan_apply(Ld ld, long long count, const long long *tsum, int *out,
+                           const long long *total, bool write_total) {
+  __shared__ int s_scan[17];
+  const int T = blockDim.x;
+  const long long TS = (long long)T * kScanIPT;
+  for (long long tile = blockIdx.x; tile * TS < count; tile += gridDim.x) {
+    long long b = tile * TS + (long long)threadIdx.x * kScanIPT;
+    int vals[kScanIPT];
+    int s = 0;
+    for (int k = 0; k < kScanIPT; k++) {
+      vals[k] = b + k < count ? ld.value(b + k) : 0;
+      s += vals[k];
+    }
+    int tot;
+    int pre = block_exclusive_sum(s, s_scan, &tot);
+    long long base = tsum[tile] + pre;
+    for (int k = 0; k < kScanIPT; k++) {
+      if (b + k < count) out[b + k] = (int)base;
+      base += vals[k];
+    }
+  }
+  if (write_total && blockIdx.x == 0 && threadIdx.x == 0) out[count] = (int)*total;
+}
+
+// Frontier degree scan for the edge-parallel strategy (only when the level
+// runs top-down).
+template <int NW>
+__global__ __launch_bounds__(1024) void k_fscan_reduce(Eng<NW> e) {
+  if (e.ctrl->done || e.ctrl->mode != 0) return;
+  const int L = e.ctrl->level;
+  FrontierDeg ld{e.R, qptr(e, L)};
+  scan_reduce(ld, e.ctrl->qlen[slot_of(L)], e.tsum);
+}
+template <int NW>
+__global__ __launch_bounds__(1024) void k_fscan_tiles(Eng<NW> e) {
+  if (e.ctrl->done || e.ctrl->mode != 0) return;
+  const int L = e.ctrl->level;
+  const long long TS = (long long)blockDim.x * kScanIPT;  // same block size as the others
+  long long nt = ((long long)e.ctrl->qlen[slot_of(L)] + TS - 1) / TS;
+  scan_tiles(e.tsum, nt, e.tsum + (size_t)e.n + 1);
+}
+template <int NW>
+__global__ __launch_bounds__(1024) void k_fscan_apply(Eng<NW> e) {
+  if (e.ctrl->done || e.ctrl->mode != 0) return;
+  const int L = e.ctrl->level;
+  FrontierDeg ld{e.R, qptr(e, L)};
+  scan_apply(ld, e.ctrl->qlen[slot_of(L)], e.tsum, e.off, e.tsum + (size_t)e.n + 1, true);
+}
+
+// LBS scan kernels over an explicit counts array.
+__global__ __launch_bounds__(1024) void k_cscan_reduce(const int *c, long long S, long long *tsum) {
+  scan_reduce(Counts{c}, S, tsum);
+}
+__global__ __launch_bounds__(1024) void k_cscan_tiles(long long *tsum, long long nt, long long *total) {
+  scan_tiles(tsum, nt, total);
+}
+__global__ __launch_bounds__(1024) void k_cscan_apply(const int *c, long long S, const long long *tsum,
+                                                      int *out, const long long *total) {
+  if (*total >= INT_MAX) return;  // reported as a capacity error by the host
+  scan_apply(Counts{c}, S, tsum, out, total, false);
+}
+
+// load_balanced_search.cu:175-191 as a device-wide merge: result[k] =
+// upper_bound(scan, k) - 1 for k < W.  A workgroup takes kLbsIPT*blockDim
+// consecutive items; the scan entries spanning them are staged in LDS; a
+// tile inside one segment is a plain fill.
+constexpr int kLbsIPT = 16;
+__global__ __launch_bounds__(256) void k_lbs_expand(const int *scan, long long S, const long long *total,
+                                                    long long cap, int *result) {
+  const long long W = *total;
+  if (W > cap || W >= INT_MAX) return;  // reported as a capacity error by the host
+  constexpr int kCap = 256 * kLbsIPT + 1;
+  __shared__ int s_off[kCap];
+  __shared__ int s_lo, s_hi;
+  const int T = blockDim.x, tid = threadIdx.x;
+  const long long TE = (long long)T * kLbsIPT;
+  for (long long tile = blockIdx.x; tile * TE < W; tile += gridDim.x) {
+    const long long e0 = tile * TE;
+    const long long e1 = e0 + TE < W ? e0 + TE : W;
+    if (tid == 0) {
+      s_lo = seg_search(scan, 0, (int)S - 1, e0);
+      s_hi = seg_search(scan, s_lo, (int)S - 1, e1 - 1);
+    }
+    __syncthreads();
+    const int lo = s_lo, hi = s_hi;
+    if (lo == hi) {
+      for (long long k = e0 + tid; k < e1; k += T) result[k] = lo;
+    } else {
+      const bool in_lds = hi - lo + 1 <= kCap;
+      if (in_lds)
+        for (int i = tid; i <= hi - lo; i += T) s_off[i] = scan[lo + i];
+      __syncthreads();
+      for (long long k = e0 + tid; k < e1; k += T)
+        result[k] = in_lds ? lo + seg_search(s_off, 0, hi - lo, k) : seg_search(scan, lo, hi, k);
+    }
+    __syncthreads();
+  }
+}
+
+// ---- outputs ----------------------------------------------------------------
+// Transpose vertex-major distances / path counts of the batch into the
+// caller's source-major rows; lanes that did not reach v get INT_MAX / 0.
+template <int NW>
+__global__ __launch_bounds__(256) void k_output(Eng<NW> e, int *dout, u64 *sout) {
+  __shared__ int td[64][65];
+  __shared__ u64 ts[64][65];
+  const int tid = threadIdx.x, T = blockDim.x;
+  const int col = tid % 64;
+  for (long long v0 = (long long)blockIdx.x * 64; v0 < e.n; v0 += (long long)gridDim.x * 64) {
+    for (int j = 0; j < NW; j++) {
+      if (j * kWL >= e.nb) break;
+      for (int rr = tid / 64; rr < 64; rr += T / 64) {
+        long long v = v0 + rr;
+        bool ok = false;
+        if (v < e.n) ok = (e.visit[(size_t)v * NW + j] >> col) & 1ull;
+        size_t idx = (size_t)v * (kWL * NW) + j * kWL + col;
+        if (dout) td[rr][col] = ok ? e.dvm[idx] : INT_MAX;
+        if (sout) ts[rr][col] = ok ? e.svm[idx] : 0ull;
+      }
+      __syncthreads();
+      for (int bb = tid / 64; bb < 64; bb += T / 64) {
+        int b = j * kWL + bb;
+        long long v = v0 + col;
+        if (b < e.nb && v < e.n) {
+          if (dout) dout[(size_t)b * e.n + v] = td[col][bb];
+          if (sout) sout[(size_t)b * e.n + v] = ts[col][bb];
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Sum over lanes of out-degrees (TEPS units) and of reached vertices.
+template <int NW>
+__global__ __launch_bounds__(256) void k_teps(Eng<NW> e) {
+  __shared__ u64 s_a[4], s_b[4];
+  u64 a = 0, b = 0;
+  size_t st = (size_t)gridDim.x * blockDim.x;
+  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < (size_t)e.n; v += st) {
+    int pc = 0;
+    for (int j = 0; j < NW; j++) pc += __popcll(e.visit[v * NW + j]);
+    a += (u64)pc * (u64)(e.R[v + 1] - e.R[v]);
+    b += (u64)pc;
+  }
+  for (int o = 32; o >= 1; o >>= 1) {
+    a += __shfl_xor(a, o, kWave);
+    b += __shfl_xor(b, o, kWave);
+  }
+  if (lane_id() == 0) {
+    s_a[wave_id()] = a;
+    s_b[wave_id()] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u64 ta = 0, tb = 0;
+    for (int w = 0; w < (int)blockDim.x / kWave; w++) {
+      ta += s_a[w];
+      tb += s_b[w];
+    }
+    atomicAdd(&e.ctrl->teps, ta);
+    atomicAdd(&e.ctrl->reached, tb);
+  }
+}
+
+// ---- Brandes backward (shuffle_based.cu:217-354 pull form) ----------------
+// Level L, lanes mk of vertex v (lanes for which d(v) = L):
+//   delta_b(v) = sum over neighbours w with d_b(w) = L+1 of
+//                (float)sigma_b(v) / (float)sigma_b(w) * (1 + delta_b(w))
+// summed in FP32 with Neumaier compensation.  Successor lanes come from a
+// dense per-parity mask array tagged with the level that wrote it, so a
+// level never needs a clearing pass.
+template <int NW>
+__global__ __launch_bounds__(256) void k_backward(Eng<NW> e, float *dl, int *btag, u64 *blm,
+                                                  unsigned *bwork, int L) {
+  const int lane = lane_id();
+  const long long beg = e.ep[L], end = e.ep[L + 1];
+  const long long cnt_all = end - beg;
+  const int pr = (L + 1) & 1, pw = L & 1;
+  const int *tagr = btag + (size_t)pr * e.n;
+  const u64 *lmr = blm + (size_t)pr * e.n * NW;
+  int *tagw = btag + (size_t)pw * e.n;
+  u64 *lmw = blm + (size_t)pw * e.n * NW;
+  while (true) {
+    unsigned base = 0;
+    if (lane == 0) base = atomicAdd(&bwork[L], (unsigned)kWave);
+    base = bcast(base);
+    if ((long long)base >= cnt_all) break;
+    const int cnt = cnt_all - base < kWave ? (int)(cnt_all - base) : kWave;
+    for (int i = 0; i < cnt; i++) {
+      const long long pos = beg + base + i;
+      const int v = e.q[pos];
+      u64 mk[NW];
+      float sv[NW], sum[NW], comp[NW];
+#pragma unroll
+      for (int j = 0; j < NW; j++) {
+        mk[j] = e.smask[(size_t)pos * NW + j];
+        sum[j] = 0.0f;
+        comp[j] = 0.0f;
+        sv[j] = ((mk[j] >> lane) & 1ull) ? (float)e.svm[(size_t)v * (kWL * NW) + j * kWL + lane] : 0.0f;
+      }
+      const int r0 = e.R[v], r1 = e.R[v + 1];
+      for (int e0 = r0; e0 < r1; e0 += kWave) {
+        int ed = e0 + lane;
+        bool valid = ed < r1;
+        int w = valid ? e.C[ed] : 0;
+        bool ok = valid && tagr[w] == L + 1;
+        u64 t[NW];
+        bool has = false;
+#pragma unroll
+        for (int j = 0; j < NW; j++) {
+          t[j] = ok ? (lmr[(size_t)w * NW + j] & mk[j]) : 0ull;
+          has |= t[j] != 0;
+        }
+        uint64_t pm = ballot(has);
+        while (pm) {
+          int k = __ffsll((unsigned long long)pm) - 1;
+          pm &= pm - 1;
+          int wk = readlane(w, k);
+          const size_t rowb = (size_t)wk * (kWL * NW);
+#pragma unroll
+          for (int j = 0; j < NW; j++) {
+            u64 tk = readlane64(t[j], k);
+            if ((tk >> lane) & 1ull) {
+              float sw = (float)e.svm[rowb + j * kWL + lane];
+              float dw = dl[rowb + j * kWL + lane];
+              float x = (sv[j] / sw) * (1.0f + dw);
+              float s = sum[j] + x;
+              if (fabsf(sum[j]) >= fabsf(x))
+                comp[j] += (sum[j] - s) + x;
+              else
+                comp[j] += (x - s) + sum[j];
+              sum[j] = s;
+            }
+          }
+        }
+      }
+      const size_t vrow = (size_t)v * (kWL * NW);
+#pragma unroll
+      for (int j = 0; j < NW; j++)
+        if ((mk[j] >> lane) & 1ull) dl[vrow + j * kWL + lane] = sum[j] + comp[j];
+      if (lane == 0) {
+#pragma unroll
+        for (int j = 0; j < NW; j++) lmw[(size_t)v * NW + j] = mk[j];
+        tagw[v] = L;
+      }
+    }
+  }
+}
+
+// Dependency rows (source-major, optional) and the FP64 BC accumulation
+// bc[v] += sum over lanes of delta_b(v), summed lane by lane in a fixed
+// order so the result is deterministic.
+template <int NW>
+__global__ __launch_bounds__(256) void k_bc_out(Eng<NW> e, const float *dl, float *dout, double *bc) {
+  __shared__ float tl[64][65];
+  const int tid = threadIdx.x, T = blockDim.x;
+  const int col = tid % 64;
+  for (long long v0 = (long long)blockIdx.x * 64; v0 < e.n; v0 += (long long)gridDim.x * 64) {
+    double acc = 0.0;
+    for (int j = 0; j < NW; j++) {
+      if (j * kWL >= e.nb) break;
+      for (int rr = tid / 64; rr < 64; rr += T / 64) {
+        long long v = v0 + rr;
+        float x = 0.0f;
+        if (v < e.n && ((e.visit[(size_t)v * NW + j] >> col) & 1ull)) {
+          size_t idx = (size_t)v * (kWL * NW) + j * kWL + col;
+          if (e.dvm[idx] != 0) x = dl[idx];  // the source itself has delta 0
+        }
+        tl[rr][col] = x;
+      }
+      __syncthreads();
+      if (dout) {
+        for (int bb = tid / 64; bb < 64; bb += T / 64) {
+          int b = j * kWL + bb;
+          long long v = v0 + col;
+          if (b < e.nb && v < e.n) dout[(size_t)b * e.n + v] = tl[col][bb];
+        }
+      }
+      if (tid < 64) {
+        for (int c = 0; c < 64; c++) acc += (double)tl[tid][c];
+      }
+      __syncthreads();
+    }
+    if (bc && tid < 64 && v0 + tid < e.n) bc[v0 + tid] += acc;
+  }
+}
+
+}  // namespace gu
+
+// =============================================================================
+// Host side
+// =============================================================================
+using namespace gu;
+
+namespace {
+
+#define GU_HIP(call)                      \
+  do {                                    \
+    hipError_t err_ = (call);             \
+    if (err_ != hipSuccess) return fail(err_); \
+  } while (0)
+
+thread_local char g_last_err[256];
+
+int fail(hipError_t e) {
+  std::snprintf(g_last_err, sizeof(g_last_err), "%s", hipGetErrorString(e));
+  return e == hipErrorOutOfMemory ? GU_ERR_NOMEM : GU_ERR_HIP;
+}
+
+template <class T>
+hipError_t dalloc(T **p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  return hipMalloc((void **)p, count * sizeof(T));
+}
+
+struct Workspace {
+  int NW = 0;
+  int n = 0;
+  bool stack = false;
+  u64 *visit = nullptr, *fr = nullptr, *svm = nullptr, *smask = nullptr, *blm = nullptr;
+  int *q = nullptr, *tag = nullptr, *dvm = nullptr, *off = nullptr, *srcs = nullptr, *btag = nullptr;
+  float *dl = nullptr;
+  long long *ep = nullptr, *tsum = nullptr;
+  unsigned *bwork = nullptr;
+  size_t S_cap = 0;   // entries of q in stack mode
+  size_t bytes = 0;
+  void release() {
+    void *ps[] = {visit, fr, svm, smask, blm, q, tag, dvm, off, srcs, btag, dl, ep, tsum, bwork};
+    for (void *p : ps)
+      if (p) (void)hipFree(p);
+    *this = Workspace();
+  }
+};
+
+}  // namespace
+
+struct gu_graph {
+  int n = 0;
+  long long m = 0;
+  int directed = 0;
+  int *R = nullptr, *C = nullptr, *F = nullptr;
+  int cus = 0;
+  int device = 0;
+  Workspace ws;
+  Ctrl *ctrl = nullptr;       // device
+  Ctrl *ctrl_h = nullptr;     // pinned host mirror
+  int *srcs_h = nullptr;      // pinned staging for batch sources
+  int max_deg = 0;
+  int tuned_variant = -1;     // autotuner cache
+  int tuned_direction_alpha = 0;
+};
+
+namespace {
+
+int pick_grid(const gu_graph *g, int per_cu) { return std::max(1, g->cus * per_cu); }
+
+int ensure_ws(gu_graph *g, int NW, bool stack, size_t want_S, hipStream_t st) {
+  Workspace &w = g->ws;
+  const size_t n = (size_t)g->n;
+  const size_t WL = (size_t)kWL * NW;
+  if (w.NW != NW || (stack && !w.stack)) {
+    w.release();
+    w.NW = NW;
+    w.n = g->n;
+    GU_HIP(dalloc(&w.visit, n * NW));
+    GU_HIP(dalloc(&w.fr, 3 * n * NW));
+    GU_HIP(dalloc(&w.tag, n));
+    GU_HIP(dalloc(&w.dvm, n * WL));
+    GU_HIP(dalloc(&w.svm, n * WL));
+    GU_HIP(dalloc(&w.off, n + 1));
+    GU_HIP(dalloc(&w.tsum, n + 8));
+    GU_HIP(dalloc(&w.srcs, 256));
+    GU_HIP(dalloc(&w.ep, n + 3));
+    w.bytes = n * NW * 8 * 4 + n * 4 * 2 + n * WL * 12 + (n + 8) * 8 + 1024 + (n + 3) * 8;
+    w.S_cap = 3 * n;
+    GU_HIP(dalloc(&w.q, w.S_cap));
+    w.bytes += w.S_cap * 4;
+    if (stack) {
+      w.stack = true;
+      GU_HIP(dalloc(&w.smask, w.S_cap * NW));
+      GU_HIP(dalloc(&w.dl, n * WL));
+      GU_HIP(dalloc(&w.btag, 2 * n));
+      GU_HIP(dalloc(&w.blm, 2 * n * NW));
+      GU_HIP(dalloc(&w.bwork, n + 3));
+      w.bytes += w.S_cap * NW * 8 + n * WL * 4 + 2 * n * 4 + 2 * n * NW * 8 + (n + 3) * 4;
+    }
+  }
+  if (stack && want_S > w.S_cap) {  // grow the stack, keeping its contents
+    size_t cap = std::max(want_S, w.S_cap * 2);
+    int *q2;
+    u64 *m2;
+    GU_HIP(dalloc(&q2, cap));
+    GU_HIP(dalloc(&m2, cap * NW));
+    GU_HIP(hipMemcpyAsync(q2, w.q, w.S_cap * 4, hipMemcpyDeviceToDevice, st));
+    GU_HIP(hipMemcpyAsync(m2, w.smask, w.S_cap * NW * 8, hipMemcpyDeviceToDevice, st));
+    GU_HIP(hipStreamSynchronize(st));
+    (void)hipFree(w.q);
+    (void)hipFree(w.smask);
+    w.bytes += (cap - w.S_cap) * (4 + NW * 8);
+    w.q = q2;
+    w.smask = m2;
+    w.S_cap = cap;
+  }
+  return GU_OK;
+}
+
+enum Job { JOB_BFS = 0, JOB_BC = 1, JOB_DIAM = 2 };
+
+struct BatchOut {
+  int *dist = nullptr;      // caller rows for this batch (pitch n)
+  u64 *sigma = nullptr;
+  float *delta = nullptr;
+  double *bc = nullptr;
+};
+
+struct RunInfo {
+  int last_level = 0;
+  int td = 0, bu = 0;
+  int overflow = 0;
+  u64 teps = 0, reached = 0, dag = 0;
+};
+
+template <int NW>
+Eng<NW> make_eng(gu_graph *g, int nb, bool stack, bool sigma) {
+  Workspace &w = g->ws;
+  Eng<NW> e{};
+  e.R = g->R;
+  e.C = g->C;
+  e.n = g->n;
+  e.nb = nb;
+  e.visit = w.visit;
+  e.fr = w.fr;
+  e.q = w.q;
+  e.smask = w.smask;
+  e.ep = w.ep;
+  e.tag = w.tag;
+  e.dvm = w.dvm;
+  e.svm = w.svm;
+  e.ctrl = g->ctrl;
+  e.off = w.off;
+  e.tsum = w.tsum;
+  e.srcs = w.srcs;
+  for (int j = 0; j < NW; j++) {
+    int lanes = std::min(std::max(nb - j * kWL, 0), kWL);
+    e.all[j] = lanes == 64 ? ~0ull : ((1ull << lanes) - 1);
+  }
+  e.stack = stack ? 1 : 0;
+  e.sigma = sigma ? 1 : 0;
+  return e;
+}
+
+template <int NW, bool SIGMA>
+void launch_level(gu_graph *g, const Eng<NW> &e, int variant, int block, int grid, hipStream_t st) {
+  if (variant == V_EDGE_PARALLEL) {
+    k_fscan_reduce<NW><<<grid, 256, 0, st>>>(e);
+    k_fscan_tiles<NW><<<1, 256, 0, st>>>(e);
+    k_fscan_apply<NW><<<grid, 256, 0, st>>>(e);
+  }
+  switch (variant) {
+    case V_LINEAR: k_expand<NW, V_LINEAR, SIGMA><<<grid, block, 0, st>>>(e); break;
+    case V_EDGE_PARALLEL: k_expand<NW, V_EDGE_PARALLEL, SIGMA><<<grid, block, 0, st>>>(e); break;
+    case V_WARP: k_expand<NW, V_WARP, SIGMA><<<grid, block, 0, st>>>(e); break;
+    case V_SCAN: k_expand<NW, V_SCAN, SIGMA><<<grid, block, 0, st>>>(e); break;
+    case V_CTA_WARP: k_expand<NW, V_CTA_WARP, SIGMA><<<grid, block, 0, st>>>(e); break;
+    default: k_expand<NW, V_SHUFFLE, SIGMA><<<grid, block, 0, st>>>(e); break;
+  }
+  k_finalize<NW, SIGMA><<<grid, 256, 0, st>>>(e);
+  k_control<NW><<<1, 64, 0, st>>>(e);
+}
+
+int default_alpha(const gu_config &c) { return c.alpha > 0 ? c.alpha : 4; }
+int default_beta(const gu_config &c) { return c.beta > 0 ? c.beta : 64; }
+
+// One batch of nb <= 64*NW sources through the level loop.
+template <int NW>
+int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const gu_config &cfg,
+              int variant, const BatchOut &out, RunInfo *info, hipStream_t st) {
+  const bool stack = job == JOB_BC;
+  int rc = ensure_ws(g, NW, stack, 0, st);
+  if (rc) return rc;
+  Workspace &w = g->ws;
+  const size_t n = (size_t)g->n;
+  const int fgrid = pick_grid(g, 4);
+  std::memcpy(g->srcs_h, srcs, sizeof(int) * nb);
+  GU_HIP(hipMemcpyAsync(w.srcs, g->srcs_h, sizeof(int) * nb, hipMemcpyHostToDevice, st));
+  k_fill_u64<<<fgrid, 256, 0, st>>>(w.visit, n * NW, 0ull);
+  k_fill_u64<<<fgrid, 256, 0, st>>>(w.fr, 3 * n * NW, 0ull);
+  k_fill_i32<<<fgrid, 256, 0, st>>>(w.tag, n, -1);
+  Ctrl c0{};
+  c0.direction = cfg.direction;
+  c0.alpha = default_alpha(cfg);
+  c0.beta = default_beta(cfg);
+  c0.mode = cfg.direction == GU_DIR_BOTTOMUP ? 1 : 0;
+  c0.unvis_edges = (u64)g->m;
+  c0.n = g->n;
+  *g->ctrl_h = c0;
+  GU_HIP(hipMemcpyAsync(g->ctrl, g->ctrl_h, sizeof(Ctrl), hipMemcpyHostToDevice, st));
+  Eng<NW> e = make_eng<NW>(g, nb, stack, sigma);
+  k_seed<NW><<<1, 256, 0, st>>>(e);
+  const int block = cfg.wg_size > 0 ? cfg.wg_size : 256;
+  const int grid = pick_grid(g, block >= 1024 ? 2 : (block >= 512 ? 4 : 8));
+  int K = cfg.sync_every > 0 ? cfg.sync_every : 2;
+  // stack mode reserves room for every level of a chunk, so its chunks stay short
+  const int Kmax = cfg.sync_every > 0 ? cfg.sync_every : (stack ? 8 : 64);
+  int levels_launched = 0;
+  while (true) {
+    if (stack) {
+      // every level adds at most n stack entries
+      rc = ensure_ws(g, NW, true, (size_t)g->ctrl_h->S_len + (size_t)(K + 2) * n, st);
+      if (rc) return rc;
+      e = make_eng<NW>(g, nb, stack, sigma);
+    }
+    for (int i = 0; i < K; i++) {
+      if (sigma)
+        launch_level<NW, true>(g, e, variant, block, grid, st);
+      else
+        launch_level<NW, false>(g, e, variant, block, grid, st);
+    }
+    levels_launched += K;
+    GU_HIP(hipGetLastError());
+    GU_HIP(hipMemcpyAsync(g->ctrl_h, g->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
+    GU_HIP(hipStreamSynchronize(st));
+    if (g->ctrl_h->done) break;
+    if (levels_launched > g->n + 2) return GU_ERR_INVALID;  // cannot happen for a valid graph
+    K = std::min(K * 2, Kmax);
+  }
+  const int last = g->ctrl_h->last_level;
+  if (job == JOB_BFS && (out.dist || out.sigma))
+    k_output<NW><<<pick_grid(g, 8), 256, 0, st>>>(e, out.dist, out.sigma);
+  if (job == JOB_BC) {
+    k_fill_i32<<<fgrid, 256, 0, st>>>(w.btag, 2 * n, -1);
+    k_fill_i32<<<1, 256, 0, st>>>((int *)w.bwork, (size_t)last + 2, 0);
+    for (int L = last; L >= 1; L--)
+      k_backward<NW><<<pick_grid(g, 8), 256, 0, st>>>(e, w.dl, w.btag, w.blm, w.bwork, L);
+    k_bc_out<NW><<<pick_grid(g, 8), 256, 0, st>>>(e, w.dl, out.delta, out.bc);
+  }
+  k_teps<NW><<<pick_grid(g, 4), 256, 0, st>>>(e);
+  GU_HIP(hipGetLastError());
+  GU_HIP(hipMemcpyAsync(g->ctrl_h, g->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
+  GU_HIP(hipStreamSynchronize(st));
+  info->last_level = last;
+  info->td = g->ctrl_h->td_levels;
+  info->bu = g->ctrl_h->bu_levels;
+  info->overflow = g->ctrl_h->overflow;
+  info->teps = g->ctrl_h->teps;
+  info->reached = g->ctrl_h->reached;
+  info->dag = g->ctrl_h->dag;
+  return GU_OK;
+}
+
+int resolve_variant(gu_graph *g, const gu_config &cfg) {
+  int v = cfg.variant;
+  if (v >= V_LINEAR && v <= V_SHUFFLE) return v;
+  if (g->tuned_variant >= 0) return g->tuned_variant;
+  // Degree-profile default: heavy-tailed graphs need the merge-path split
+  // of hub adjacency lists; bounded-degree graphs do well with one wave per
+  // frontier vertex.
+  return g->max_deg > 4096 ? V_EDGE_PARALLEL : V_SHUFFLE;
+}
+
+int resolve_width(const gu_config &cfg, int nsrc) {
+  int w = cfg.batch_width;
+  if (w == 64 || w == 128 || w == 256) return w;
+  if (w != 0) return -1;
+  return nsrc <= 64 ? 64 : (nsrc <= 128 ? 128 : 256);
+}
+
+int run_job(gu_graph *g, const int32_t *sources, int32_t nsrc, Job job, int32_t *dist,
+            uint64_t *sigma, float *delta, double *bc, int32_t *diam, const gu_config *cfgp,
+            gu_stats *stats) {
+  if (!g) return GU_ERR_INVALID;
+  gu_config cfg{};
+  if (cfgp) cfg = *cfgp;
+  if (cfg.direction < 0 || cfg.direction > 2) return GU_ERR_INVALID;
+  if (cfg.variant < 0 || cfg.variant > GU_VARIANT_AUTO) return GU_ERR_INVALID;
+  if (cfg.wg_size != 0 && cfg.wg_size != 256 && cfg.wg_size != 512 && cfg.wg_size != 1024)
+    return GU_ERR_INVALID;
+  if (nsrc < 0 || (nsrc > 0 && !sources)) return GU_ERR_INVALID;
+  for (int i = 0; i < nsrc; i++)
+    if (sources[i] < 0 || sources[i] >= g->n) return GU_ERR_RANGE;
+  const int width = resolve_width(cfg, nsrc);
+  if (width < 0) return GU_ERR_INVALID;
+  const int variant = resolve_variant(g, cfg);
+  // Path counts are computed for Brandes and when the caller asks for them.
+  const bool want_sigma = job == JOB_BC || (job == JOB_BFS && sigma != nullptr);
+  hipStream_t st = (hipStream_t)cfg.stream;
+  GU_HIP(hipSetDevice(g->device));
+  hipEvent_t e0, e1;
+  GU_HIP(hipEventCreate(&e0));
+  GU_HIP(hipEventCreate(&e1));
+  GU_HIP(hipEventRecord(e0, st));
+  gu_stats s{};
+  s.variant_used = variant;
+  s.batch_width_used = width;
+  int rc = GU_OK;
+  int dmax = 0;
+  for (int b0 = 0; b0 < nsrc && rc == GU_OK; b0 += width) {
+    int nb = std::min(width, nsrc - b0);
+    BatchOut o;
+    const size_t row = (size_t)b0 * (size_t)g->n;
+    if (dist) o.dist = dist + row;
+    if (sigma) o.sigma = (u64 *)sigma + row;
+    if (delta) o.delta = delta + row;
+    o.bc = bc;
+    RunInfo info;
+    if (width == 64)
+      rc = run_batch<1>(g, sources + b0, nb, job, want_sigma, cfg, variant, o, &info, st);
+    else if (width == 128)
+      rc = run_batch<2>(g, sources + b0, nb, job, want_sigma, cfg, variant, o, &info, st);
+    else
+      rc = run_batch<4>(g, sources + b0, nb, job, want_sigma, cfg, variant, o, &info, st);
+    if (rc) break;
+    s.batches++;
+    s.topdown_levels += info.td;
+    s.bottomup_levels += info.bu;
+    s.sigma_overflow |= info.overflow;
+    s.teps_units += (int64_t)info.teps;
+    s.reached += (int64_t)info.reached;
+    s.dag_edges += (int64_t)info.dag;
+    dmax = std::max(dmax, info.last_level);
+  }
+  GU_HIP(hipEventRecord(e1, st));
+  GU_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  GU_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  s.device_ms = ms;
+  s.max_level = dmax;
+  if (diam) *diam = dmax;
+  if (stats) *stats = s;
+  return rc;
+}
+
+}  // namespace
+
+// =============================================================================
+// C ABI
+// =============================================================================
+extern "C" {
+
+const char *gu_strerror(int s) {
+  switch (s) {
+    case GU_OK: return "ok";
+    case GU_ERR_INVALID: return "invalid argument";
+    case GU_ERR_HIP: return g_last_err[0] ? g_last_err : "HIP runtime error";
+    case GU_ERR_NOMEM: return "out of memory";
+    case GU_ERR_RANGE: return "vertex id out of range";
+    case GU_ERR_PARSE: return "malformed graph file";
+    case GU_ERR_WEIGHTED: return "weighted graphs are not supported";
+    case GU_ERR_IO: return "cannot open file";
+    case GU_ERR_NODEVICE: return "no HIP device";
+    case GU_ERR_CAPACITY: return "output capacity exceeded";
+    default: return "unknown status";
+  }
+}
+
+int gu_device_count(int32_t *count) {
+  if (!count) return GU_ERR_INVALID;
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) {
+    *count = 0;
+    return GU_ERR_NODEVICE;
+  }
+  *count = c;
+  return GU_OK;
+}
+
+int gu_choose_device(int32_t requested, gu_device_props *props) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return GU_ERR_NODEVICE;
+  int dev = requested;
+  if (requested < 0) {
+    double best = -1;
+    for (int i = 0; i < count; i++) {
+      hipDeviceProp_t p;
+      GU_HIP(hipGetDeviceProperties(&p, i));
+      double bw = (p.memoryClockRate * 1000.0) * (p.memoryBusWidth / 8 * 2) / 1.0e9;
+      if (bw > best) {
+        best = bw;
+        dev = i;
+      }
+    }
+  } else if (requested >= count) {
+    return GU_ERR_INVALID;
+  }
+  GU_HIP(hipSetDevice(dev));
+  if (props) {
+    hipDeviceProp_t p;
+    GU_HIP(hipGetDeviceProperties(&p, dev));
+    std::memset(props, 0, sizeof(*props));
+    std::snprintf(props->name, sizeof(props->name), "%s", p.name);
+    std::snprintf(props->arch, sizeof(props->arch), "%s", p.gcnArchName);
+    props->device = dev;
+    props->compute_units = p.multiProcessorCount;
+    props->max_threads_per_block = p.maxThreadsPerBlock;
+    props->wave_size = p.warpSize;
+    props->total_mem = (int64_t)p.totalGlobalMem;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess) props->free_mem = (int64_t)fr;
+    props->mem_bandwidth_gbs = (p.memoryClockRate * 1000.0) * (p.memoryBusWidth / 8 * 2) / 1.0e9;
+    props->l2_bytes = p.l2CacheSize;
+    props->clock_khz = p.clockRate;
+  }
+  return GU_OK;
+}
+
+struct gu_timer_s {
+  hipEvent_t a, b;
+  hipStream_t s;
+};
+
+int gu_timer_start(void *stream, void **timer) {
+  if (!timer) return GU_ERR_INVALID;
+  gu_timer_s *t = new (std::nothrow) gu_timer_s;
+  if (!t) return GU_ERR_NOMEM;
+  t->s = (hipStream_t)stream;
+  GU_HIP(hipEventCreate(&t->a));
+  GU_HIP(hipEventCreate(&t->b));
+  GU_HIP(hipEventRecord(t->a, t->s));
+  *timer = t;
+  return GU_OK;
+}
+
+int gu_timer_stop(void *timer, double *ms) {
+  if (!timer) return GU_ERR_INVALID;
+  gu_timer_s *t = (gu_timer_s *)timer;
+  GU_HIP(hipEventRecord(t->b, t->s));
+  GU_HIP(hipEventSynchronize(t->b));
+  float f = 0;
+  GU_HIP(hipEventElapsedTime(&f, t->a, t->b));
+  if (ms) *ms = f;
+  (void)hipEventDestroy(t->a);
+  (void)hipEventDestroy(t->b);
+  delete t;
+  return GU_OK;
+}
+
+int gu_malloc(void **p, size_t bytes) {
+  if (!p) return GU_ERR_INVALID;
+  GU_HIP(hipMalloc(p, bytes ? bytes : 1));
+  return GU_OK;
+}
+int gu_free(void *p) {
+  GU_HIP(hipFree(p));
+  return GU_OK;
+}
+int gu_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+  GU_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return GU_OK;
+}
+int gu_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+  GU_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return GU_OK;
+}
+int gu_memset_d(void *dst, int value, size_t bytes) {
+  GU_HIP(hipMemset(dst, value, bytes));
+  return GU_OK;
+}
+int gu_synchronize(void) {
+  GU_HIP(hipDeviceSynchronize());
+  return GU_OK;
+}
+
+int gu_graph_create(const int32_t *R, const int32_t *C, const int32_t *F, int32_t n, int64_t m,
+                    int32_t directed, gu_graph_t *out) {
+  if (!out || !R || n < 0 || m < 0 || m >= INT32_MAX || (m > 0 && !C)) return GU_ERR_INVALID;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return GU_ERR_NODEVICE;
+  // host-side checks of the CSR shape (the reference trusts its parser)
+  if (R[0] != 0 || (int64_t)R[n] != m) return GU_ERR_INVALID;
+  int maxd = 0;
+  for (int32_t v = 0; v < n; v++) {
+    if (R[v + 1] < R[v]) return GU_ERR_INVALID;
+    maxd = std::max(maxd, R[v + 1] - R[v]);
+  }
+  for (int64_t k = 0; k < m; k++)
+    if (C[k] < 0 || C[k] >= n) return GU_ERR_RANGE;
+  gu_graph *g = new (std::nothrow) gu_graph;
+  if (!g) return GU_ERR_NOMEM;
+  g->n = n;
+  g->m = m;
+  g->directed = directed;
+  g->max_deg = maxd;
+  int rc = GU_OK;
+  do {
+    if (hipGetDevice(&g->device) != hipSuccess) { rc = GU_ERR_HIP; break; }
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, g->device) != hipSuccess) { rc = GU_ERR_HIP; break; }
+    g->cus = p.multiProcessorCount;
+    if (dalloc(&g->R, (size_t)n + 1) != hipSuccess || dalloc(&g->C, (size_t)m) != hipSuccess ||
+        dalloc(&g->F, (size_t)m) != hipSuccess || dalloc(&g->ctrl, 1) != hipSuccess) {
+      rc = GU_ERR_NOMEM;
+      break;
+    }
+    if (hipHostMalloc((void **)&g->ctrl_h, sizeof(Ctrl), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&g->srcs_h, sizeof(int) * 256, hipHostMallocDefault) != hipSuccess) {
+      rc = GU_ERR_NOMEM;
+      break;
+    }
+    std::memset(g->ctrl_h, 0, sizeof(Ctrl));
+    if (hipMemcpy(g->R, R, sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice) != hipSuccess ||
+        (m > 0 && hipMemcpy(g->C, C, sizeof(int) * (size_t)m, hipMemcpyHostToDevice) != hipSuccess)) {
+      rc = GU_ERR_HIP;
+      break;
+    }
+    if (F) {
+      if (m > 0 && hipMemcpy(g->F, F, sizeof(int) * (size_t)m, hipMemcpyHostToDevice) != hipSuccess) {
+        rc = GU_ERR_HIP;
+        break;
+      }
+    } else if (n > 0) {
+      k_build_F<<<std::max(1, g->cus * 4), 256>>>(g->R, n, g->F);
+      if (hipDeviceSynchronize() != hipSuccess) { rc = GU_ERR_HIP; break; }
+    }
+  } while (0);
+  if (rc != GU_OK) {
+    gu_graph_destroy(g);
+    return rc;
+  }
+  *out = g;
+  return GU_OK;
+}
+
+int gu_graph_destroy(gu_graph_t g) {
+  if (!g) return GU_ERR_INVALID;
+  g->ws.release();
+  if (g->R) (void)hipFree(g->R);
+  if (g->C) (void)hipFree(g->C);
+  if (g->F) (void)hipFree(g->F);
+  if (g->ctrl) (void)hipFree(g->ctrl);
+  if (g->ctrl_h) (void)hipHostFree(g->ctrl_h);
+  if (g->srcs_h) (void)hipHostFree(g->srcs_h);
+  delete g;
+  return GU_OK;
+}
+
+int gu_graph_arrays(gu_graph_t g, const int32_t **R, const int32_t **C, const int32_t **F,
+                    int32_t *n, int64_t *m) {
+  if (!g) return GU_ERR_INVALID;
+  if (R) *R = g->R;
+  if (C) *C = g->C;
+  if (F) *F = g->F;
+  if (n) *n = g->n;
+  if (m) *m = g->m;
+  return GU_OK;
+}
+
+int gu_graph_device_bytes(gu_graph_t g, int64_t *bytes) {
+  if (!g || !bytes) return GU_ERR_INVALID;
+  *bytes = (int64_t)(((size_t)g->n + 1) * 4 + (size_t)g->m * 8 + g->ws.bytes);
+  return GU_OK;
+}
+
+int gu_multi_search(gu_graph_t g, const int32_t *sources, int32_t nsrc, int32_t *dist,
+                    uint64_t *sigma, const gu_config *cfg, gu_stats *stats) {
+  return run_job(g, sources, nsrc, JOB_BFS, dist, sigma, nullptr, nullptr, nullptr, cfg, stats);
+}
+
+int gu_betweenness(gu_graph_t g, const int32_t *sources, int32_t nsrc, float *delta, double *bc,
+                   const gu_config *cfg, gu_stats *stats) {
+  return run_job(g, sources, nsrc, JOB_BC, nullptr, nullptr, delta, bc, nullptr, cfg, stats);
+}
+
+int gu_diameter(gu_graph_t g, const int32_t *sources, int32_t nsrc, int32_t *diameter,
+                const gu_config *cfg, gu_stats *stats) {
+  if (!diameter) return GU_ERR_INVALID;
+  *diameter = 0;
+  return run_job(g, sources, nsrc, JOB_DIAM, nullptr, nullptr, nullptr, nullptr, diameter, cfg, stats);
+}
+
+int gu_lbs(const int32_t *counts, int64_t S, int32_t *scan, int32_t *result, int64_t result_cap,
+           int64_t *W_out, const gu_config *cfgp, gu_stats *stats) {
+  if (S < 0 || (S > 0 && (!counts || !scan)) || S >= INT32_MAX) return GU_ERR_INVALID;
+  gu_config cfg{};
+  if (cfgp) cfg = *cfgp;
+  hipStream_t st = (hipStream_t)cfg.stream;
+  if (S == 0) {
+    if (W_out) *W_out = 0;
+    return GU_OK;
+  }
+  int dev = 0, cus = 0;
+  GU_HIP(hipGetDevice(&dev));
+  GU_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const int T = 256;
+  const long long TS = (long long)T * kScanIPT;
+  const long long nt = (S + TS - 1) / TS;
+  long long *tsum = nullptr;
+  GU_HIP(hipMalloc((void **)&tsum, sizeof(long long) * (size_t)(nt + 2)));
+  long long *total_d = tsum + nt + 1;
+  hipEvent_t e0, e1;
+  GU_HIP(hipEventCreate(&e0));
+  GU_HIP(hipEventCreate(&e1));
+  GU_HIP(hipEventRecord(e0, st));
+  const int grid = std::max(1, cus * 8);
+  k_cscan_reduce<<<grid, T, 0, st>>>(counts, S, tsum);
+  k_cscan_tiles<<<1, T, 0, st>>>(tsum, nt, total_d);
+  k_cscan_apply<<<grid, T, 0, st>>>(counts, S, tsum, scan, total_d);
+  if (result) k_lbs_expand<<<grid, 256, 0, st>>>(scan, S, total_d, result_cap, result);
+  GU_HIP(hipGetLastError());
+  GU_HIP(hipEventRecord(e1, st));
+  long long W = 0;
+  GU_HIP(hipMemcpyAsync(&W, total_d, sizeof(long long), hipMemcpyDeviceToHost, st));
+  GU_HIP(hipStreamSynchronize(st));
+  int rc = GU_OK;
+  GU_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  GU_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(tsum);
+  if (W >= INT_MAX || (W > 0 && (!result || W > result_cap))) rc = GU_ERR_CAPACITY;
+  if (W_out) *W_out = W;
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->device_ms = ms;
+  }
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,207 @@
+/*
+ * graph_utils_amd.h -- C ABI of the MI355X (gfx950) multi-source search path.
+ *
+ * These entry points replace graph-utils' host-side operator surface for the
+ * multi-search family; each declaration cites the reference interface it
+ * stands in for.  The C++ signatures of the reference (device_graph,
+ * multi_search_*_setup, count_shortest_paths_setup,
+ * betweenness_centrality_setup, diameter_sampling_setup, configure_grid,
+ * transfer_result) are provided on top of this ABI by
+ * graph-utils_amd/host/graph_utils.hpp.
+ *
+ * Conventions
+ *  - Every call returns int status: GU_OK (0) or a negative GU_ERR_*; nothing
+ *    exits the process (the reference's checkCudaErrors calls exit(),
+ *    util_device.cuh:10-22).
+ *  - Pointers named *_dev are device (HBM) pointers; all others are host.
+ *    Host buffers are caller-owned and only borrowed for the call.
+ *  - A gu_graph_t owns its device copy of the graph and a reusable search
+ *    workspace; use one handle from one host thread at a time.
+ *  - Result rows are source-major: row i of a [nsrc x n] output belongs to
+ *    sources[i] (pitch n elements), as the reference's pitched rows.
+ */
+#ifndef GRAPH_UTILS_AMD_H
+#define GRAPH_UTILS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GU_OK 0
+#define GU_ERR_INVALID (-1)     /* bad argument (null pointer, size, option) */
+#define GU_ERR_HIP (-2)         /* HIP runtime error */
+#define GU_ERR_NOMEM (-3)       /* device or host allocation failed */
+#define GU_ERR_RANGE (-4)       /* vertex id / source out of range */
+#define GU_ERR_PARSE (-5)       /* malformed graph text */
+#define GU_ERR_WEIGHTED (-6)    /* weighted METIS header (parse.cpp:164-173) */
+#define GU_ERR_IO (-7)          /* file cannot be opened */
+#define GU_ERR_NODEVICE (-8)    /* no HIP device */
+#define GU_ERR_CAPACITY (-9)    /* output buffer too small */
+
+/* Expansion strategies for top-down levels.  Each mirrors one reference
+ * kernel; all produce identical distances and path counts. */
+#define GU_VARIANT_LINEAR 0        /* linear_with_atomics.cu:42-159 */
+#define GU_VARIANT_EDGE_PARALLEL 1 /* edge_parallel.cu:37-92 (merge-path LBS) */
+#define GU_VARIANT_WARP 2          /* warp_based.cu:47-190 */
+#define GU_VARIANT_SCAN 3          /* scan_based.cu:43-194 */
+#define GU_VARIANT_CTA_WARP 4      /* CTA_warp.cu:42-289 */
+#define GU_VARIANT_SHUFFLE 5       /* shuffle_based.cuh:81-286 */
+#define GU_VARIANT_AUTO 6          /* autotuner picks per graph */
+
+#define GU_DIR_AUTO 0      /* switch top-down / bottom-up per level */
+#define GU_DIR_TOPDOWN 1   /* frontier expansion only (the reference's form) */
+#define GU_DIR_BOTTOMUP 2  /* pull from the frontier only */
+
+typedef struct gu_graph *gu_graph_t;
+
+typedef struct {
+  int32_t variant;      /* GU_VARIANT_* (default AUTO) */
+  int32_t direction;    /* GU_DIR_* (default AUTO) */
+  int32_t batch_width;  /* sources per launch: 64, 128 or 256 (0 = auto) */
+  int32_t wg_size;      /* threads per workgroup: 256, 512, 1024 (0 = auto) */
+  int32_t alpha;        /* top-down -> bottom-up when frontier edges exceed
+                           unvisited edges / alpha (0 = default) */
+  int32_t beta;         /* bottom-up -> top-down when the frontier has fewer
+                           than n / beta vertices (0 = default) */
+  int32_t sync_every;   /* levels launched between host checks (0 = adaptive) */
+  int32_t reserved;
+  void *stream;         /* hipStream_t; NULL = the default stream */
+} gu_config;
+
+typedef struct {
+  double device_ms;        /* device time of the call (HIP events) */
+  int64_t teps_units;      /* sum over sources of edges scanned from reached
+                              vertices (GTEPS numerator) */
+  int64_t reached;         /* sum over sources of reached vertices */
+  int64_t dag_edges;       /* sum over sources of shortest-path DAG edges
+                              (entries v->w with d[w] = d[v]+1); counted
+                              when path counts are computed */
+  int32_t max_level;       /* deepest BFS level over all sources */
+  int32_t batches;         /* launches of the batched engine */
+  int32_t topdown_levels;  /* levels run top-down (summed over batches) */
+  int32_t bottomup_levels; /* levels run bottom-up */
+  int32_t sigma_overflow;  /* 1 if any path count wrapped modulo 2^64 */
+  int32_t variant_used;    /* GU_VARIANT_* actually run */
+  int32_t batch_width_used;
+  int32_t reserved;
+} gu_stats;
+
+typedef struct {
+  char name[256];
+  char arch[64];           /* e.g. gfx950 */
+  int32_t device;
+  int32_t compute_units;   /* the reference's multiProcessorCount */
+  int32_t max_threads_per_block;
+  int32_t wave_size;
+  int64_t total_mem;
+  int64_t free_mem;
+  double mem_bandwidth_gbs; /* bus width x memory clock x 2 */
+  int32_t l2_bytes;
+  int32_t clock_khz;
+} gu_device_props;
+
+/* ---- util_device ---------------------------------------------------- */
+/* choose_device (util_device.cu:23-76): requested < 0 picks the device with
+ * the highest memory bandwidth; fills *props for the chosen device. */
+int gu_choose_device(int32_t requested, gu_device_props *props);
+int gu_device_count(int32_t *count);
+/* start_clock / end_clock (util_device.cu:4-21): HIP-event timer in ms on
+ * the given stream. */
+int gu_timer_start(void *stream, void **timer);
+int gu_timer_stop(void *timer, double *ms);
+/* Memory helpers so host code needs no other runtime. */
+int gu_malloc(void **p_dev, size_t bytes);
+int gu_free(void *p_dev);
+int gu_memcpy_h2d(void *dst_dev, const void *src, size_t bytes);
+int gu_memcpy_d2h(void *dst, const void *src_dev, size_t bytes);
+int gu_memset_d(void *dst_dev, int value, size_t bytes);
+int gu_synchronize(void);
+const char *gu_strerror(int status);
+
+/* ---- device_graph (device_graph.h:7-22, device_graph.cpp:3-14) ------- */
+/* Copies R[n+1], C[m] and (optional) F[m] to HBM.  F may be NULL: it is then
+ * rebuilt on the device from R.  Validates R monotone and C in range. */
+int gu_graph_create(const int32_t *R, const int32_t *C, const int32_t *F,
+                    int32_t n, int64_t m, int32_t directed, gu_graph_t *out);
+int gu_graph_destroy(gu_graph_t g);
+/* Device pointers and sizes of a resident graph. */
+int gu_graph_arrays(gu_graph_t g, const int32_t **R_dev, const int32_t **C_dev,
+                    const int32_t **F_dev, int32_t *n, int64_t *m);
+/* Bytes of HBM the handle holds (graph + workspace). */
+int gu_graph_device_bytes(gu_graph_t g, int64_t *bytes);
+
+/* ---- multi_search ---------------------------------------------------- */
+/* Multi-source BFS.  Replaces multi_search_{linear_atomics,edge_parallel,
+ * warp_based,scan_based,CTA_warp_based,shuffle_based}_setup
+ * (linear_with_atomics.cuh:14, edge_parallel.cuh:13, warp_based.cuh:15,
+ * scan_based.cuh:15, CTA_warp.cuh:15, shuffle_based.cuh:16) and
+ * count_shortest_paths_setup (count_shortest_paths.cuh:13).
+ * dist_dev: [nsrc x n] int32 (INT_MAX = unreachable), may be NULL.
+ * sigma_dev: [nsrc x n] uint64 path counts mod 2^64, may be NULL. */
+int gu_multi_search(gu_graph_t g, const int32_t *sources, int32_t nsrc,
+                    int32_t *dist_dev, uint64_t *sigma_dev,
+                    const gu_config *cfg, gu_stats *stats);
+
+/* Brandes betweenness over the given sources.  Replaces
+ * betweenness_centrality_setup (betweenness_centrality.cuh:14).
+ * delta_dev: [nsrc x n] float dependency rows (the reference's delta_h),
+ * may be NULL.  bc_dev: [n] double, BC(v) = sum_s delta_s(v), accumulated
+ * into (caller zeroes it first), may be NULL. */
+int gu_betweenness(gu_graph_t g, const int32_t *sources, int32_t nsrc,
+                   float *delta_dev, double *bc_dev, const gu_config *cfg,
+                   gu_stats *stats);
+
+/* Largest finite distance from any of the sources.  Replaces
+ * diameter_sampling_setup (diameter_sampling.cuh:13). */
+int gu_diameter(gu_graph_t g, const int32_t *sources, int32_t nsrc,
+                int32_t *diameter, const gu_config *cfg, gu_stats *stats);
+
+/* ---- load_balanced_search (load_balanced_search.cuh:10-13) ----------- */
+/* Device-wide form of extract_edges_block: scan_dev[i] = exclusive prefix of
+ * counts_dev; result_dev[k] = upper_bound(scan, k) - 1 for k < W, the last
+ * segment whose offset is <= k.  *W_out receives the total W.
+ * Fails with GU_ERR_CAPACITY when W > result_cap (then nothing is written
+ * to result_dev).  Counts must be >= 0 and W < 2^31. */
+int gu_lbs(const int32_t *counts_dev, int64_t S, int32_t *scan_dev,
+           int32_t *result_dev, int64_t result_cap, int64_t *W_out,
+           const gu_config *cfg, gu_stats *stats);
+
+/* ---- host graph I/O and generators (parse.h:15-40) -------------------- */
+/* Host arrays returned by these calls are released with gu_host_free. */
+/* parse_metis (parse.cpp:115-208), including its tokenising behaviour:
+ * the graph is undirected and m = 2 x header edge count. */
+int gu_parse_metis(const char *path, int32_t **R, int32_t **C, int32_t **F,
+                   int32_t *n, int64_t *m);
+/* Writes a METIS file with single spaces and '\n' (1-based ids). */
+int gu_write_metis(const char *path, const int32_t *R, const int32_t *C,
+                   int32_t n, int64_t m);
+/* Uniform simple undirected G(n, m_undirected): distinct pairs u<v drawn
+ * without replacement, seeded. */
+int gu_gen_er(int32_t n, int64_t m_undirected, uint64_t seed, int32_t **R,
+              int32_t **C, int32_t **F, int32_t *n_out, int64_t *m_out);
+/* R-MAT(scale, edge_factor, a, b, c), symmetrised, self loops and
+ * duplicates removed, no label permutation, seeded. */
+int gu_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b,
+                double c, uint64_t seed, int32_t **R, int32_t **C,
+                int32_t **F, int32_t *n_out, int64_t *m_out);
+/* side x side 4-neighbour grid, each edge deleted with probability p_delete,
+ * then (if lcc_only) the largest component relabelled in increasing old id. */
+int gu_gen_grid(int32_t side, double p_delete, uint64_t seed, int32_t lcc_only,
+                int32_t **R, int32_t **C, int32_t **F, int32_t *n_out,
+                int64_t *m_out);
+/* Zipf(s) counts capped at cap, drawn until the sum reaches W, the last one
+ * clipped so the sum is exactly W. */
+int gu_gen_zipf_counts(double s, int32_t cap, int64_t W, uint64_t seed,
+                       int32_t **counts, int64_t *S);
+/* Seeded sample of k vertices with out-degree > 0 (with replacement). */
+int gu_sample_sources(const int32_t *R, int32_t n, int32_t k, uint64_t seed,
+                      int32_t *sources);
+void gu_host_free(void *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

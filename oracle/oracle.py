@@ -1,0 +1,174 @@
+"""ctypes wrapper of oracle/liboracle.so -- TEST INFRASTRUCTURE ONLY.
+
+The CPU restatement of the reference's sequential checkers (see oracle.h for
+the file:line each function follows).  Only tests/, __graft_entry__.smoke()
+and bench.py's cpu_baseline leg load this module; the product never does.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_PARSE = os.path.join(HERE, "_ref", "libref_parse.so")
+INT_MAX = 2147483647
+
+_lib = None
+_P = ctypes.c_void_p
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise ImportError("oracle/liboracle.so is not built (make -C oracle)")
+        L = ctypes.CDLL(LIB)
+        i32, i64 = ctypes.c_int32, ctypes.c_int64
+        L.or_bfs.argtypes = [_P, _P, i32, i32, _P]
+        L.or_sigma.argtypes = [_P, _P, i32, i32, _P, _P]
+        L.or_brandes_f32.argtypes = [_P, _P, i32, i32, _P, _P, _P]
+        L.or_brandes_f64.argtypes = [_P, _P, i32, i32, _P, _P, _P]
+        L.or_bc_f64.argtypes = [_P, _P, i32, _P, i32, _P]
+        L.or_eccentricity.argtypes = [_P, _P, i32, i32]
+        L.or_eccentricity.restype = i32
+        L.or_exclusive_scan.argtypes = [_P, i64, _P]
+        L.or_exclusive_scan.restype = i64
+        L.or_lbs.argtypes = [i64, _P, i64, _P]
+        L.or_rows_for_range.argtypes = [i32, i32, i32, _P]
+        L.or_rows_for_range.restype = i32
+        L.or_teps_units.argtypes = [_P, i32, _P]
+        L.or_teps_units.restype = i64
+        L.or_sigma_batch_timed.argtypes = [_P, _P, i32, _P, i32, i32, ctypes.POINTER(ctypes.c_double)]
+        L.or_sigma_batch_timed.restype = i64
+        L.or_lbs_timed.argtypes = [_P, i64, _P, _P, i64]
+        L.or_lbs_timed.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _csr(R, C):
+    return np.ascontiguousarray(R, dtype=np.int32), np.ascontiguousarray(C, dtype=np.int32)
+
+
+def bfs(R, C, src):
+    R, C = _csr(R, C)
+    n = len(R) - 1
+    d = np.empty(n, dtype=np.int32)
+    lib().or_bfs(_p(R), _p(C), n, int(src), _p(d))
+    return d
+
+
+def sigma(R, C, src):
+    R, C = _csr(R, C)
+    n = len(R) - 1
+    d = np.empty(n, dtype=np.int32)
+    s = np.empty(n, dtype=np.uint64)
+    lib().or_sigma(_p(R), _p(C), n, int(src), _p(d), _p(s))
+    return d, s
+
+
+def brandes_f32(R, C, src):
+    R, C = _csr(R, C)
+    n = len(R) - 1
+    delta = np.empty(n, dtype=np.float32)
+    lib().or_brandes_f32(_p(R), _p(C), n, int(src), None, None, _p(delta))
+    return delta
+
+
+def brandes_f64(R, C, src):
+    R, C = _csr(R, C)
+    n = len(R) - 1
+    delta = np.empty(n, dtype=np.float64)
+    lib().or_brandes_f64(_p(R), _p(C), n, int(src), None, None, _p(delta))
+    return delta
+
+
+def bc_f64(R, C, sources):
+    R, C = _csr(R, C)
+    n = len(R) - 1
+    src = np.ascontiguousarray(sources, dtype=np.int32)
+    bc = np.empty(n, dtype=np.float64)
+    lib().or_bc_f64(_p(R), _p(C), n, _p(src), len(src), _p(bc))
+    return bc
+
+
+def eccentricity(R, C, src):
+    R, C = _csr(R, C)
+    return int(lib().or_eccentricity(_p(R), _p(C), len(R) - 1, int(src)))
+
+
+def exclusive_scan(counts):
+    c = np.ascontiguousarray(counts, dtype=np.int32)
+    out = np.empty(len(c), dtype=np.int32)
+    W = lib().or_exclusive_scan(_p(c), len(c), _p(out))
+    return out, int(W)
+
+
+def lbs(W, scan):
+    s = np.ascontiguousarray(scan, dtype=np.int32)
+    out = np.empty(int(W), dtype=np.int32)
+    lib().or_lbs(int(W), _p(s), len(s), _p(out))
+    return out
+
+
+def rows_for_range(start, end, G):
+    out = np.zeros(max(min(end - start, G), 1), dtype=np.int32)
+    r = lib().or_rows_for_range(start, end, G, _p(out))
+    return out[:r]
+
+
+def teps_units(R, dist):
+    R = np.ascontiguousarray(R, dtype=np.int32)
+    d = np.ascontiguousarray(dist, dtype=np.int32)
+    return int(lib().or_teps_units(_p(R), len(R) - 1, _p(d)))
+
+
+def sigma_batch_timed(R, C, sources, nthreads=1):
+    R, C = _csr(R, C)
+    src = np.ascontiguousarray(sources, dtype=np.int32)
+    secs = ctypes.c_double()
+    units = lib().or_sigma_batch_timed(_p(R), _p(C), len(R) - 1, _p(src), len(src),
+                                       int(nthreads), ctypes.byref(secs))
+    return int(units), secs.value
+
+
+def lbs_timed(counts):
+    c = np.ascontiguousarray(counts, dtype=np.int32)
+    W = int(c.astype(np.int64).sum())
+    scan = np.empty(len(c), dtype=np.int32)
+    res = np.empty(W, dtype=np.int32)
+    secs = lib().or_lbs_timed(_p(c), len(c), _p(scan), _p(res), W)
+    return scan, res, secs
+
+
+def ref_parse_available():
+    return os.path.exists(REF_PARSE)
+
+
+def ref_parse_metis(path):
+    """The reference's own parse_metis (parse.cpp:115-208), compiled from the
+    reference checkout into oracle/_ref.  Returns (R, C, F, n, m, directed)."""
+    L = ctypes.CDLL(REF_PARSE)
+    I32P = ctypes.POINTER(ctypes.c_int32)
+    L.ref_parse_metis.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32),
+                                  ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(I32P),
+                                  ctypes.POINTER(I32P), ctypes.POINTER(I32P),
+                                  ctypes.POINTER(ctypes.c_int32)]
+    L.ref_free.argtypes = [ctypes.c_void_p]
+    n, m, dr = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int32()
+    R, C, F = I32P(), I32P(), I32P()
+    L.ref_parse_metis(os.fsencode(path), ctypes.byref(n), ctypes.byref(m), ctypes.byref(R),
+                      ctypes.byref(C), ctypes.byref(F), ctypes.byref(dr))
+
+    def take(p, k):
+        a = np.ctypeslib.as_array(p, shape=(k,)).copy() if k > 0 else np.zeros(0, np.int32)
+        L.ref_free(ctypes.cast(p, ctypes.c_void_p))
+        return a
+
+    return take(R, n.value + 1), take(C, m.value), take(F, m.value), n.value, m.value, bool(dr.value)

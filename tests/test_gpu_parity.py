@@ -1,0 +1,239 @@
+"""GPU parity: the HIP path against the CPU oracle, through the C ABI.
+
+Distances, path counts and LBS outputs must match bit for bit; dependencies
+and BC must be within 1e-5 relative of the FP64 Brandes oracle (BC exactly 0
+where the oracle's is 0).
+"""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from conftest import FIXTURES, announce, graph_path
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = ["linear", "edge_parallel", "warp", "scan", "cta_warp", "shuffle"]
+DIRS = {"auto": 0, "topdown": 1, "bottomup": 2}
+BC_RTOL = 1e-5
+
+
+def _er_graph(gu):
+    hg = gu.HostGraph.er(198, 2742, seed=0)
+    with tempfile.TemporaryDirectory() as t:
+        p = os.path.join(t, "er198.graph")
+        hg.write_metis(p)
+        return gu.HostGraph.parse_metis(p)
+
+
+def _load(gu, name):
+    if name == "er198":
+        return _er_graph(gu)
+    return gu.HostGraph.parse_metis(graph_path(name))
+
+
+def _oracle_rows(orc, hg, sources):
+    d = np.empty((len(sources), hg.n), dtype=np.int32)
+    s = np.empty((len(sources), hg.n), dtype=np.uint64)
+    for i, src in enumerate(sources):
+        d[i], s[i] = orc.sigma(hg.R, hg.C, int(src))
+    return d, s
+
+
+def _search_and_compare(gu, orc, hg, sources, variant, direction, width, label, ref=None):
+    announce("%s variant=%s dir=%s width=%d nsrc=%d n=%d" % (label, variant, direction, width,
+                                                             len(sources), hg.n))
+    cfg = gu.make_config(variant, DIRS[direction], width)
+    with gu.DeviceGraph(hg) as dg:
+        d, s, st = dg.multi_search(sources, dist=True, sigma=True, config=cfg)
+    od, os_ = ref if ref is not None else _oracle_rows(orc, hg, sources)
+    bad = np.argwhere(d != od)
+    assert bad.size == 0, "distance mismatch at (row, v) = %s" % (bad[:5].tolist(),)
+    bad = np.argwhere(s != os_)
+    assert bad.size == 0, "path count mismatch at (row, v) = %s" % (bad[:5].tolist(),)
+    return st
+
+
+@pytest.mark.parametrize("name", FIXTURES + ["er198"])
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("direction", ["auto", "topdown", "bottomup"])
+def test_fixture_all_sources(gu, orc, gpu, name, variant, direction):
+    hg = _load(gu, name)
+    sources = np.arange(hg.n, dtype=np.int32)
+    _search_and_compare(gu, orc, hg, sources, variant, direction, 64, name)
+
+
+@pytest.mark.parametrize("width", [128, 256])
+@pytest.mark.parametrize("variant", ["shuffle", "edge_parallel", "scan"])
+def test_er_wide_batches(gu, orc, gpu, width, variant):
+    hg = _er_graph(gu)
+    sources = np.arange(hg.n, dtype=np.int32)
+    _search_and_compare(gu, orc, hg, sources, variant, "auto", width, "er198")
+
+
+@pytest.fixture(scope="module")
+def rmat14(gu, orc):
+    hg = gu.HostGraph.rmat(14, 16, seed=1)
+    src = hg.sample_sources(64, seed=2)
+    return hg, src, _oracle_rows(orc, hg, src)
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("direction", ["auto", "topdown"])
+def test_rmat14_64_sources(gu, orc, gpu, rmat14, variant, direction):
+    hg, src, ref = rmat14
+    _search_and_compare(gu, orc, hg, src, variant, direction, 64, "rmat14", ref=ref)
+
+
+def test_rmat14_duplicate_and_partial_batch(gu, orc, gpu, rmat14):
+    hg, src, _ = rmat14
+    sources = np.concatenate([src[:10], src[:10], src[3:40]]).astype(np.int32)  # 57 lanes, repeats
+    _search_and_compare(gu, orc, hg, sources, "shuffle", "auto", 64, "rmat14-dups")
+
+
+def test_grid_256_sources_and_diameter(gu, orc, gpu):
+    hg = gu.HostGraph.grid(48, 0.10, seed=3)
+    src = hg.sample_sources(256, seed=4)
+    _search_and_compare(gu, orc, hg, src, "shuffle", "auto", 256, "grid48")
+    _search_and_compare(gu, orc, hg, src[:64], "warp", "topdown", 64, "grid48")
+    announce("grid48 diameter over 64 sources")
+    with gu.DeviceGraph(hg) as dg:
+        diam, st = dg.diameter(src[:64])
+    assert diam == max(orc.eccentricity(hg.R, hg.C, int(s)) for s in src[:64])
+
+
+@pytest.mark.parametrize("name", FIXTURES + ["er198"])
+def test_diameter_fixtures(gu, orc, gpu, name):
+    hg = _load(gu, name)
+    announce("diameter %s" % name)
+    with gu.DeviceGraph(hg) as dg:
+        diam, _ = dg.diameter(np.arange(hg.n, dtype=np.int32))
+    assert diam == max(orc.eccentricity(hg.R, hg.C, s) for s in range(hg.n))
+
+
+def _check_bc(gu, orc, hg, sources, width, label, variant="auto"):
+    announce("bc %s width=%d nsrc=%d" % (label, width, len(sources)))
+    cfg = gu.make_config(variant, 0, width)
+    with gu.DeviceGraph(hg) as dg:
+        delta, bc, st = dg.betweenness(sources, delta=True, bc=True, config=cfg)
+    ref_bc = np.zeros(hg.n)
+    for i, s in enumerate(sources):
+        ref = orc.brandes_f64(hg.R, hg.C, int(s))
+        ref_bc += ref
+        nz = ref != 0
+        assert np.all(delta[i][~nz] == 0), "delta should be 0 where FP64 delta is 0 (src %d)" % s
+        rel = np.abs(delta[i][nz] - ref[nz]) / np.abs(ref[nz])
+        assert rel.max(initial=0) <= BC_RTOL, "delta rel err %.3g (src %d)" % (rel.max(), s)
+    nz = ref_bc != 0
+    assert np.all(bc[~nz] == 0)
+    rel = np.abs(bc[nz] - ref_bc[nz]) / np.abs(ref_bc[nz])
+    assert rel.max(initial=0) <= BC_RTOL, "BC rel err %.3g" % rel.max()
+    return st
+
+
+@pytest.mark.parametrize("name", FIXTURES + ["er198"])
+def test_bc_fixtures(gu, orc, gpu, name):
+    hg = _load(gu, name)
+    _check_bc(gu, orc, hg, np.arange(hg.n, dtype=np.int32), 64, name)
+
+
+def test_bc_closed_forms(gu, gpu):
+    # unhalved BC over all sources: star center (n-1)(n-2), path 2i(n-1-i)
+    with gu.DeviceGraph(_load(gu, "test2")) as dg:
+        _, bc, _ = dg.betweenness(np.arange(6, dtype=np.int32), delta=False)
+    assert bc[0] == 20 and np.all(bc[1:] == 0)
+    with gu.DeviceGraph(_load(gu, "test5")) as dg:
+        _, bc, _ = dg.betweenness(np.arange(7, dtype=np.int32), delta=False)
+    assert np.allclose(bc, [0, 10, 16, 18, 16, 10, 0], rtol=0, atol=1e-9)
+
+
+@pytest.mark.parametrize("width", [64, 128, 256])
+def test_bc_rmat12(gu, orc, gpu, width):
+    hg = gu.HostGraph.rmat(12, 16, seed=5)
+    src = hg.sample_sources(200, seed=6)
+    _check_bc(gu, orc, hg, src, width, "rmat12")
+
+
+def test_bc_grid_deep(gu, orc, gpu):
+    hg = gu.HostGraph.grid(24, 0.10, seed=7)
+    src = hg.sample_sources(64, seed=8)
+    _check_bc(gu, orc, hg, src, 64, "grid24")
+
+
+LBS_CASES = {
+    "small": [0, 3, 0, 5, 1, 0, 2],
+    "zeros_ends": [0, 0, 4, 0, 0, 0, 7, 1, 0, 0],
+    "single": [17],
+    "all_zero": [0, 0, 0],
+    "one_item_each": [1] * 5000,
+}
+
+
+@pytest.mark.parametrize("case", sorted(LBS_CASES))
+def test_lbs_cases(gu, orc, gpu, case):
+    announce("lbs %s" % case)
+    c = np.array(LBS_CASES[case], dtype=np.int32)
+    scan, res, W = gu.lbs(c)
+    oscan, oW = orc.exclusive_scan(c)
+    assert W == oW and np.array_equal(scan, oscan)
+    assert np.array_equal(res, orc.lbs(oW, oscan))
+    assert np.array_equal(res, np.searchsorted(oscan, np.arange(W), side="right") - 1)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_lbs_random_with_zero_segments(gu, orc, gpu, seed):
+    rng = np.random.default_rng(seed)
+    c = rng.integers(0, 300, size=20000).astype(np.int32)
+    c[rng.random(c.size) < 0.1] = 0
+    c[:3] = 0
+    c[-3:] = 0
+    announce("lbs random seed=%d" % seed)
+    scan, res, W = gu.lbs(c)
+    oscan, oW = orc.exclusive_scan(c)
+    assert W == oW and np.array_equal(scan, oscan)
+    assert np.array_equal(res, orc.lbs(oW, oscan))
+
+
+def test_lbs_zipf_2_22(gu, orc, gpu):
+    c = gu.zipf_counts(1.1, 1 << 20, 1 << 22, seed=9)
+    announce("lbs zipf W=2^22 S=%d" % len(c))
+    scan, res, W = gu.lbs(c)
+    assert W == 1 << 22
+    oscan, oW = orc.exclusive_scan(c)
+    assert np.array_equal(scan, oscan)
+    assert np.array_equal(res, orc.lbs(oW, oscan))
+
+
+def test_lbs_capacity_error(gu, gpu):
+    c = np.array([5, 5], dtype=np.int32)
+    cb = gu.DeviceBuffer.from_array(c)
+    sb = gu.DeviceBuffer(8)
+    rb = gu.DeviceBuffer(4 * 4)
+    rb.zero()
+    announce("lbs capacity error")
+    with pytest.raises(gu.GuError) as ei:
+        gu.lbs_device(cb, 2, sb, rb, 4)
+    assert ei.value.status == gu.ERR_CAPACITY
+    assert np.all(rb.to_host(np.int32) == 0)  # nothing written
+
+
+def test_source_out_of_range(gu, gpu):
+    hg = _load(gu, "test1")
+    announce("source out of range")
+    with gu.DeviceGraph(hg) as dg:
+        with pytest.raises(gu.GuError) as ei:
+            dg.multi_search(np.array([0, 9], dtype=np.int32))
+        assert ei.value.status == gu.ERR_RANGE
+
+
+def test_repeatable(gu, gpu, rmat14):
+    hg, src, _ = rmat14
+    announce("repeatability rmat14")
+    with gu.DeviceGraph(hg) as dg:
+        a = dg.multi_search(src, True, True)
+        b = dg.multi_search(src, True, True)
+        c1 = dg.betweenness(src[:32])
+        c2 = dg.betweenness(src[:32])
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert np.array_equal(c1[0], c2[0]) and np.array_equal(c1[1], c2[1])
