@@ -119,8 +119,27 @@ __global__ __launch_bounds__(1024) void k_expand(Eng<NW> e) {
 
 template <int NW, bool SIGMA>
 __global__ __launch_bounds__(1024) void k_finalize(Eng<NW> e) {
-  if (e.ctrl->done || e.ctrl->mode != 0) return;
-  td_finalize<NW, SIGMA>(e, e.ctrl->level);
+  if (e.ctrl->done) return;
+  const int L = e.ctrl->level;
+  const int mode = e.ctrl->mode;
+  if (mode == 0) td_finalize<NW, SIGMA>(e, L);
+  hub_segments<NW, SIGMA>(e, L, mode);
+}
+
+template <int NW>
+__global__ __launch_bounds__(256) void k_hub_commit(Eng<NW> e) {
+  if (e.ctrl->done) return;
+  hub_commit<NW>(e, e.ctrl->level, e.ctrl->mode);
+}
+
+// Hubs take their path counts by atomic adds, so their rows start at zero.
+template <int NW>
+__global__ void k_zero_hub_rows(Eng<NW> e) {
+  const int waves = gridDim.x * (blockDim.x / kWave);
+  for (int h = blockIdx.x * (blockDim.x / kWave) + wave_id(); h < e.nhubs; h += waves) {
+    u64 *row = e.svm + (size_t)e.hubs[h] * (kWL * NW);
+    for (int j = 0; j < NW; j++) row[j * kWL + lane_id()] = 0;
+  }
 }
 
 template <int NW>
@@ -415,26 +434,118 @@ __global__ __launch_bounds__(256) void k_teps(Eng<NW> e) {
 // summed in FP32 with Neumaier compensation.  Successor lanes come from a
 // dense per-parity mask array tagged with the level that wrote it, so a
 // level never needs a clearing pass.
+struct Bw {
+  float *dl;          // [n][64*NW] dependencies
+  int *btag;          // [2][n] level that last wrote each parity's mask
+  u64 *blm;           // [2][n][NW] lanes at that level
+  unsigned *bwork;    // per-level work counters
+  float2 *part;       // [nsegs][64*NW] hub segment partials (sum, compensation)
+  const int *hub_seg0;  // [nhubs+1] first segment of each hub
+};
+
+__device__ __forceinline__ void neumaier(float &sum, float &comp, float x) {
+  float t = sum + x;
+  if (fabsf(sum) >= fabsf(x))
+    comp += (sum - t) + x;
+  else
+    comp += (x - t) + sum;
+  sum = t;
+}
+
+// Accumulate the dependency terms of v's neighbours in [r0, r1) for lanes mk
+// (sv = float(sigma_b(v)) per lane).  Up to four successor rows are loaded
+// before any is used.
 template <int NW>
-__global__ __launch_bounds__(256) void k_backward(Eng<NW> e, float *dl, int *btag, u64 *blm,
-                                                  unsigned *bwork, int L) {
+__device__ __forceinline__ void bw_accum(const Eng<NW> &e, const Bw &b, int L, const u64 *mk,
+                                         const float *sv, int r0, int r1, float *sum, float *comp) {
+  const int lane = lane_id();
+  const int pr = (L + 1) & 1;
+  const int *tagr = b.btag + (size_t)pr * e.n;
+  const u64 *lmr = b.blm + (size_t)pr * e.n * NW;
+  for (int e0 = r0; e0 < r1; e0 += kWave) {
+    int ed = e0 + lane;
+    bool valid = ed < r1;
+    int w = valid ? e.C[ed] : 0;
+    bool ok = valid && tagr[w] == L + 1;
+    u64 t[NW];
+    bool has = false;
+#pragma unroll
+    for (int j = 0; j < NW; j++) {
+      t[j] = ok ? (lmr[(size_t)w * NW + j] & mk[j]) : 0ull;
+      has |= t[j] != 0;
+    }
+    uint64_t pm = ballot(has);
+    while (pm) {
+      int k[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        if (pm) {
+          k[u] = __ffsll((unsigned long long)pm) - 1;
+          pm &= pm - 1;
+        } else {
+          k[u] = -1;
+        }
+      }
+      float sw[4][NW], dw[4][NW];
+      bool use[4][NW];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        if (k[u] >= 0) {
+          const int wk = readlane(w, k[u]);
+          const size_t rowb = (size_t)wk * (kWL * NW);
+#pragma unroll
+          for (int j = 0; j < NW; j++) {
+            u64 tk = readlane64(t[j], k[u]);
+            use[u][j] = (tk >> lane) & 1ull;
+            sw[u][j] = use[u][j] ? (float)e.svm[rowb + j * kWL + lane] : 1.0f;
+            dw[u][j] = use[u][j] ? b.dl[rowb + j * kWL + lane] : 0.0f;
+          }
+        } else {
+#pragma unroll
+          for (int j = 0; j < NW; j++) use[u][j] = false;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int j = 0; j < NW; j++)
+          if (use[u][j]) neumaier(sum[j], comp[j], (sv[j] / sw[u][j]) * (1.0f + dw[u][j]));
+    }
+  }
+}
+
+template <int NW>
+__device__ __forceinline__ void bw_lanes(const Eng<NW> &e, int v, int L, u64 *mk) {
+  // lanes b with d_b(v) = L, from v's own distance row
+  const int lane = lane_id();
+  const int *drow = e.dvm + (size_t)v * (kWL * NW);
+#pragma unroll
+  for (int j = 0; j < NW; j++) {
+    bool on = ((e.visit[(size_t)v * NW + j] >> lane) & 1ull) && drow[j * kWL + lane] == L;
+    mk[j] = ballot(on);
+  }
+}
+
+template <int NW>
+__global__ __launch_bounds__(256) void k_backward(Eng<NW> e, Bw b, int L) {
   const int lane = lane_id();
   const long long beg = e.ep[L], end = e.ep[L + 1];
   const long long cnt_all = end - beg;
-  const int pr = (L + 1) & 1, pw = L & 1;
-  const int *tagr = btag + (size_t)pr * e.n;
-  const u64 *lmr = blm + (size_t)pr * e.n * NW;
-  int *tagw = btag + (size_t)pw * e.n;
-  u64 *lmw = blm + (size_t)pw * e.n * NW;
+  const int pw = L & 1;
+  int *tagw = b.btag + (size_t)pw * e.n;
+  u64 *lmw = b.blm + (size_t)pw * e.n * NW;
+  constexpr int CH = 16;
   while (true) {
     unsigned base = 0;
-    if (lane == 0) base = atomicAdd(&bwork[L], (unsigned)kWave);
+    if (lane == 0) base = atomicAdd(&b.bwork[L], (unsigned)CH);
     base = bcast(base);
     if ((long long)base >= cnt_all) break;
-    const int cnt = cnt_all - base < kWave ? (int)(cnt_all - base) : kWave;
+    const int cnt = cnt_all - base < CH ? (int)(cnt_all - base) : CH;
     for (int i = 0; i < cnt; i++) {
       const long long pos = beg + base + i;
       const int v = e.q[pos];
+      const int r0 = e.R[v], r1 = e.R[v + 1];
+      if (r1 - r0 > e.hub_deg) continue;  // hubs: segment partials + combine
       u64 mk[NW];
       float sv[NW], sum[NW], comp[NW];
 #pragma unroll
@@ -444,51 +555,79 @@ __global__ __launch_bounds__(256) void k_backward(Eng<NW> e, float *dl, int *bta
         comp[j] = 0.0f;
         sv[j] = ((mk[j] >> lane) & 1ull) ? (float)e.svm[(size_t)v * (kWL * NW) + j * kWL + lane] : 0.0f;
       }
-      const int r0 = e.R[v], r1 = e.R[v + 1];
-      for (int e0 = r0; e0 < r1; e0 += kWave) {
-        int ed = e0 + lane;
-        bool valid = ed < r1;
-        int w = valid ? e.C[ed] : 0;
-        bool ok = valid && tagr[w] == L + 1;
-        u64 t[NW];
-        bool has = false;
-#pragma unroll
-        for (int j = 0; j < NW; j++) {
-          t[j] = ok ? (lmr[(size_t)w * NW + j] & mk[j]) : 0ull;
-          has |= t[j] != 0;
-        }
-        uint64_t pm = ballot(has);
-        while (pm) {
-          int k = __ffsll((unsigned long long)pm) - 1;
-          pm &= pm - 1;
-          int wk = readlane(w, k);
-          const size_t rowb = (size_t)wk * (kWL * NW);
-#pragma unroll
-          for (int j = 0; j < NW; j++) {
-            u64 tk = readlane64(t[j], k);
-            if ((tk >> lane) & 1ull) {
-              float sw = (float)e.svm[rowb + j * kWL + lane];
-              float dw = dl[rowb + j * kWL + lane];
-              float x = (sv[j] / sw) * (1.0f + dw);
-              float s = sum[j] + x;
-              if (fabsf(sum[j]) >= fabsf(x))
-                comp[j] += (sum[j] - s) + x;
-              else
-                comp[j] += (x - s) + sum[j];
-              sum[j] = s;
-            }
-          }
-        }
-      }
+      bw_accum<NW>(e, b, L, mk, sv, r0, r1, sum, comp);
       const size_t vrow = (size_t)v * (kWL * NW);
 #pragma unroll
       for (int j = 0; j < NW; j++)
-        if ((mk[j] >> lane) & 1ull) dl[vrow + j * kWL + lane] = sum[j] + comp[j];
+        if ((mk[j] >> lane) & 1ull) b.dl[vrow + j * kWL + lane] = sum[j] + comp[j];
       if (lane == 0) {
 #pragma unroll
         for (int j = 0; j < NW; j++) lmw[(size_t)v * NW + j] = mk[j];
         tagw[v] = L;
       }
+    }
+  }
+}
+
+// Hub segments of the backward pass: each wave writes its segment's
+// compensated partial sums; the combine kernel adds them in segment order,
+// so the result does not depend on scheduling.
+template <int NW>
+__global__ __launch_bounds__(256) void k_bw_hub_segs(Eng<NW> e, Bw b, int L) {
+  const int lane = lane_id();
+  const int waves = gridDim.x * (blockDim.x / kWave);
+  for (int si = blockIdx.x * (blockDim.x / kWave) + wave_id(); si < e.nsegs; si += waves) {
+    const int4 sg = e.segs[si];
+    const int v = sg.x;
+    u64 mk[NW];
+    bw_lanes<NW>(e, v, L, mk);
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < NW; j++) any |= mk[j] != 0;
+    if (!any) continue;
+    float sv[NW], sum[NW], comp[NW];
+#pragma unroll
+    for (int j = 0; j < NW; j++) {
+      sum[j] = 0.0f;
+      comp[j] = 0.0f;
+      sv[j] = ((mk[j] >> lane) & 1ull) ? (float)e.svm[(size_t)v * (kWL * NW) + j * kWL + lane] : 0.0f;
+    }
+    bw_accum<NW>(e, b, L, mk, sv, sg.y, sg.z, sum, comp);
+#pragma unroll
+    for (int j = 0; j < NW; j++) b.part[(size_t)si * (kWL * NW) + j * kWL + lane] = make_float2(sum[j], comp[j]);
+  }
+}
+
+template <int NW>
+__global__ __launch_bounds__(256) void k_bw_hub_combine(Eng<NW> e, Bw b, int L) {
+  const int lane = lane_id();
+  const int pw = L & 1;
+  int *tagw = b.btag + (size_t)pw * e.n;
+  u64 *lmw = b.blm + (size_t)pw * e.n * NW;
+  const int waves = gridDim.x * (blockDim.x / kWave);
+  for (int h = blockIdx.x * (blockDim.x / kWave) + wave_id(); h < e.nhubs; h += waves) {
+    const int v = e.hubs[h];
+    u64 mk[NW];
+    bw_lanes<NW>(e, v, L, mk);
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < NW; j++) any |= mk[j] != 0;
+    if (!any) continue;
+    const size_t vrow = (size_t)v * (kWL * NW);
+#pragma unroll
+    for (int j = 0; j < NW; j++) {
+      float sum = 0.0f, comp = 0.0f;
+      for (int si = b.hub_seg0[h]; si < b.hub_seg0[h + 1]; si++) {
+        float2 p = b.part[(size_t)si * (kWL * NW) + j * kWL + lane];
+        neumaier(sum, comp, p.x);
+        comp += p.y;
+      }
+      if ((mk[j] >> lane) & 1ull) b.dl[vrow + j * kWL + lane] = sum + comp;
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int j = 0; j < NW; j++) lmw[(size_t)v * NW + j] = mk[j];
+      tagw[v] = L;
     }
   }
 }
@@ -565,14 +704,16 @@ struct Workspace {
   int n = 0;
   bool stack = false;
   u64 *visit = nullptr, *fr = nullptr, *svm = nullptr, *smask = nullptr, *blm = nullptr;
+  u64 *hub_found = nullptr;
   int *q = nullptr, *tag = nullptr, *dvm = nullptr, *off = nullptr, *srcs = nullptr, *btag = nullptr;
   float *dl = nullptr;
+  float2 *part = nullptr;
   long long *ep = nullptr, *tsum = nullptr;
   unsigned *bwork = nullptr;
   size_t S_cap = 0;   // entries of q in stack mode
   size_t bytes = 0;
   void release() {
-    void *ps[] = {visit, fr, svm, smask, blm, q, tag, dvm, off, srcs, btag, dl, ep, tsum, bwork};
+    void *ps[] = {visit, fr, svm, smask, blm, hub_found, q, tag, dvm, off, srcs, btag, dl, part, ep, tsum, bwork};
     for (void *p : ps)
       if (p) (void)hipFree(p);
     *this = Workspace();
@@ -593,6 +734,14 @@ struct gu_graph {
   Ctrl *ctrl_h = nullptr;     // pinned host mirror
   int *srcs_h = nullptr;      // pinned staging for batch sources
   int max_deg = 0;
+  // degree-balanced work units, fixed per graph
+  int2 *blocks = nullptr;     // bottom-up vertex blocks
+  int nblocks = 0;
+  int4 *segs = nullptr;       // hub adjacency segments
+  int nsegs = 0;
+  int *hubs = nullptr;
+  int nhubs = 0;
+  int *hub_seg0 = nullptr;    // [nhubs+1] first segment of each hub
   int tuned_variant = -1;     // autotuner cache
   int tuned_direction_alpha = 0;
 };
@@ -600,6 +749,62 @@ struct gu_graph {
 namespace {
 
 int pick_grid(const gu_graph *g, int per_cu) { return std::max(1, g->cus * per_cu); }
+
+// Work-unit shape: a vertex of degree above kHubDeg is a hub whose adjacency
+// is pulled in segments of kSeg entries by separate waves; every other
+// vertex belongs to a block of at most 64 consecutive vertices holding at
+// most kBlockEdges entries (or a single vertex).
+constexpr int kHubDeg = 1024;
+constexpr int kSeg = 1024;
+constexpr int kBlockEdges = 2048;
+
+int build_units(gu_graph *g, const int32_t *R) {
+  std::vector<int2> blocks;
+  std::vector<int4> segs;
+  std::vector<int> hubs, seg0;
+  int b0 = -1, bcount = 0;
+  long long bedges = 0;
+  auto close = [&](int v) {
+    if (b0 >= 0) blocks.push_back(make_int2(b0, v));
+    b0 = -1;
+    bcount = 0;
+    bedges = 0;
+  };
+  for (int v = 0; v < g->n; v++) {
+    const int d = R[v + 1] - R[v];
+    if (d > kHubDeg) {
+      close(v);
+      const int h = (int)hubs.size();
+      hubs.push_back(v);
+      seg0.push_back((int)segs.size());
+      for (int e0 = R[v]; e0 < R[v + 1]; e0 += kSeg)
+        segs.push_back(make_int4(v, e0, std::min(e0 + kSeg, R[v + 1]), h));
+      continue;
+    }
+    if (b0 >= 0 && (bcount == kWave || bedges + d > kBlockEdges)) close(v);
+    if (b0 < 0) b0 = v;
+    bcount++;
+    bedges += d;
+  }
+  close(g->n);
+  seg0.push_back((int)segs.size());
+  g->nblocks = (int)blocks.size();
+  g->nsegs = (int)segs.size();
+  g->nhubs = (int)hubs.size();
+  if (dalloc(&g->blocks, blocks.size()) != hipSuccess || dalloc(&g->segs, segs.size()) != hipSuccess ||
+      dalloc(&g->hubs, hubs.size()) != hipSuccess || dalloc(&g->hub_seg0, seg0.size()) != hipSuccess)
+    return GU_ERR_NOMEM;
+  if (hipMemcpy(g->hub_seg0, seg0.data(), sizeof(int) * seg0.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return GU_ERR_HIP;
+  if ((!blocks.empty() && hipMemcpy(g->blocks, blocks.data(), sizeof(int2) * blocks.size(),
+                                    hipMemcpyHostToDevice) != hipSuccess) ||
+      (!segs.empty() && hipMemcpy(g->segs, segs.data(), sizeof(int4) * segs.size(),
+                                  hipMemcpyHostToDevice) != hipSuccess) ||
+      (!hubs.empty() && hipMemcpy(g->hubs, hubs.data(), sizeof(int) * hubs.size(),
+                                  hipMemcpyHostToDevice) != hipSuccess))
+    return GU_ERR_HIP;
+  return GU_OK;
+}
 
 int ensure_ws(gu_graph *g, int NW, bool stack, size_t want_S, hipStream_t st) {
   Workspace &w = g->ws;
@@ -618,6 +823,7 @@ int ensure_ws(gu_graph *g, int NW, bool stack, size_t want_S, hipStream_t st) {
     GU_HIP(dalloc(&w.tsum, n + 8));
     GU_HIP(dalloc(&w.srcs, 256));
     GU_HIP(dalloc(&w.ep, n + 3));
+    GU_HIP(dalloc(&w.hub_found, (size_t)std::max(g->nhubs, 1) * NW));
     w.bytes = n * NW * 8 * 4 + n * 4 * 2 + n * WL * 12 + (n + 8) * 8 + 1024 + (n + 3) * 8;
     w.S_cap = 3 * n;
     GU_HIP(dalloc(&w.q, w.S_cap));
@@ -629,6 +835,7 @@ int ensure_ws(gu_graph *g, int NW, bool stack, size_t want_S, hipStream_t st) {
       GU_HIP(dalloc(&w.btag, 2 * n));
       GU_HIP(dalloc(&w.blm, 2 * n * NW));
       GU_HIP(dalloc(&w.bwork, n + 3));
+      GU_HIP(dalloc(&w.part, (size_t)std::max(g->nsegs, 1) * WL));
       w.bytes += w.S_cap * NW * 8 + n * WL * 4 + 2 * n * 4 + 2 * n * NW * 8 + (n + 3) * 4;
     }
   }
@@ -687,6 +894,14 @@ Eng<NW> make_eng(gu_graph *g, int nb, bool stack, bool sigma) {
   e.off = w.off;
   e.tsum = w.tsum;
   e.srcs = w.srcs;
+  e.blocks = g->blocks;
+  e.nblocks = g->nblocks;
+  e.segs = g->segs;
+  e.nsegs = g->nsegs;
+  e.hubs = g->hubs;
+  e.nhubs = g->nhubs;
+  e.hub_deg = kHubDeg;
+  e.hub_found = w.hub_found;
   for (int j = 0; j < NW; j++) {
     int lanes = std::min(std::max(nb - j * kWL, 0), kWL);
     e.all[j] = lanes == 64 ? ~0ull : ((1ull << lanes) - 1);
@@ -712,6 +927,7 @@ void launch_level(gu_graph *g, const Eng<NW> &e, int variant, int block, int gri
     default: k_expand<NW, V_SHUFFLE, SIGMA><<<grid, block, 0, st>>>(e); break;
   }
   k_finalize<NW, SIGMA><<<grid, 256, 0, st>>>(e);
+  if (e.nhubs) k_hub_commit<NW><<<std::max(1, std::min(grid, (e.nhubs + 3) / 4)), 256, 0, st>>>(e);
   k_control<NW><<<1, 64, 0, st>>>(e);
 }
 
@@ -743,6 +959,10 @@ int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const g
   *g->ctrl_h = c0;
   GU_HIP(hipMemcpyAsync(g->ctrl, g->ctrl_h, sizeof(Ctrl), hipMemcpyHostToDevice, st));
   Eng<NW> e = make_eng<NW>(g, nb, stack, sigma);
+  if (g->nhubs) {
+    k_fill_u64<<<1, 256, 0, st>>>(w.hub_found, (size_t)g->nhubs * NW, 0ull);
+    if (sigma) k_zero_hub_rows<NW><<<std::max(1, std::min(fgrid, (g->nhubs + 3) / 4)), 256, 0, st>>>(e);
+  }
   k_seed<NW><<<1, 256, 0, st>>>(e);
   const int block = cfg.wg_size > 0 ? cfg.wg_size : 256;
   const int grid = pick_grid(g, block >= 1024 ? 2 : (block >= 512 ? 4 : 8));
@@ -777,8 +997,15 @@ int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const g
   if (job == JOB_BC) {
     k_fill_i32<<<fgrid, 256, 0, st>>>(w.btag, 2 * n, -1);
     k_fill_i32<<<1, 256, 0, st>>>((int *)w.bwork, (size_t)last + 2, 0);
-    for (int L = last; L >= 1; L--)
-      k_backward<NW><<<pick_grid(g, 8), 256, 0, st>>>(e, w.dl, w.btag, w.blm, w.bwork, L);
+    Bw b{w.dl, w.btag, w.blm, w.bwork, w.part, g->hub_seg0};
+    const int hgrid = std::max(1, std::min(pick_grid(g, 8), (g->nsegs + 3) / 4));
+    for (int L = last; L >= 1; L--) {
+      k_backward<NW><<<pick_grid(g, 8), 256, 0, st>>>(e, b, L);
+      if (g->nhubs) {
+        k_bw_hub_segs<NW><<<hgrid, 256, 0, st>>>(e, b, L);
+        k_bw_hub_combine<NW><<<std::max(1, std::min(hgrid, (g->nhubs + 3) / 4)), 256, 0, st>>>(e, b, L);
+      }
+    }
     k_bc_out<NW><<<pick_grid(g, 8), 256, 0, st>>>(e, w.dl, out.delta, out.bc);
   }
   k_teps<NW><<<pick_grid(g, 4), 256, 0, st>>>(e);
@@ -1053,6 +1280,7 @@ int gu_graph_create(const int32_t *R, const int32_t *C, const int32_t *F, int32_
       rc = GU_ERR_HIP;
       break;
     }
+    if ((rc = build_units(g, R)) != GU_OK) break;
     if (F) {
       if (m > 0 && hipMemcpy(g->F, F, sizeof(int) * (size_t)m, hipMemcpyHostToDevice) != hipSuccess) {
         rc = GU_ERR_HIP;
@@ -1077,6 +1305,10 @@ int gu_graph_destroy(gu_graph_t g) {
   if (g->R) (void)hipFree(g->R);
   if (g->C) (void)hipFree(g->C);
   if (g->F) (void)hipFree(g->F);
+  if (g->blocks) (void)hipFree(g->blocks);
+  if (g->segs) (void)hipFree(g->segs);
+  if (g->hubs) (void)hipFree(g->hubs);
+  if (g->hub_seg0) (void)hipFree(g->hub_seg0);
   if (g->ctrl) (void)hipFree(g->ctrl);
   if (g->ctrl_h) (void)hipHostFree(g->ctrl_h);
   if (g->srcs_h) (void)hipHostFree(g->srcs_h);

@@ -70,6 +70,14 @@ struct Eng {
   int *off;            // edge-parallel: frontier degree offsets [n+1]
   long long *tsum;     // scan tile sums
   const int *srcs;     // [nb] batch sources
+  const int2 *blocks;  // bottom-up vertex blocks: [v0, v1), <= 64 vertices,
+  int nblocks;         //   bounded edge count, hubs excluded
+  const int4 *segs;    // hub adjacency segments: (w, e0, e1, hub index)
+  int nsegs;
+  const int *hubs;     // vertices of degree > hub_deg
+  int nhubs;
+  int hub_deg;
+  u64 *hub_found;      // [nhubs][NW] lanes found by the hub's segments
   u64 all[NW];         // mask of active lanes per word
   int stack;           // keep every level's queue (Brandes backward needs it)
   int sigma;           // compute path counts
@@ -403,6 +411,54 @@ __device__ void td_edge_parallel(const Eng<NW> &e, int L) {
   }
 }
 
+// Add the path-count rows of the predecessor lanes selected by pm (lane k
+// holds neighbour v with lane mask mv): acc[b] += sigma[v_k][b] for every b in
+// mv_k.  Four rows are loaded before any is consumed so each wave keeps four
+// row loads in flight.
+template <int NW>
+__device__ __forceinline__ void add_rows(const Eng<NW> &e, int v, const u64 *mv, uint64_t pm, u64 *acc,
+                                         int *ovf, u64 *dag) {
+  const int lane = lane_id();
+  while (pm) {
+    int k[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (pm) {
+        k[u] = __ffsll((unsigned long long)pm) - 1;
+        pm &= pm - 1;
+      } else {
+        k[u] = -1;
+      }
+    }
+    u64 x[4][NW];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (k[u] >= 0) {
+        const int vk = readlane(v, k[u]);
+        const u64 *row = e.svm + (size_t)vk * (kWL * NW);
+#pragma unroll
+        for (int j = 0; j < NW; j++) {
+          const u64 mk = readlane64(mv[j], k[u]);
+          *dag += (u64)__popcll(mk);
+          x[u][j] = ((mk >> lane) & 1ull) ? row[j * kWL + lane] : 0ull;
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < NW; j++) x[u][j] = 0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+#pragma unroll
+      for (int j = 0; j < NW; j++) {
+        u64 sum = acc[j] + x[u][j];
+        *ovf |= sum < x[u][j];
+        acc[j] = sum;
+      }
+    }
+  }
+}
+
 // ---- path-count pull (lane = source) -------------------------------------
 
 // For vertex w and target lanes tgt (lanes that reach w at level L+1),
@@ -433,24 +489,7 @@ __device__ __forceinline__ void pull(const Eng<NW> &e, const u64 *fc, int r0, in
     if (SIGMA) {
 #pragma unroll
       for (int j = 0; j < NW; j++) found[j] |= mv[j];
-      uint64_t pm = ballot(has);
-      while (pm) {
-        int k = __ffsll((unsigned long long)pm) - 1;
-        pm &= pm - 1;
-        int vk = readlane(v, k);
-        const u64 *row = e.svm + (size_t)vk * (kWL * NW);
-#pragma unroll
-        for (int j = 0; j < NW; j++) {
-          u64 mk = readlane64(mv[j], k);
-          *dag += (u64)__popcll(mk);
-          if ((mk >> lane) & 1ull) {
-            u64 x = row[j * kWL + lane];
-            u64 s = acc[j] + x;
-            *ovf |= s < x;
-            acc[j] = s;
-          }
-        }
-      }
+      add_rows<NW>(e, v, mv, ballot(has), acc, ovf, dag);
     } else {
       bool full = true;
 #pragma unroll
@@ -496,8 +535,9 @@ __device__ __forceinline__ bool full_mask(const Eng<NW> &e, const u64 *vo, const
 }
 
 // Bottom-up level: every vertex that some lane has not reached pulls from
-// its neighbours on the frontier.  Waves take blocks of 64 consecutive
-// vertices; the block's discoveries are pushed with one atomic.
+// its neighbours on the frontier.  Waves take precomputed blocks of at most
+// 64 consecutive vertices with a bounded edge count (hubs are left to the
+// segment pass); a block's discoveries are pushed with one atomic.
 template <int NW, bool SIGMA>
 __device__ void bu_level(const Eng<NW> &e, int L) {
   const int lane = lane_id();
@@ -507,17 +547,20 @@ __device__ void bu_level(const Eng<NW> &e, int L) {
   u64 fe_acc = 0, ud_acc = 0, dag = 0;
   int ovf = 0;
   while (true) {
-    unsigned blk = 0;
-    if (lane == 0) blk = atomicAdd(&e.ctrl->work[1], (unsigned)kWave);
-    blk = bcast(blk);
-    if (blk >= (unsigned)e.n) break;
-    const int nblk = e.n - (int)blk < kWave ? e.n - (int)blk : kWave;
+    unsigned bi = 0;
+    if (lane == 0) bi = atomicAdd(&e.ctrl->work[1], 1u);
+    bi = bcast(bi);
+    if (bi >= (unsigned)e.nblocks) break;
+    const int2 blk = e.blocks[bi];
+    const int nblk = blk.y - blk.x;
     uint64_t disc = 0;
     u64 mine[NW];
 #pragma unroll
     for (int j = 0; j < NW; j++) mine[j] = 0;
     for (int i = 0; i < nblk; i++) {
-      const int w = (int)blk + i;
+      const int w = blk.x + i;
+      const int r0 = e.R[w], r1 = e.R[w + 1];
+      if (r0 == r1) continue;
       u64 vo[NW], tgt[NW];
       bool any = false;
 #pragma unroll
@@ -527,8 +570,6 @@ __device__ void bu_level(const Eng<NW> &e, int L) {
         any |= tgt[j] != 0;
       }
       if (!any) continue;
-      const int r0 = e.R[w], r1 = e.R[w + 1];
-      if (r0 == r1) continue;
       u64 found[NW], acc[NW];
       pull<NW, SIGMA>(e, fc, r0, r1, tgt, found, acc, &ovf, &dag);
       bool anyn = false;
@@ -554,7 +595,7 @@ __device__ void bu_level(const Eng<NW> &e, int L) {
       base = bcast(base);
       if ((disc >> lane) & 1ull) {
         unsigned pos = base + (unsigned)rank_in(disc);
-        Qn[pos] = (int)blk + lane;
+        Qn[pos] = blk.x + lane;
         if (e.stack) {
 #pragma unroll
           for (int j = 0; j < NW; j++) e.smask[((size_t)e.ep[L + 1] + pos) * NW + j] = mine[j];
@@ -570,10 +611,12 @@ __device__ void bu_level(const Eng<NW> &e, int L) {
   }
 }
 
-// Top-down finalize: for every vertex pushed this level, commit its new
-// lanes and (with path counts) pull sigma from its frontier neighbours.
+// Top-down finalize: for every non-hub vertex pushed this level, commit its
+// new lanes and (with path counts) pull sigma from its frontier neighbours.
+// Hubs only record their stack lanes here; their pull runs in segments.
 template <int NW, bool SIGMA>
 __device__ void td_finalize(const Eng<NW> &e, int L) {
+  constexpr int CH = 16;
   const int lane = lane_id();
   const int *Qn = qptr(e, L + 1);
   const unsigned qn = e.ctrl->qlen[slot_of(L + 1)];
@@ -583,10 +626,10 @@ __device__ void td_finalize(const Eng<NW> &e, int L) {
   int ovf = 0;
   while (true) {
     unsigned base = 0;
-    if (lane == 0) base = atomicAdd(&e.ctrl->work[2], (unsigned)kWave);
+    if (lane == 0) base = atomicAdd(&e.ctrl->work[2], (unsigned)CH);
     base = bcast(base);
     if (base >= qn) break;
-    const int cnt = qn - base < (unsigned)kWave ? (int)(qn - base) : kWave;
+    const int cnt = qn - base < (unsigned)CH ? (int)(qn - base) : CH;
     const int wl = lane < cnt ? Qn[base + lane] : 0;
     for (int i = 0; i < cnt; i++) {
       const int w = readlane(wl, i);
@@ -597,16 +640,17 @@ __device__ void td_finalize(const Eng<NW> &e, int L) {
         nw[j] = fn[(size_t)w * NW + j];
         acc[j] = 0;
       }
+      if (e.stack && lane == 0) {
+#pragma unroll
+        for (int j = 0; j < NW; j++) e.smask[((size_t)e.ep[L + 1] + base + i) * NW + j] = nw[j];
+      }
       const int r0 = e.R[w], r1 = e.R[w + 1];
+      if (r1 - r0 > e.hub_deg) continue;  // committed by the hub pass
       if (SIGMA) {
         u64 found[NW];
         pull<NW, true>(e, fc, r0, r1, nw, found, acc, &ovf, &dag);
       }
       commit<NW, SIGMA>(e, L, w, vo, nw, acc);
-      if (e.stack && lane == 0) {
-#pragma unroll
-        for (int j = 0; j < NW; j++) e.smask[((size_t)e.ep[L + 1] + base + i) * NW + j] = nw[j];
-      }
       fe_acc += (u64)(r1 - r0);
       if (full_mask<NW>(e, vo, nw)) ud_acc += (u64)(r1 - r0);
     }
@@ -616,6 +660,100 @@ __device__ void td_finalize(const Eng<NW> &e, int L) {
     if (fe_acc) atomicAdd(&e.ctrl->fr_edges[slot_of(L + 1)], fe_acc);
     if (ud_acc) atomicAdd(&e.ctrl->unvis_dec, ud_acc);
     if (dag) atomicAdd(&e.ctrl->dag, dag);
+  }
+}
+
+// Hub segments (both directions): each wave pulls one segment of a hub's
+// adjacency for the hub's target lanes (top-down: the lanes pushed this
+// level; bottom-up: the lanes not yet reached) and adds its partial path
+// counts with 64-bit atomics -- exact, so the sum does not depend on the
+// order of the segments.  Bottom-up also ORs the lanes it found.
+template <int NW, bool SIGMA>
+__device__ void hub_segments(const Eng<NW> &e, int L, int mode) {
+  const int lane = lane_id();
+  const u64 *fc = frs(e, L);
+  const u64 *fn = frs(e, L + 1);
+  u64 dag = 0;
+  int ovf = 0;
+  while (true) {
+    unsigned si = 0;
+    if (lane == 0) si = atomicAdd(&e.ctrl->work[3], 1u);
+    si = bcast(si);
+    if (si >= (unsigned)e.nsegs) break;
+    const int4 sg = e.segs[si];
+    const int w = sg.x;
+    u64 tgt[NW];
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < NW; j++) {
+      tgt[j] = mode ? (e.all[j] & ~e.visit[(size_t)w * NW + j]) : fn[(size_t)w * NW + j];
+      any |= tgt[j] != 0;
+    }
+    if (!any) continue;
+    u64 found[NW], acc[NW];
+    pull<NW, SIGMA>(e, fc, sg.y, sg.z, tgt, found, acc, &ovf, &dag);
+    if (mode && lane == 0) {
+#pragma unroll
+      for (int j = 0; j < NW; j++)
+        if (found[j]) atomicOr(&e.hub_found[(size_t)sg.w * NW + j], found[j]);
+    }
+    if (SIGMA) {
+      u64 *srow = e.svm + (size_t)w * (kWL * NW);
+#pragma unroll
+      for (int j = 0; j < NW; j++) {
+        if (acc[j] != 0) {
+          u64 old = atomicAdd(&srow[j * kWL + lane], acc[j]);
+          ovf |= old + acc[j] < acc[j];
+        }
+      }
+    }
+  }
+  if (ballot(ovf != 0) && lane == 0) atomicOr(&e.ctrl->overflow, 1);
+  if (lane == 0 && dag) atomicAdd(&e.ctrl->dag, dag);
+}
+
+// Hub commit: new lanes of every hub (bottom-up: found by its segments, then
+// pushed; top-down: already pushed by the expansion), distances, visit mask.
+template <int NW>
+__device__ void hub_commit(const Eng<NW> &e, int L, int mode) {
+  const int lane = lane_id();
+  u64 *fn = frs(e, L + 1);
+  int *Qn = qptr(e, L + 1);
+  const int waves = gridDim.x * (blockDim.x / kWave);
+  u64 fe_acc = 0, ud_acc = 0;
+  for (int h = blockIdx.x * (blockDim.x / kWave) + wave_id(); h < e.nhubs; h += waves) {
+    const int w = e.hubs[h];
+    u64 vo[NW], nw[NW];
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < NW; j++) {
+      vo[j] = e.visit[(size_t)w * NW + j];
+      nw[j] = mode ? e.hub_found[(size_t)h * NW + j] : fn[(size_t)w * NW + j];
+      any |= nw[j] != 0;
+    }
+    if (mode && lane == 0) {
+#pragma unroll
+      for (int j = 0; j < NW; j++) e.hub_found[(size_t)h * NW + j] = 0;
+    }
+    if (!any) continue;
+    commit<NW, false>(e, L, w, vo, nw, nullptr);
+    if (mode && lane == 0) {
+#pragma unroll
+      for (int j = 0; j < NW; j++) fn[(size_t)w * NW + j] = nw[j];
+      unsigned pos = atomicAdd(&e.ctrl->qlen[slot_of(L + 1)], 1u);
+      Qn[pos] = w;
+      if (e.stack) {
+#pragma unroll
+        for (int j = 0; j < NW; j++) e.smask[((size_t)e.ep[L + 1] + pos) * NW + j] = nw[j];
+      }
+    }
+    const u64 deg = (u64)(e.R[w + 1] - e.R[w]);
+    fe_acc += deg;
+    if (full_mask<NW>(e, vo, nw)) ud_acc += deg;
+  }
+  if (lane == 0) {
+    if (fe_acc) atomicAdd(&e.ctrl->fr_edges[slot_of(L + 1)], fe_acc);
+    if (ud_acc) atomicAdd(&e.ctrl->unvis_dec, ud_acc);
   }
 }
 

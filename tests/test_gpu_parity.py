@@ -86,6 +86,18 @@ def test_rmat14_64_sources(gu, orc, gpu, rmat14, variant, direction):
     _search_and_compare(gu, orc, hg, src, variant, direction, 64, "rmat14", ref=ref)
 
 
+def test_rmat14_has_hubs(gu, rmat14):
+    hg, _, _ = rmat14
+    assert hg.degrees().max() > 1024
+
+
+@pytest.mark.parametrize("width", [128, 256])
+def test_rmat14_wide(gu, orc, gpu, rmat14, width):
+    hg, src, _ = rmat14
+    src2 = np.concatenate([src, hg.sample_sources(width - 64, seed=11)]).astype(np.int32)
+    _search_and_compare(gu, orc, hg, src2, "edge_parallel", "auto", width, "rmat14")
+
+
 def test_rmat14_duplicate_and_partial_batch(gu, orc, gpu, rmat14):
     hg, src, _ = rmat14
     sources = np.concatenate([src[:10], src[:10], src[3:40]]).astype(np.int32)  # 57 lanes, repeats
@@ -153,6 +165,45 @@ def test_bc_rmat12(gu, orc, gpu, width):
     hg = gu.HostGraph.rmat(12, 16, seed=5)
     src = hg.sample_sources(200, seed=6)
     _check_bc(gu, orc, hg, src, width, "rmat12")
+
+
+def _hub_graph(gu):
+    """A 4000-vertex path plus two hubs (degrees 2000 and 1334) wired to
+    every 2nd / 3rd path vertex: hub adjacency lists span several segments
+    and the hubs sit on many shortest paths."""
+    n = 4002
+    edges = [(i, i + 1) for i in range(3999)]
+    edges += [(4000, i) for i in range(0, 4000, 2)]
+    edges += [(4001, i) for i in range(1, 4000, 3)]
+    adj = [[] for _ in range(n)]
+    for u, v in edges:
+        adj[u].append(v)
+        adj[v].append(u)
+    R = np.zeros(n + 1, dtype=np.int32)
+    R[1:] = np.cumsum([len(a) for a in adj])
+    C = np.array([c for a in adj for c in sorted(a)], dtype=np.int32)
+    return gu.HostGraph(R, C)
+
+
+@pytest.mark.parametrize("direction", ["auto", "topdown", "bottomup"])
+@pytest.mark.parametrize("variant", ["shuffle", "edge_parallel"])
+def test_hub_graph_search(gu, orc, gpu, direction, variant):
+    hg = _hub_graph(gu)
+    src = np.array([0, 1, 2, 3999, 2000, 4000, 4001, 1234] * 8, dtype=np.int32)
+    _search_and_compare(gu, orc, hg, src, variant, direction, 64, "hubgraph")
+
+
+@pytest.mark.parametrize("width", [64, 256])
+def test_bc_hub_graph(gu, orc, gpu, width):
+    hg = _hub_graph(gu)
+    src = np.arange(0, 4002, 37, dtype=np.int32)
+    _check_bc(gu, orc, hg, src, width, "hubgraph")
+
+
+def test_bc_rmat14_hubs(gu, orc, gpu, rmat14):
+    hg, src, _ = rmat14
+    assert hg.degrees().max() > 1024  # exercises the hub segments
+    _check_bc(gu, orc, hg, src[:48], 64, "rmat14")
 
 
 def test_bc_grid_deep(gu, orc, gpu):
