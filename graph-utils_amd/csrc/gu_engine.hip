@@ -76,10 +76,6 @@ __global__ void k_seed(Eng<NW> e) {
   __syncthreads();
   if (e.stack) {
     unsigned q0 = atomicAdd(&e.ctrl->qlen[0], 0u);
-    for (unsigned i = tid; i < q0; i += blockDim.x) {
-      int s = e.q[i];
-      for (int j = 0; j < NW; j++) e.smask[(size_t)i * NW + j] = atomicOr(&f0[(size_t)s * NW + j], 0ull);
-    }
     if (tid == 0) {
       e.ep[0] = 0;
       e.ep[1] = q0;
@@ -91,30 +87,47 @@ __global__ void k_seed(Eng<NW> e) {
 // ---- one level ------------------------------------------------------------
 template <int NW, int VAR, bool SIGMA>
 __global__ __launch_bounds__(1024) void k_expand(Eng<NW> e) {
+  __shared__ int s_pb[16][kPB];
   if (e.ctrl->done) return;
   const int L = e.ctrl->level;
   const int mode = e.ctrl->mode;
+  const unsigned nt = gridDim.x * blockDim.x;
+  const unsigned gt = blockIdx.x * blockDim.x + threadIdx.x;
   if (L >= 1) {  // clear the stale frontier slot (level L-1) for reuse at L+1
     const int *Qs = qptr(e, L - 1);
     const unsigned qs = e.ctrl->qlen[slot_of(L - 1)];
     u64 *fs = frs(e, L - 1);
-    const unsigned nt = gridDim.x * blockDim.x;
-    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < qs; i += nt) {
+    for (unsigned i = gt; i < qs; i += nt) {
       int v = Qs[i];
 #pragma unroll
       for (int j = 0; j < NW; j++) fs[(size_t)v * NW + j] = 0;
     }
   }
-  if (mode != 0) {
-    bu_level<NW, SIGMA>(e, L);
-    return;
+  if (e.stack) {  // lanes of every stack entry of this level's queue
+    const int *Q = qptr(e, L);
+    const unsigned qn = e.ctrl->qlen[slot_of(L)];
+    const u64 *fc = frs(e, L);
+    const long long b = e.ep[L];
+    for (unsigned i = gt; i < qn; i += nt) {
+      int v = Q[i];
+#pragma unroll
+      for (int j = 0; j < NW; j++) e.smask[((size_t)b + i) * NW + j] = fc[(size_t)v * NW + j];
+    }
   }
-  if (VAR == V_SHUFFLE) td_shuffle<NW>(e, L);
-  else if (VAR == V_WARP) td_warp<NW>(e, L);
-  else if (VAR == V_CTA_WARP) td_cta_warp<NW>(e, L);
-  else if (VAR == V_SCAN) td_scan<NW>(e, L);
-  else if (VAR == V_EDGE_PARALLEL) td_edge_parallel<NW>(e, L);
-  else td_linear<NW>(e, L);
+  Pusher pb{s_pb[wave_id()], 0};
+  Acc a;
+  if (mode != 0) {
+    bu_level<NW, SIGMA>(e, L, pb, a);
+  } else {
+    if (VAR == V_SHUFFLE) td_shuffle<NW>(e, L, pb);
+    else if (VAR == V_WARP) td_warp<NW>(e, L, pb);
+    else if (VAR == V_CTA_WARP) td_cta_warp<NW>(e, L, pb);
+    else if (VAR == V_SCAN) td_scan<NW>(e, L, pb);
+    else if (VAR == V_EDGE_PARALLEL) td_edge_parallel<NW>(e, L, pb);
+    else td_linear<NW>(e, L);
+  }
+  pb_flush<NW>(e, L, pb);
+  write_acct<NW>(e, 0, a);
 }
 
 template <int NW, bool SIGMA>
@@ -122,14 +135,22 @@ __global__ __launch_bounds__(1024) void k_finalize(Eng<NW> e) {
   if (e.ctrl->done) return;
   const int L = e.ctrl->level;
   const int mode = e.ctrl->mode;
-  if (mode == 0) td_finalize<NW, SIGMA>(e, L);
-  hub_segments<NW, SIGMA>(e, L, mode);
+  Acc a;
+  if (mode == 0) td_finalize<NW, SIGMA>(e, L, a);
+  hub_segments<NW, SIGMA>(e, L, mode, a);
+  write_acct<NW>(e, 1, a);
 }
 
 template <int NW>
 __global__ __launch_bounds__(256) void k_hub_commit(Eng<NW> e) {
+  __shared__ int s_pb[16][kPB];
   if (e.ctrl->done) return;
-  hub_commit<NW>(e, e.ctrl->level, e.ctrl->mode);
+  const int L = e.ctrl->level;
+  Pusher pb{s_pb[wave_id()], 0};
+  Acc a;
+  hub_commit<NW>(e, L, e.ctrl->mode, pb, a);
+  pb_flush<NW>(e, L, pb);
+  write_acct<NW>(e, 2, a);
 }
 
 // Hubs take their path counts by atomic adds, so their rows start at zero.
@@ -142,15 +163,42 @@ __global__ void k_zero_hub_rows(Eng<NW> e) {
   }
 }
 
+// One workgroup: sum the level's accounting slots, then terminate or rotate
+// the queues and choose the next level's direction.
 template <int NW>
-__global__ void k_control(Eng<NW> e) {
-  if (threadIdx.x != 0) return;
+__global__ __launch_bounds__(256) void k_control(Eng<NW> e) {
+  __shared__ u64 s_red[4][4];
   Ctrl *c = e.ctrl;
   if (c->done) return;
+  u64 t[4] = {0, 0, 0, 0};
+  for (int kid = 0; kid < 3; kid++)
+    for (int b = threadIdx.x; b < e.grid_k[kid]; b += blockDim.x) {
+      const u64 *slot = e.acct + ((size_t)kid * kAcctMax + b) * 4;
+      t[0] += slot[0];
+      t[1] += slot[1];
+      t[2] += slot[2];
+      t[3] |= slot[3];
+    }
+  for (int o = 32; o >= 1; o >>= 1)
+    for (int i = 0; i < 4; i++) {
+      u64 y = __shfl_xor(t[i], o, kWave);
+      t[i] = i == 3 ? (t[i] | y) : t[i] + y;
+    }
+  if (lane_id() == 0)
+    for (int i = 0; i < 4; i++) s_red[wave_id()][i] = t[i];
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int i = 0; i < 3; i++) {
+    t[i] = 0;
+    for (int w = 0; w < (int)blockDim.x / kWave; w++) t[i] += s_red[w][i];
+  }
+  t[3] = 0;
+  for (int w = 0; w < (int)blockDim.x / kWave; w++) t[3] |= s_red[w][3];
   const int L = c->level;
   const unsigned qn = c->qlen[slot_of(L + 1)];
-  c->unvis_edges -= c->unvis_dec;
-  c->unvis_dec = 0;
+  c->unvis_edges -= t[1];
+  c->dag += t[2];
+  if (t[3]) c->overflow = 1;
   if (c->mode == 0)
     c->td_levels++;
   else
@@ -164,11 +212,11 @@ __global__ void k_control(Eng<NW> e) {
     e.ep[L + 2] = e.ep[L + 1] + qn;
     c->S_len = e.ep[L + 2];
   }
+  c->fr_edges[slot_of(L + 1)] = t[0];
   c->level = L + 1;
   c->qlen[slot_of(L + 2)] = 0;
-  c->fr_edges[slot_of(L + 2)] = 0;
   for (int i = 0; i < 4; i++) c->work[i] = 0;
-  const u64 mf = c->fr_edges[slot_of(L + 1)];
+  const u64 mf = t[0];
   const u64 mu = c->unvis_edges;
   int mode = c->mode;
   if (c->direction == GU_DIR_TOPDOWN)
@@ -705,6 +753,7 @@ struct Workspace {
   bool stack = false;
   u64 *visit = nullptr, *fr = nullptr, *svm = nullptr, *smask = nullptr, *blm = nullptr;
   u64 *hub_found = nullptr;
+  u64 *acct = nullptr;
   int *q = nullptr, *tag = nullptr, *dvm = nullptr, *off = nullptr, *srcs = nullptr, *btag = nullptr;
   float *dl = nullptr;
   float2 *part = nullptr;
@@ -713,7 +762,7 @@ struct Workspace {
   size_t S_cap = 0;   // entries of q in stack mode
   size_t bytes = 0;
   void release() {
-    void *ps[] = {visit, fr, svm, smask, blm, hub_found, q, tag, dvm, off, srcs, btag, dl, part, ep, tsum, bwork};
+    void *ps[] = {visit, fr, svm, smask, blm, hub_found, acct, q, tag, dvm, off, srcs, btag, dl, part, ep, tsum, bwork};
     for (void *p : ps)
       if (p) (void)hipFree(p);
     *this = Workspace();
@@ -824,6 +873,7 @@ int ensure_ws(gu_graph *g, int NW, bool stack, size_t want_S, hipStream_t st) {
     GU_HIP(dalloc(&w.srcs, 256));
     GU_HIP(dalloc(&w.ep, n + 3));
     GU_HIP(dalloc(&w.hub_found, (size_t)std::max(g->nhubs, 1) * NW));
+    GU_HIP(dalloc(&w.acct, (size_t)3 * kAcctMax * 4));
     w.bytes = n * NW * 8 * 4 + n * 4 * 2 + n * WL * 12 + (n + 8) * 8 + 1024 + (n + 3) * 8;
     w.S_cap = 3 * n;
     GU_HIP(dalloc(&w.q, w.S_cap));
@@ -902,6 +952,7 @@ Eng<NW> make_eng(gu_graph *g, int nb, bool stack, bool sigma) {
   e.nhubs = g->nhubs;
   e.hub_deg = kHubDeg;
   e.hub_found = w.hub_found;
+  e.acct = w.acct;
   for (int j = 0; j < NW; j++) {
     int lanes = std::min(std::max(nb - j * kWL, 0), kWL);
     e.all[j] = lanes == 64 ? ~0ull : ((1ull << lanes) - 1);
@@ -919,16 +970,16 @@ void launch_level(gu_graph *g, const Eng<NW> &e, int variant, int block, int gri
     k_fscan_apply<NW><<<grid, 256, 0, st>>>(e);
   }
   switch (variant) {
-    case V_LINEAR: k_expand<NW, V_LINEAR, SIGMA><<<grid, block, 0, st>>>(e); break;
-    case V_EDGE_PARALLEL: k_expand<NW, V_EDGE_PARALLEL, SIGMA><<<grid, block, 0, st>>>(e); break;
-    case V_WARP: k_expand<NW, V_WARP, SIGMA><<<grid, block, 0, st>>>(e); break;
-    case V_SCAN: k_expand<NW, V_SCAN, SIGMA><<<grid, block, 0, st>>>(e); break;
-    case V_CTA_WARP: k_expand<NW, V_CTA_WARP, SIGMA><<<grid, block, 0, st>>>(e); break;
-    default: k_expand<NW, V_SHUFFLE, SIGMA><<<grid, block, 0, st>>>(e); break;
+    case V_LINEAR: k_expand<NW, V_LINEAR, SIGMA><<<e.grid_k[0], block, 0, st>>>(e); break;
+    case V_EDGE_PARALLEL: k_expand<NW, V_EDGE_PARALLEL, SIGMA><<<e.grid_k[0], block, 0, st>>>(e); break;
+    case V_WARP: k_expand<NW, V_WARP, SIGMA><<<e.grid_k[0], block, 0, st>>>(e); break;
+    case V_SCAN: k_expand<NW, V_SCAN, SIGMA><<<e.grid_k[0], block, 0, st>>>(e); break;
+    case V_CTA_WARP: k_expand<NW, V_CTA_WARP, SIGMA><<<e.grid_k[0], block, 0, st>>>(e); break;
+    default: k_expand<NW, V_SHUFFLE, SIGMA><<<e.grid_k[0], block, 0, st>>>(e); break;
   }
-  k_finalize<NW, SIGMA><<<grid, 256, 0, st>>>(e);
-  if (e.nhubs) k_hub_commit<NW><<<std::max(1, std::min(grid, (e.nhubs + 3) / 4)), 256, 0, st>>>(e);
-  k_control<NW><<<1, 64, 0, st>>>(e);
+  k_finalize<NW, SIGMA><<<e.grid_k[1], 256, 0, st>>>(e);
+  if (e.grid_k[2]) k_hub_commit<NW><<<e.grid_k[2], 256, 0, st>>>(e);
+  k_control<NW><<<1, 256, 0, st>>>(e);
 }
 
 int default_alpha(const gu_config &c) { return c.alpha > 0 ? c.alpha : 4; }
@@ -965,7 +1016,13 @@ int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const g
   }
   k_seed<NW><<<1, 256, 0, st>>>(e);
   const int block = cfg.wg_size > 0 ? cfg.wg_size : 256;
-  const int grid = pick_grid(g, block >= 1024 ? 2 : (block >= 512 ? 4 : 8));
+  const int grid = std::min(kAcctMax, pick_grid(g, block >= 1024 ? 2 : (block >= 512 ? 4 : 8)));
+  auto set_grids = [&](Eng<NW> &x) {
+    x.grid_k[0] = grid;
+    x.grid_k[1] = std::min(kAcctMax, pick_grid(g, 8));
+    x.grid_k[2] = g->nhubs ? std::max(1, std::min(pick_grid(g, 2), (g->nhubs + 255) / 256)) : 0;
+  };
+  set_grids(e);
   int K = cfg.sync_every > 0 ? cfg.sync_every : 2;
   // stack mode reserves room for every level of a chunk, so its chunks stay short
   const int Kmax = cfg.sync_every > 0 ? cfg.sync_every : (stack ? 8 : 64);
@@ -976,6 +1033,7 @@ int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const g
       rc = ensure_ws(g, NW, true, (size_t)g->ctrl_h->S_len + (size_t)(K + 2) * n, st);
       if (rc) return rc;
       e = make_eng<NW>(g, nb, stack, sigma);
+      set_grids(e);
     }
     for (int i = 0; i < K; i++) {
       if (sigma)

@@ -78,6 +78,8 @@ struct Eng {
   int nhubs;
   int hub_deg;
   u64 *hub_found;      // [nhubs][NW] lanes found by the hub's segments
+  u64 *acct;           // [3][kAcctMax][4] per-workgroup level accounting
+  int grid_k[3];       // grid sizes of the three accounting kernels
   u64 all[NW];         // mask of active lanes per word
   int stack;           // keep every level's queue (Brandes backward needs it)
   int sigma;           // compute path counts
@@ -131,16 +133,69 @@ __device__ __forceinline__ bool discover(const Eng<NW> &e, int L, const u64 *f, 
   }
 }
 
-// Wave-aggregated push of w into the next level's queue: one atomic per wave.
+// Per-level accounting of one wave: out-degree sum of the vertices it
+// pushed (next frontier edges), of the vertices it completed (every lane
+// reached), shortest-path DAG edges pulled, path-count overflow.  Summed per
+// workgroup into a slot of e.acct (no shared counter), then by k_control.
+struct Acc {
+  u64 fe = 0, ud = 0, dag = 0;
+  int ovf = 0;
+};
+
+constexpr int kAcctMax = 8192;  // largest grid of an accounting kernel
+
+// Whole workgroup: every thread must call it once.
 template <int NW>
-__device__ __forceinline__ void push_wave(const Eng<NW> &e, int L, bool p, int w) {
+__device__ void write_acct(const Eng<NW> &e, int kid, const Acc &a) {
+  __shared__ u64 s_acc[16][4];
+  const int lane = lane_id(), wid = wave_id();
+  const bool ovf = ballot(a.ovf != 0) != 0;
+  if (lane == 0) {
+    s_acc[wid][0] = a.fe;
+    s_acc[wid][1] = a.ud;
+    s_acc[wid][2] = a.dag;
+    s_acc[wid][3] = ovf ? 1ull : 0ull;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u64 t[4] = {0, 0, 0, 0};
+    for (int w = 0; w < (int)blockDim.x / kWave; w++)
+      for (int i = 0; i < 4; i++) t[i] = i == 3 ? (t[i] | s_acc[w][i]) : t[i] + s_acc[w][i];
+    u64 *slot = e.acct + ((size_t)kid * kAcctMax + blockIdx.x) * 4;
+    for (int i = 0; i < 4; i++) slot[i] = t[i];
+  }
+}
+
+// Per-wave queue buffer in LDS: pushes are appended with ballot ranks and
+// written out, coalesced, behind one global atomic per flush.
+constexpr int kPB = 128;
+
+struct Pusher {
+  int *buf;  // this wave's kPB LDS entries
+  int cnt;   // wave-uniform fill
+};
+
+template <int NW>
+__device__ __forceinline__ void pb_flush(const Eng<NW> &e, int L, Pusher &pb) {
+  if (pb.cnt == 0) return;
+  unsigned base = 0;
+  if (lane_id() == 0) base = atomicAdd(&e.ctrl->qlen[slot_of(L + 1)], (unsigned)pb.cnt);
+  base = (unsigned)__builtin_amdgcn_readfirstlane((int)base);
+  int *Qn = qptr(e, L + 1);
+  __builtin_amdgcn_wave_barrier();
+  for (int i = lane_id(); i < pb.cnt; i += kWave) Qn[base + i] = pb.buf[i];
+  __builtin_amdgcn_wave_barrier();
+  pb.cnt = 0;
+}
+
+template <int NW>
+__device__ __forceinline__ void pb_push(const Eng<NW> &e, int L, Pusher &pb, bool p, int w) {
   uint64_t m = ballot(p);
   if (!m) return;
-  int leader = __ffsll((unsigned long long)m) - 1;
-  unsigned base = 0;
-  if (lane_id() == leader) base = atomicAdd(&e.ctrl->qlen[slot_of(L + 1)], (unsigned)__popcll(m));
-  base = (unsigned)__builtin_amdgcn_readlane((int)base, leader);
-  if (p) qptr(e, L + 1)[base + rank_in(m)] = w;
+  if (p) pb.buf[pb.cnt + rank_in(m)] = w;
+  pb.cnt += __popcll(m);
+  __builtin_amdgcn_wave_barrier();
+  if (pb.cnt > kPB - kWave) pb_flush<NW>(e, L, pb);
 }
 
 template <int NW>
@@ -152,14 +207,15 @@ __device__ __forceinline__ void load_mask(const u64 *p, u64 *f) {
 // Strip-mine the adjacency list [r0,r1) of a frontier vertex with masks f,
 // 64 entries per step (uniform trip count across the wave).
 template <int NW>
-__device__ __forceinline__ void strip_wave(const Eng<NW> &e, int L, const u64 *f, int r0, int r1) {
+__device__ __forceinline__ void strip_wave(const Eng<NW> &e, int L, Pusher &pb, const u64 *f, int r0,
+                                           int r1) {
   const int lane = lane_id();
   for (int e0 = r0; e0 < r1; e0 += kWave) {
     int ed = e0 + lane;
     bool valid = ed < r1;
     int w = valid ? e.C[ed] : 0;
     bool p = valid && discover<NW>(e, L, f, w);
-    push_wave<NW>(e, L, p, w);
+    pb_push<NW>(e, L, pb, p, w);
   }
 }
 
@@ -168,12 +224,12 @@ __device__ __forceinline__ void strip_wave(const Eng<NW> &e, int L, const u64 *f
 // shuffle_based.cuh:192-223: one wave per frontier vertex, waves fetch the
 // next queue entries dynamically; lanes stride the adjacency list.
 template <int NW>
-__device__ void td_shuffle(const Eng<NW> &e, int L) {
+__device__ void td_shuffle(const Eng<NW> &e, int L, Pusher &pb) {
   const int lane = lane_id();
   const int *Q = qptr(e, L);
   const unsigned qn = e.ctrl->qlen[slot_of(L)];
   const u64 *fc = frs(e, L);
-  constexpr unsigned CH = 4;
+  constexpr unsigned CH = 16;
   while (true) {
     unsigned base = 0;
     if (lane == 0) base = atomicAdd(&e.ctrl->work[0], CH);
@@ -184,7 +240,7 @@ __device__ void td_shuffle(const Eng<NW> &e, int L) {
       int v = Q[k];
       u64 f[NW];
       load_mask<NW>(fc + (size_t)v * NW, f);
-      strip_wave<NW>(e, L, f, e.R[v], e.R[v + 1]);
+      strip_wave<NW>(e, L, pb, f, e.R[v], e.R[v + 1]);
     }
   }
 }
@@ -193,7 +249,7 @@ __device__ void td_shuffle(const Eng<NW> &e, int L) {
 // repeatedly elects the highest lane that still has adjacency left
 // (race_and_resolve) and strip-mines that lane's list.
 template <int NW>
-__device__ void td_warp_core(const Eng<NW> &e, int L, int v, int r, int re) {
+__device__ void td_warp_core(const Eng<NW> &e, int L, Pusher &pb, int v, int r, int re) {
   const int lane = lane_id();
   const u64 *fc = frs(e, L);
   while (true) {
@@ -204,12 +260,12 @@ __device__ void td_warp_core(const Eng<NW> &e, int L, int v, int r, int re) {
     if (lane == win) r = re;  // the same lane cannot win twice
     u64 f[NW];
     load_mask<NW>(fc + (size_t)vw * NW, f);
-    strip_wave<NW>(e, L, f, rw, rew);
+    strip_wave<NW>(e, L, pb, f, rw, rew);
   }
 }
 
 template <int NW>
-__device__ void td_warp(const Eng<NW> &e, int L) {
+__device__ void td_warp(const Eng<NW> &e, int L, Pusher &pb) {
   const int lane = lane_id();
   const int *Q = qptr(e, L);
   const unsigned qn = e.ctrl->qlen[slot_of(L)];
@@ -221,7 +277,7 @@ __device__ void td_warp(const Eng<NW> &e, int L) {
     int v = have ? Q[k] : 0;
     int r = have ? e.R[v] : 0;
     int re = have ? e.R[v + 1] : 0;
-    td_warp_core<NW>(e, L, v, r, re);
+    td_warp_core<NW>(e, L, pb, v, r, re);
   }
 }
 
@@ -229,7 +285,7 @@ __device__ void td_warp(const Eng<NW> &e, int L) {
 // threshold are expanded by the whole workgroup, one at a time; the rest by
 // the wave election above.
 template <int NW>
-__device__ void td_cta_warp(const Eng<NW> &e, int L) {
+__device__ void td_cta_warp(const Eng<NW> &e, int L, Pusher &pb) {
   constexpr int kThreshold = 2048;  // CTA_warp.cu:44
   __shared__ int s_win, s_v, s_r, s_re;
   const int tid = threadIdx.x;
@@ -265,18 +321,18 @@ __device__ void td_cta_warp(const Eng<NW> &e, int L) {
         bool valid = ed < rew;
         int w = valid ? e.C[ed] : 0;
         bool p = valid && discover<NW>(e, L, f, w);
-        push_wave<NW>(e, L, p, w);
+        pb_push<NW>(e, L, pb, p, w);
       }
       __syncthreads();
     }
-    td_warp_core<NW>(e, L, v, r, re);
+    td_warp_core<NW>(e, L, pb, v, r, re);
   }
 }
 
 // scan_based.cu:105-169: a workgroup scans the degrees of its frontier
 // tile, then gathers the tile's edges blockDim at a time through LDS.
 template <int NW>
-__device__ void td_scan(const Eng<NW> &e, int L) {
+__device__ void td_scan(const Eng<NW> &e, int L, Pusher &pb) {
   __shared__ int s_e[1024];
   __shared__ int s_slot[1024];
   __shared__ int s_vert[1024];
@@ -314,7 +370,7 @@ __device__ void td_scan(const Eng<NW> &e, int L) {
         w = e.C[ed];
         p = discover<NW>(e, L, f, w);
       }
-      push_wave<NW>(e, L, p, w);
+      pb_push<NW>(e, L, pb, p, w);
       __syncthreads();
     }
   }
@@ -362,7 +418,7 @@ __device__ __forceinline__ int seg_search(const int *off, int lo, int hi, long l
 // indices, each workgroup taking an equal slice of edges (merge-path load
 // balancing; the reference scans all m edges every level instead).
 template <int NW>
-__device__ void td_edge_parallel(const Eng<NW> &e, int L) {
+__device__ void td_edge_parallel(const Eng<NW> &e, int L, Pusher &pb) {
   constexpr int IPT = 4;
   constexpr int kCap = 1024 * IPT + 1;
   __shared__ int s_off[kCap];
@@ -405,7 +461,7 @@ __device__ void td_edge_parallel(const Eng<NW> &e, int L) {
         w = e.C[e.R[v] + (int)(ed - e.off[vi])];
         p = discover<NW>(e, L, f, w);
       }
-      push_wave<NW>(e, L, p, w);
+      pb_push<NW>(e, L, pb, p, w);
     }
     __syncthreads();
   }
@@ -537,26 +593,17 @@ __device__ __forceinline__ bool full_mask(const Eng<NW> &e, const u64 *vo, const
 // Bottom-up level: every vertex that some lane has not reached pulls from
 // its neighbours on the frontier.  Waves take precomputed blocks of at most
 // 64 consecutive vertices with a bounded edge count (hubs are left to the
-// segment pass); a block's discoveries are pushed with one atomic.
+// segment pass), statically: the blocks are already edge-balanced.
 template <int NW, bool SIGMA>
-__device__ void bu_level(const Eng<NW> &e, int L) {
+__device__ void bu_level(const Eng<NW> &e, int L, Pusher &pb, Acc &a) {
   const int lane = lane_id();
   const u64 *fc = frs(e, L);
   u64 *fn = frs(e, L + 1);
-  int *Qn = qptr(e, L + 1);
-  u64 fe_acc = 0, ud_acc = 0, dag = 0;
-  int ovf = 0;
-  while (true) {
-    unsigned bi = 0;
-    if (lane == 0) bi = atomicAdd(&e.ctrl->work[1], 1u);
-    bi = bcast(bi);
-    if (bi >= (unsigned)e.nblocks) break;
+  const int waves = gridDim.x * (blockDim.x / kWave);
+  for (int bi = blockIdx.x * (blockDim.x / kWave) + wave_id(); bi < e.nblocks; bi += waves) {
     const int2 blk = e.blocks[bi];
     const int nblk = blk.y - blk.x;
     uint64_t disc = 0;
-    u64 mine[NW];
-#pragma unroll
-    for (int j = 0; j < NW; j++) mine[j] = 0;
     for (int i = 0; i < nblk; i++) {
       const int w = blk.x + i;
       const int r0 = e.R[w], r1 = e.R[w + 1];
@@ -571,7 +618,7 @@ __device__ void bu_level(const Eng<NW> &e, int L) {
       }
       if (!any) continue;
       u64 found[NW], acc[NW];
-      pull<NW, SIGMA>(e, fc, r0, r1, tgt, found, acc, &ovf, &dag);
+      pull<NW, SIGMA>(e, fc, r0, r1, tgt, found, acc, &a.ovf, &a.dag);
       bool anyn = false;
 #pragma unroll
       for (int j = 0; j < NW; j++) anyn |= found[j] != 0;
@@ -581,58 +628,34 @@ __device__ void bu_level(const Eng<NW> &e, int L) {
 #pragma unroll
         for (int j = 0; j < NW; j++) fn[(size_t)w * NW + j] = found[j];
       }
-      if (lane == i) {
-#pragma unroll
-        for (int j = 0; j < NW; j++) mine[j] = found[j];
-      }
       disc |= 1ull << i;
-      fe_acc += (u64)(r1 - r0);
-      if (full_mask<NW>(e, vo, found)) ud_acc += (u64)(r1 - r0);
+      a.fe += (u64)(r1 - r0);
+      if (full_mask<NW>(e, vo, found)) a.ud += (u64)(r1 - r0);
     }
-    if (disc) {
-      unsigned base = 0;
-      if (lane == 0) base = atomicAdd(&e.ctrl->qlen[slot_of(L + 1)], (unsigned)__popcll(disc));
-      base = bcast(base);
-      if ((disc >> lane) & 1ull) {
-        unsigned pos = base + (unsigned)rank_in(disc);
-        Qn[pos] = blk.x + lane;
-        if (e.stack) {
-#pragma unroll
-          for (int j = 0; j < NW; j++) e.smask[((size_t)e.ep[L + 1] + pos) * NW + j] = mine[j];
-        }
-      }
-    }
-  }
-  if (ballot(ovf != 0) && lane == 0) atomicOr(&e.ctrl->overflow, 1);
-  if (lane == 0) {
-    if (fe_acc) atomicAdd(&e.ctrl->fr_edges[slot_of(L + 1)], fe_acc);
-    if (ud_acc) atomicAdd(&e.ctrl->unvis_dec, ud_acc);
-    if (dag) atomicAdd(&e.ctrl->dag, dag);
+    pb_push<NW>(e, L, pb, (disc >> lane) & 1ull, blk.x + lane);
   }
 }
 
 // Top-down finalize: for every non-hub vertex pushed this level, commit its
 // new lanes and (with path counts) pull sigma from its frontier neighbours.
-// Hubs only record their stack lanes here; their pull runs in segments.
+// Waves take chunks of the queue in a fixed interleave; hubs are committed
+// by the hub pass.
 template <int NW, bool SIGMA>
-__device__ void td_finalize(const Eng<NW> &e, int L) {
-  constexpr int CH = 16;
+__device__ void td_finalize(const Eng<NW> &e, int L, Acc &a) {
+  constexpr int CH = 8;
   const int lane = lane_id();
   const int *Qn = qptr(e, L + 1);
   const unsigned qn = e.ctrl->qlen[slot_of(L + 1)];
   const u64 *fc = frs(e, L);
   const u64 *fn = frs(e, L + 1);
-  u64 fe_acc = 0, ud_acc = 0, dag = 0;
-  int ovf = 0;
-  while (true) {
-    unsigned base = 0;
-    if (lane == 0) base = atomicAdd(&e.ctrl->work[2], (unsigned)CH);
-    base = bcast(base);
-    if (base >= qn) break;
+  const unsigned waves = gridDim.x * (blockDim.x / kWave);
+  for (unsigned base = (blockIdx.x * (blockDim.x / kWave) + wave_id()) * CH; base < qn; base += waves * CH) {
     const int cnt = qn - base < (unsigned)CH ? (int)(qn - base) : CH;
     const int wl = lane < cnt ? Qn[base + lane] : 0;
     for (int i = 0; i < cnt; i++) {
       const int w = readlane(wl, i);
+      const int r0 = e.R[w], r1 = e.R[w + 1];
+      if (r1 - r0 > e.hub_deg) continue;  // committed by the hub pass
       u64 vo[NW], nw[NW], acc[NW];
 #pragma unroll
       for (int j = 0; j < NW; j++) {
@@ -640,26 +663,14 @@ __device__ void td_finalize(const Eng<NW> &e, int L) {
         nw[j] = fn[(size_t)w * NW + j];
         acc[j] = 0;
       }
-      if (e.stack && lane == 0) {
-#pragma unroll
-        for (int j = 0; j < NW; j++) e.smask[((size_t)e.ep[L + 1] + base + i) * NW + j] = nw[j];
-      }
-      const int r0 = e.R[w], r1 = e.R[w + 1];
-      if (r1 - r0 > e.hub_deg) continue;  // committed by the hub pass
       if (SIGMA) {
         u64 found[NW];
-        pull<NW, true>(e, fc, r0, r1, nw, found, acc, &ovf, &dag);
+        pull<NW, true>(e, fc, r0, r1, nw, found, acc, &a.ovf, &a.dag);
       }
       commit<NW, SIGMA>(e, L, w, vo, nw, acc);
-      fe_acc += (u64)(r1 - r0);
-      if (full_mask<NW>(e, vo, nw)) ud_acc += (u64)(r1 - r0);
+      a.fe += (u64)(r1 - r0);
+      if (full_mask<NW>(e, vo, nw)) a.ud += (u64)(r1 - r0);
     }
-  }
-  if (ballot(ovf != 0) && lane == 0) atomicOr(&e.ctrl->overflow, 1);
-  if (lane == 0) {
-    if (fe_acc) atomicAdd(&e.ctrl->fr_edges[slot_of(L + 1)], fe_acc);
-    if (ud_acc) atomicAdd(&e.ctrl->unvis_dec, ud_acc);
-    if (dag) atomicAdd(&e.ctrl->dag, dag);
   }
 }
 
@@ -669,17 +680,12 @@ __device__ void td_finalize(const Eng<NW> &e, int L) {
 // counts with 64-bit atomics -- exact, so the sum does not depend on the
 // order of the segments.  Bottom-up also ORs the lanes it found.
 template <int NW, bool SIGMA>
-__device__ void hub_segments(const Eng<NW> &e, int L, int mode) {
+__device__ void hub_segments(const Eng<NW> &e, int L, int mode, Acc &a) {
   const int lane = lane_id();
   const u64 *fc = frs(e, L);
   const u64 *fn = frs(e, L + 1);
-  u64 dag = 0;
-  int ovf = 0;
-  while (true) {
-    unsigned si = 0;
-    if (lane == 0) si = atomicAdd(&e.ctrl->work[3], 1u);
-    si = bcast(si);
-    if (si >= (unsigned)e.nsegs) break;
+  const int waves = gridDim.x * (blockDim.x / kWave);
+  for (int si = blockIdx.x * (blockDim.x / kWave) + wave_id(); si < e.nsegs; si += waves) {
     const int4 sg = e.segs[si];
     const int w = sg.x;
     u64 tgt[NW];
@@ -691,7 +697,7 @@ __device__ void hub_segments(const Eng<NW> &e, int L, int mode) {
     }
     if (!any) continue;
     u64 found[NW], acc[NW];
-    pull<NW, SIGMA>(e, fc, sg.y, sg.z, tgt, found, acc, &ovf, &dag);
+    pull<NW, SIGMA>(e, fc, sg.y, sg.z, tgt, found, acc, &a.ovf, &a.dag);
     if (mode && lane == 0) {
 #pragma unroll
       for (int j = 0; j < NW; j++)
@@ -703,57 +709,63 @@ __device__ void hub_segments(const Eng<NW> &e, int L, int mode) {
       for (int j = 0; j < NW; j++) {
         if (acc[j] != 0) {
           u64 old = atomicAdd(&srow[j * kWL + lane], acc[j]);
-          ovf |= old + acc[j] < acc[j];
+          a.ovf |= old + acc[j] < acc[j];
         }
       }
     }
   }
-  if (ballot(ovf != 0) && lane == 0) atomicOr(&e.ctrl->overflow, 1);
-  if (lane == 0 && dag) atomicAdd(&e.ctrl->dag, dag);
 }
 
-// Hub commit: new lanes of every hub (bottom-up: found by its segments, then
-// pushed; top-down: already pushed by the expansion), distances, visit mask.
+// Hub commit, one lane per hub: new lanes (bottom-up: found by the hub's
+// segments, then pushed; top-down: already pushed by the expansion),
+// visit mask and distances.
 template <int NW>
-__device__ void hub_commit(const Eng<NW> &e, int L, int mode) {
+__device__ void hub_commit(const Eng<NW> &e, int L, int mode, Pusher &pb, Acc &a) {
   const int lane = lane_id();
   u64 *fn = frs(e, L + 1);
-  int *Qn = qptr(e, L + 1);
   const int waves = gridDim.x * (blockDim.x / kWave);
-  u64 fe_acc = 0, ud_acc = 0;
-  for (int h = blockIdx.x * (blockDim.x / kWave) + wave_id(); h < e.nhubs; h += waves) {
-    const int w = e.hubs[h];
+  for (int h0 = (blockIdx.x * (blockDim.x / kWave) + wave_id()) * kWave; h0 < e.nhubs; h0 += waves * kWave) {
+    const int h = h0 + lane;
+    const bool have = h < e.nhubs;
+    const int w = have ? e.hubs[h] : 0;
     u64 vo[NW], nw[NW];
     bool any = false;
 #pragma unroll
     for (int j = 0; j < NW; j++) {
-      vo[j] = e.visit[(size_t)w * NW + j];
-      nw[j] = mode ? e.hub_found[(size_t)h * NW + j] : fn[(size_t)w * NW + j];
+      vo[j] = have ? e.visit[(size_t)w * NW + j] : 0ull;
+      nw[j] = !have ? 0ull : (mode ? e.hub_found[(size_t)h * NW + j] : fn[(size_t)w * NW + j]);
       any |= nw[j] != 0;
+      if (have && mode) e.hub_found[(size_t)h * NW + j] = 0;
     }
-    if (mode && lane == 0) {
+    if (any) {
 #pragma unroll
-      for (int j = 0; j < NW; j++) e.hub_found[(size_t)h * NW + j] = 0;
-    }
-    if (!any) continue;
-    commit<NW, false>(e, L, w, vo, nw, nullptr);
-    if (mode && lane == 0) {
-#pragma unroll
-      for (int j = 0; j < NW; j++) fn[(size_t)w * NW + j] = nw[j];
-      unsigned pos = atomicAdd(&e.ctrl->qlen[slot_of(L + 1)], 1u);
-      Qn[pos] = w;
-      if (e.stack) {
-#pragma unroll
-        for (int j = 0; j < NW; j++) e.smask[((size_t)e.ep[L + 1] + pos) * NW + j] = nw[j];
+      for (int j = 0; j < NW; j++) {
+        e.visit[(size_t)w * NW + j] = vo[j] | nw[j];
+        if (mode) fn[(size_t)w * NW + j] = nw[j];
       }
     }
-    const u64 deg = (u64)(e.R[w + 1] - e.R[w]);
-    fe_acc += deg;
-    if (full_mask<NW>(e, vo, nw)) ud_acc += deg;
-  }
-  if (lane == 0) {
-    if (fe_acc) atomicAdd(&e.ctrl->fr_edges[slot_of(L + 1)], fe_acc);
-    if (ud_acc) atomicAdd(&e.ctrl->unvis_dec, ud_acc);
+    // distances: every lane writes its own entry of each committed hub row
+    uint64_t pm = ballot(any);
+    while (pm) {
+      const int k = __ffsll((unsigned long long)pm) - 1;
+      pm &= pm - 1;
+      const int wk = readlane(w, k);
+      int *drow = e.dvm + (size_t)wk * (kWL * NW);
+#pragma unroll
+      for (int j = 0; j < NW; j++) {
+        const u64 nk = readlane64(nw[j], k);
+        if ((nk >> lane) & 1ull) drow[j * kWL + lane] = L + 1;
+      }
+    }
+    if (mode) pb_push<NW>(e, L, pb, any, w);
+    u64 fe = any ? (u64)(e.R[w + 1] - e.R[w]) : 0ull;
+    u64 ud = (any && full_mask<NW>(e, vo, nw)) ? fe : 0ull;
+    for (int o = 32; o >= 1; o >>= 1) {
+      fe += __shfl_xor(fe, o, kWave);
+      ud += __shfl_xor(ud, o, kWave);
+    }
+    a.fe += fe;
+    a.ud += ud;
   }
 }
 
