@@ -87,7 +87,6 @@ __global__ void k_seed(Eng<NW> e) {
 // ---- one level ------------------------------------------------------------
 template <int NW, int VAR, bool SIGMA>
 __global__ __launch_bounds__(1024) void k_expand(Eng<NW> e) {
-  __shared__ int s_pb[16][kPB];
   if (e.ctrl->done) return;
   const int L = e.ctrl->level;
   const int mode = e.ctrl->mode;
@@ -114,10 +113,10 @@ __global__ __launch_bounds__(1024) void k_expand(Eng<NW> e) {
       for (int j = 0; j < NW; j++) e.smask[((size_t)b + i) * NW + j] = fc[(size_t)v * NW + j];
     }
   }
-  Pusher pb{s_pb[wave_id()], 0};
+  Pusher pb{wave_pushbuf<NW>(), 0};
   Acc a;
   if (mode != 0) {
-    bu_level<NW, SIGMA>(e, L, pb, a);
+    bu_level<NW, SIGMA>(e, L, pb, wave_group<NW>(), a);
   } else {
     if (VAR == V_SHUFFLE) td_shuffle<NW>(e, L, pb);
     else if (VAR == V_WARP) td_warp<NW>(e, L, pb);
@@ -136,17 +135,16 @@ __global__ __launch_bounds__(1024) void k_finalize(Eng<NW> e) {
   const int L = e.ctrl->level;
   const int mode = e.ctrl->mode;
   Acc a;
-  if (mode == 0) td_finalize<NW, SIGMA>(e, L, a);
+  if (mode == 0) td_finalize<NW, SIGMA>(e, L, wave_group<NW>(), a);
   hub_segments<NW, SIGMA>(e, L, mode, a);
   write_acct<NW>(e, 1, a);
 }
 
 template <int NW>
 __global__ __launch_bounds__(256) void k_hub_commit(Eng<NW> e) {
-  __shared__ int s_pb[16][kPB];
   if (e.ctrl->done) return;
   const int L = e.ctrl->level;
-  Pusher pb{s_pb[wave_id()], 0};
+  Pusher pb{wave_pushbuf<NW>(), 0};
   Acc a;
   hub_commit<NW>(e, L, e.ctrl->mode, pb, a);
   pb_flush<NW>(e, L, pb);
@@ -771,6 +769,26 @@ struct Workspace {
 
 }  // namespace
 
+// Degree-balanced work units of one adjacency (fixed per graph).
+struct Units {
+  int2 *blocks = nullptr;  // vertex blocks
+  int nblocks = 0;
+  int4 *segs = nullptr;    // hub adjacency segments (w, e0, e1, hub index)
+  int nsegs = 0;
+  int *hubs = nullptr;
+  int nhubs = 0;
+  int *seg0 = nullptr;     // [nhubs+1] first segment of each hub
+  bool owned = false;
+  void release() {
+    if (owned) {
+      void *ps[] = {blocks, segs, hubs, seg0};
+      for (void *p : ps)
+        if (p) (void)hipFree(p);
+    }
+    *this = Units();
+  }
+};
+
 struct gu_graph {
   int n = 0;
   long long m = 0;
@@ -783,14 +801,9 @@ struct gu_graph {
   Ctrl *ctrl_h = nullptr;     // pinned host mirror
   int *srcs_h = nullptr;      // pinned staging for batch sources
   int max_deg = 0;
-  // degree-balanced work units, fixed per graph
-  int2 *blocks = nullptr;     // bottom-up vertex blocks
-  int nblocks = 0;
-  int4 *segs = nullptr;       // hub adjacency segments
-  int nsegs = 0;
-  int *hubs = nullptr;
-  int nhubs = 0;
-  int *hub_seg0 = nullptr;    // [nhubs+1] first segment of each hub
+  int *RT = nullptr, *CT = nullptr;  // in-edge CSR of a directed graph
+  Units pu;                   // work units over in-edges (pulls)
+  Units ou;                   // work units over out-edges (Brandes backward)
   int tuned_variant = -1;     // autotuner cache
   int tuned_direction_alpha = 0;
 };
@@ -807,7 +820,7 @@ constexpr int kHubDeg = 1024;
 constexpr int kSeg = 1024;
 constexpr int kBlockEdges = 2048;
 
-int build_units(gu_graph *g, const int32_t *R) {
+int build_units(int n, const int32_t *R, Units *u) {
   std::vector<int2> blocks;
   std::vector<int4> segs;
   std::vector<int> hubs, seg0;
@@ -819,7 +832,7 @@ int build_units(gu_graph *g, const int32_t *R) {
     bcount = 0;
     bedges = 0;
   };
-  for (int v = 0; v < g->n; v++) {
+  for (int v = 0; v < n; v++) {
     const int d = R[v + 1] - R[v];
     if (d > kHubDeg) {
       close(v);
@@ -835,24 +848,36 @@ int build_units(gu_graph *g, const int32_t *R) {
     bcount++;
     bedges += d;
   }
-  close(g->n);
+  close(n);
   seg0.push_back((int)segs.size());
-  g->nblocks = (int)blocks.size();
-  g->nsegs = (int)segs.size();
-  g->nhubs = (int)hubs.size();
-  if (dalloc(&g->blocks, blocks.size()) != hipSuccess || dalloc(&g->segs, segs.size()) != hipSuccess ||
-      dalloc(&g->hubs, hubs.size()) != hipSuccess || dalloc(&g->hub_seg0, seg0.size()) != hipSuccess)
+  u->owned = true;
+  u->nblocks = (int)blocks.size();
+  u->nsegs = (int)segs.size();
+  u->nhubs = (int)hubs.size();
+  if (dalloc(&u->blocks, blocks.size()) != hipSuccess || dalloc(&u->segs, segs.size()) != hipSuccess ||
+      dalloc(&u->hubs, hubs.size()) != hipSuccess || dalloc(&u->seg0, seg0.size()) != hipSuccess)
     return GU_ERR_NOMEM;
-  if (hipMemcpy(g->hub_seg0, seg0.data(), sizeof(int) * seg0.size(), hipMemcpyHostToDevice) != hipSuccess)
-    return GU_ERR_HIP;
-  if ((!blocks.empty() && hipMemcpy(g->blocks, blocks.data(), sizeof(int2) * blocks.size(),
+  if ((!blocks.empty() && hipMemcpy(u->blocks, blocks.data(), sizeof(int2) * blocks.size(),
                                     hipMemcpyHostToDevice) != hipSuccess) ||
-      (!segs.empty() && hipMemcpy(g->segs, segs.data(), sizeof(int4) * segs.size(),
+      (!segs.empty() && hipMemcpy(u->segs, segs.data(), sizeof(int4) * segs.size(),
                                   hipMemcpyHostToDevice) != hipSuccess) ||
-      (!hubs.empty() && hipMemcpy(g->hubs, hubs.data(), sizeof(int) * hubs.size(),
-                                  hipMemcpyHostToDevice) != hipSuccess))
+      (!hubs.empty() && hipMemcpy(u->hubs, hubs.data(), sizeof(int) * hubs.size(),
+                                  hipMemcpyHostToDevice) != hipSuccess) ||
+      hipMemcpy(u->seg0, seg0.data(), sizeof(int) * seg0.size(), hipMemcpyHostToDevice) != hipSuccess)
     return GU_ERR_HIP;
   return GU_OK;
+}
+
+// In-edge CSR (the transpose) of a directed graph, built on the host.
+void transpose_csr(int n, long long m, const int32_t *R, const int32_t *C, std::vector<int32_t> &RT,
+                   std::vector<int32_t> &CT) {
+  RT.assign((size_t)n + 1, 0);
+  for (long long k = 0; k < m; k++) RT[(size_t)C[k] + 1]++;
+  for (int v = 0; v < n; v++) RT[(size_t)v + 1] += RT[v];
+  std::vector<int32_t> pos(RT.begin(), RT.end() - 1);
+  CT.assign((size_t)m, 0);
+  for (int v = 0; v < n; v++)
+    for (int k = R[v]; k < R[v + 1]; k++) CT[(size_t)pos[C[k]]++] = v;
 }
 
 int ensure_ws(gu_graph *g, int NW, bool stack, size_t want_S, hipStream_t st) {
@@ -872,7 +897,7 @@ int ensure_ws(gu_graph *g, int NW, bool stack, size_t want_S, hipStream_t st) {
     GU_HIP(dalloc(&w.tsum, n + 8));
     GU_HIP(dalloc(&w.srcs, 256));
     GU_HIP(dalloc(&w.ep, n + 3));
-    GU_HIP(dalloc(&w.hub_found, (size_t)std::max(g->nhubs, 1) * NW));
+    GU_HIP(dalloc(&w.hub_found, (size_t)std::max(g->pu.nhubs, 1) * NW));
     GU_HIP(dalloc(&w.acct, (size_t)3 * kAcctMax * 4));
     w.bytes = n * NW * 8 * 4 + n * 4 * 2 + n * WL * 12 + (n + 8) * 8 + 1024 + (n + 3) * 8;
     w.S_cap = 3 * n;
@@ -885,7 +910,7 @@ int ensure_ws(gu_graph *g, int NW, bool stack, size_t want_S, hipStream_t st) {
       GU_HIP(dalloc(&w.btag, 2 * n));
       GU_HIP(dalloc(&w.blm, 2 * n * NW));
       GU_HIP(dalloc(&w.bwork, n + 3));
-      GU_HIP(dalloc(&w.part, (size_t)std::max(g->nsegs, 1) * WL));
+      GU_HIP(dalloc(&w.part, (size_t)std::max(g->ou.nsegs, 1) * WL));
       w.bytes += w.S_cap * NW * 8 + n * WL * 4 + 2 * n * 4 + 2 * n * NW * 8 + (n + 3) * 4;
     }
   }
@@ -944,12 +969,14 @@ Eng<NW> make_eng(gu_graph *g, int nb, bool stack, bool sigma) {
   e.off = w.off;
   e.tsum = w.tsum;
   e.srcs = w.srcs;
-  e.blocks = g->blocks;
-  e.nblocks = g->nblocks;
-  e.segs = g->segs;
-  e.nsegs = g->nsegs;
-  e.hubs = g->hubs;
-  e.nhubs = g->nhubs;
+  e.RP = g->RT ? g->RT : g->R;
+  e.CP = g->CT ? g->CT : g->C;
+  e.blocks = g->pu.blocks;
+  e.nblocks = g->pu.nblocks;
+  e.segs = g->pu.segs;
+  e.nsegs = g->pu.nsegs;
+  e.hubs = g->pu.hubs;
+  e.nhubs = g->pu.nhubs;
   e.hub_deg = kHubDeg;
   e.hub_found = w.hub_found;
   e.acct = w.acct;
@@ -969,16 +996,17 @@ void launch_level(gu_graph *g, const Eng<NW> &e, int variant, int block, int gri
     k_fscan_tiles<NW><<<1, 256, 0, st>>>(e);
     k_fscan_apply<NW><<<grid, 256, 0, st>>>(e);
   }
+  const size_t lx = level_lds_bytes<NW>(block / kWave), l256 = level_lds_bytes<NW>(4);
   switch (variant) {
-    case V_LINEAR: k_expand<NW, V_LINEAR, SIGMA><<<e.grid_k[0], block, 0, st>>>(e); break;
-    case V_EDGE_PARALLEL: k_expand<NW, V_EDGE_PARALLEL, SIGMA><<<e.grid_k[0], block, 0, st>>>(e); break;
-    case V_WARP: k_expand<NW, V_WARP, SIGMA><<<e.grid_k[0], block, 0, st>>>(e); break;
-    case V_SCAN: k_expand<NW, V_SCAN, SIGMA><<<e.grid_k[0], block, 0, st>>>(e); break;
-    case V_CTA_WARP: k_expand<NW, V_CTA_WARP, SIGMA><<<e.grid_k[0], block, 0, st>>>(e); break;
-    default: k_expand<NW, V_SHUFFLE, SIGMA><<<e.grid_k[0], block, 0, st>>>(e); break;
+    case V_LINEAR: k_expand<NW, V_LINEAR, SIGMA><<<e.grid_k[0], block, lx, st>>>(e); break;
+    case V_EDGE_PARALLEL: k_expand<NW, V_EDGE_PARALLEL, SIGMA><<<e.grid_k[0], block, lx, st>>>(e); break;
+    case V_WARP: k_expand<NW, V_WARP, SIGMA><<<e.grid_k[0], block, lx, st>>>(e); break;
+    case V_SCAN: k_expand<NW, V_SCAN, SIGMA><<<e.grid_k[0], block, lx, st>>>(e); break;
+    case V_CTA_WARP: k_expand<NW, V_CTA_WARP, SIGMA><<<e.grid_k[0], block, lx, st>>>(e); break;
+    default: k_expand<NW, V_SHUFFLE, SIGMA><<<e.grid_k[0], block, lx, st>>>(e); break;
   }
-  k_finalize<NW, SIGMA><<<e.grid_k[1], 256, 0, st>>>(e);
-  if (e.grid_k[2]) k_hub_commit<NW><<<e.grid_k[2], 256, 0, st>>>(e);
+  k_finalize<NW, SIGMA><<<e.grid_k[1], 256, l256, st>>>(e);
+  if (e.grid_k[2]) k_hub_commit<NW><<<e.grid_k[2], 256, l256, st>>>(e);
   k_control<NW><<<1, 256, 0, st>>>(e);
 }
 
@@ -1010,9 +1038,9 @@ int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const g
   *g->ctrl_h = c0;
   GU_HIP(hipMemcpyAsync(g->ctrl, g->ctrl_h, sizeof(Ctrl), hipMemcpyHostToDevice, st));
   Eng<NW> e = make_eng<NW>(g, nb, stack, sigma);
-  if (g->nhubs) {
-    k_fill_u64<<<1, 256, 0, st>>>(w.hub_found, (size_t)g->nhubs * NW, 0ull);
-    if (sigma) k_zero_hub_rows<NW><<<std::max(1, std::min(fgrid, (g->nhubs + 3) / 4)), 256, 0, st>>>(e);
+  if (g->pu.nhubs) {
+    k_fill_u64<<<1, 256, 0, st>>>(w.hub_found, (size_t)g->pu.nhubs * NW, 0ull);
+    if (sigma) k_zero_hub_rows<NW><<<std::max(1, std::min(fgrid, (g->pu.nhubs + 3) / 4)), 256, 0, st>>>(e);
   }
   k_seed<NW><<<1, 256, 0, st>>>(e);
   const int block = cfg.wg_size > 0 ? cfg.wg_size : 256;
@@ -1020,7 +1048,7 @@ int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const g
   auto set_grids = [&](Eng<NW> &x) {
     x.grid_k[0] = grid;
     x.grid_k[1] = std::min(kAcctMax, pick_grid(g, 8));
-    x.grid_k[2] = g->nhubs ? std::max(1, std::min(pick_grid(g, 2), (g->nhubs + 255) / 256)) : 0;
+    x.grid_k[2] = g->pu.nhubs ? std::max(1, std::min(pick_grid(g, 2), (g->pu.nhubs + 255) / 256)) : 0;
   };
   set_grids(e);
   int K = cfg.sync_every > 0 ? cfg.sync_every : 2;
@@ -1055,13 +1083,19 @@ int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const g
   if (job == JOB_BC) {
     k_fill_i32<<<fgrid, 256, 0, st>>>(w.btag, 2 * n, -1);
     k_fill_i32<<<1, 256, 0, st>>>((int *)w.bwork, (size_t)last + 2, 0);
-    Bw b{w.dl, w.btag, w.blm, w.bwork, w.part, g->hub_seg0};
-    const int hgrid = std::max(1, std::min(pick_grid(g, 8), (g->nsegs + 3) / 4));
+    // the backward walks out-edges: hubs and segments of the out-edge units
+    Eng<NW> eb = e;
+    eb.segs = g->ou.segs;
+    eb.nsegs = g->ou.nsegs;
+    eb.hubs = g->ou.hubs;
+    eb.nhubs = g->ou.nhubs;
+    Bw b{w.dl, w.btag, w.blm, w.bwork, w.part, g->ou.seg0};
+    const int hgrid = std::max(1, std::min(pick_grid(g, 8), (g->ou.nsegs + 3) / 4));
     for (int L = last; L >= 1; L--) {
-      k_backward<NW><<<pick_grid(g, 8), 256, 0, st>>>(e, b, L);
-      if (g->nhubs) {
-        k_bw_hub_segs<NW><<<hgrid, 256, 0, st>>>(e, b, L);
-        k_bw_hub_combine<NW><<<std::max(1, std::min(hgrid, (g->nhubs + 3) / 4)), 256, 0, st>>>(e, b, L);
+      k_backward<NW><<<pick_grid(g, 8), 256, 0, st>>>(eb, b, L);
+      if (g->ou.nhubs) {
+        k_bw_hub_segs<NW><<<hgrid, 256, 0, st>>>(eb, b, L);
+        k_bw_hub_combine<NW><<<std::max(1, std::min(hgrid, (g->ou.nhubs + 3) / 4)), 256, 0, st>>>(eb, b, L);
       }
     }
     k_bc_out<NW><<<pick_grid(g, 8), 256, 0, st>>>(e, w.dl, out.delta, out.bc);
@@ -1338,7 +1372,25 @@ int gu_graph_create(const int32_t *R, const int32_t *C, const int32_t *F, int32_
       rc = GU_ERR_HIP;
       break;
     }
-    if ((rc = build_units(g, R)) != GU_OK) break;
+    if (directed) {
+      std::vector<int32_t> RT, CT;
+      transpose_csr(n, m, R, C, RT, CT);
+      if (dalloc(&g->RT, (size_t)n + 1) != hipSuccess || dalloc(&g->CT, (size_t)m) != hipSuccess) {
+        rc = GU_ERR_NOMEM;
+        break;
+      }
+      if (hipMemcpy(g->RT, RT.data(), sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice) != hipSuccess ||
+          (m > 0 && hipMemcpy(g->CT, CT.data(), sizeof(int) * (size_t)m, hipMemcpyHostToDevice) != hipSuccess)) {
+        rc = GU_ERR_HIP;
+        break;
+      }
+      if ((rc = build_units(n, RT.data(), &g->pu)) != GU_OK) break;
+      if ((rc = build_units(n, R, &g->ou)) != GU_OK) break;
+    } else {
+      if ((rc = build_units(n, R, &g->pu)) != GU_OK) break;
+      g->ou = g->pu;
+      g->ou.owned = false;
+    }
     if (F) {
       if (m > 0 && hipMemcpy(g->F, F, sizeof(int) * (size_t)m, hipMemcpyHostToDevice) != hipSuccess) {
         rc = GU_ERR_HIP;
@@ -1363,10 +1415,10 @@ int gu_graph_destroy(gu_graph_t g) {
   if (g->R) (void)hipFree(g->R);
   if (g->C) (void)hipFree(g->C);
   if (g->F) (void)hipFree(g->F);
-  if (g->blocks) (void)hipFree(g->blocks);
-  if (g->segs) (void)hipFree(g->segs);
-  if (g->hubs) (void)hipFree(g->hubs);
-  if (g->hub_seg0) (void)hipFree(g->hub_seg0);
+  g->pu.release();
+  g->ou.release();
+  if (g->RT) (void)hipFree(g->RT);
+  if (g->CT) (void)hipFree(g->CT);
   if (g->ctrl) (void)hipFree(g->ctrl);
   if (g->ctrl_h) (void)hipHostFree(g->ctrl_h);
   if (g->srcs_h) (void)hipHostFree(g->srcs_h);

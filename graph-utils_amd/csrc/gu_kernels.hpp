@@ -54,8 +54,10 @@ struct Ctrl {
 
 template <int NW>
 struct Eng {
-  const int *R;
+  const int *R;        // out-edges (CSR): top-down expansion, Brandes successors
   const int *C;
+  const int *RP;       // in-edges (the transpose; the same arrays when the
+  const int *CP;       //   graph is undirected): pulls from the frontier
   int n;
   int nb;              // active lanes
   u64 *visit;          // [n][NW]
@@ -534,7 +536,7 @@ __device__ __forceinline__ void pull(const Eng<NW> &e, const u64 *fc, int r0, in
   for (int e0 = r0; e0 < r1; e0 += kWave) {
     int ed = e0 + lane;
     bool valid = ed < r1;
-    int v = valid ? e.C[ed] : 0;
+    int v = valid ? e.CP[ed] : 0;
     u64 mv[NW];
     bool has = false;
 #pragma unroll
@@ -590,12 +592,182 @@ __device__ __forceinline__ bool full_mask(const Eng<NW> &e, const u64 *vo, const
   return f;
 }
 
+// Group pull with path counts.  A wave takes up to 64 vertices (lane i owns
+// vertex w_i with target lanes tgt_i and in-edge range [r0_i, r0_i + deg_i);
+// deg_i = 0 drops the vertex).  The owned in-edge ranges are concatenated by
+// a wave prefix sum and walked 64 entries per step, each lane finding its
+// owner by binary search in LDS (load-balanced search), so low-degree
+// vertices share a step and vertices with nothing left to find cost nothing.
+// Predecessor rows are added in entry order, four loads in flight; when the
+// owner changes the finished vertex is committed.  BU: found lanes are OR-ed
+// per owner in LDS and the discovered vertices are pushed; TD: the target
+// lanes are the lanes pushed this level.
+template <int NW>
+struct GroupLds {
+  u64 tgt[kWave][NW];
+  u64 found[kWave][NW];
+  int off[kWave + 1];
+  int w[kWave];
+  int r0[kWave];
+};
+
+__device__ __forceinline__ int owner_of(const int *off, int t) {
+  int a = 0, b = kWave;  // largest i with off[i] <= t
+#pragma unroll
+  for (int it = 0; it < 6; it++) {
+    int mid = (a + b) >> 1;
+    if (off[mid] <= t) a = mid; else b = mid;
+  }
+  return a;
+}
+
+template <int NW, bool BU>
+__device__ void pull_group(const Eng<NW> &e, int L, int w_l, const u64 *tgt_l, int r0_l, int deg_l,
+                           GroupLds<NW> &g, Pusher &pb, Acc &a) {
+  const int lane = lane_id();
+  const int inc = wave_inclusive_sum(deg_l);
+  const int T = wave_total(inc);
+  if (T == 0) return;
+  g.off[lane] = inc - deg_l;
+  if (lane == kWave - 1) g.off[kWave] = T;
+  g.w[lane] = w_l;
+  g.r0[lane] = r0_l;
+#pragma unroll
+  for (int j = 0; j < NW; j++) {
+    g.tgt[lane][j] = tgt_l[j];
+    g.found[lane][j] = 0;
+  }
+  __builtin_amdgcn_wave_barrier();
+  const u64 *fc = frs(e, L);
+  u64 *fn = frs(e, L + 1);
+  int cur = -1;
+  uint64_t disc = 0;
+  u64 acc[NW];
+#pragma unroll
+  for (int j = 0; j < NW; j++) acc[j] = 0;
+
+  auto flush = [&](int slot) {
+    const int w = g.w[slot];
+    u64 vo[NW], nw[NW];
+#pragma unroll
+    for (int j = 0; j < NW; j++) {
+      nw[j] = BU ? g.found[slot][j] : g.tgt[slot][j];
+      vo[j] = e.visit[(size_t)w * NW + j];
+    }
+    commit<NW, true>(e, L, w, vo, nw, acc);
+    if (BU && lane == 0) {
+#pragma unroll
+      for (int j = 0; j < NW; j++) fn[(size_t)w * NW + j] = nw[j];
+    }
+    const u64 deg = (u64)(e.R[w + 1] - e.R[w]);
+    a.fe += deg;
+    if (full_mask<NW>(e, vo, nw)) a.ud += deg;
+    disc |= 1ull << slot;
+  };
+
+  int t = lane;
+  bool valid = t < T;
+  int own = valid ? owner_of(g.off, t) : 0;
+  int v = valid ? e.CP[g.r0[own] + t - g.off[own]] : 0;
+  for (int t0 = 0; t0 < T; t0 += kWave) {
+    u64 m[NW];
+    bool has = false;
+#pragma unroll
+    for (int j = 0; j < NW; j++) {
+      m[j] = valid ? (fc[(size_t)v * NW + j] & g.tgt[own][j]) : 0ull;
+      has |= m[j] != 0;
+    }
+    // start the next step's in-edge load before the row loads of this one
+    const int tn = t + kWave;
+    const bool validn = tn < T;
+    const int ownn = validn ? owner_of(g.off, tn) : 0;
+    const int vn = validn ? e.CP[g.r0[ownn] + tn - g.off[ownn]] : 0;
+    if (BU) {
+#pragma unroll
+      for (int j = 0; j < NW; j++)
+        if (m[j]) atomicOr(&g.found[own][j], m[j]);
+      __builtin_amdgcn_wave_barrier();
+    }
+    uint64_t pm = ballot(has);
+    while (pm) {
+      int k[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        if (pm) {
+          k[u] = __ffsll((unsigned long long)pm) - 1;
+          pm &= pm - 1;
+        } else {
+          k[u] = -1;
+        }
+      }
+      u64 x[4][NW];
+      int ow[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        ow[u] = -1;
+        if (k[u] >= 0) {
+          ow[u] = readlane(own, k[u]);
+          const int vk = readlane(v, k[u]);
+          const u64 *row = e.svm + (size_t)vk * (kWL * NW);
+#pragma unroll
+          for (int j = 0; j < NW; j++) {
+            const u64 mk = readlane64(m[j], k[u]);
+            a.dag += (u64)__popcll(mk);
+            x[u][j] = ((mk >> lane) & 1ull) ? row[j * kWL + lane] : 0ull;
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        if (ow[u] < 0) continue;
+        if (ow[u] != cur) {
+          if (cur >= 0) flush(cur);
+          cur = ow[u];
+#pragma unroll
+          for (int j = 0; j < NW; j++) acc[j] = 0;
+        }
+#pragma unroll
+        for (int j = 0; j < NW; j++) {
+          u64 sum = acc[j] + x[u][j];
+          a.ovf |= sum < x[u][j];
+          acc[j] = sum;
+        }
+      }
+    }
+    t = tn;
+    valid = validn;
+    own = ownn;
+    v = vn;
+  }
+  if (cur >= 0) flush(cur);
+  if (BU) pb_push<NW>(e, L, pb, (disc >> lane) & 1ull, g.w[lane]);
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Dynamic LDS of the level kernels: one GroupLds and one push buffer per
+// wave, sized by the launch.
+extern __shared__ __align__(16) unsigned char gu_dyn_lds[];
+
+template <int NW>
+__host__ __device__ constexpr size_t level_lds_bytes(int waves) {
+  return (size_t)waves * (sizeof(GroupLds<NW>) + kPB * sizeof(int));
+}
+template <int NW>
+__device__ __forceinline__ GroupLds<NW> &wave_group() {
+  return reinterpret_cast<GroupLds<NW> *>(gu_dyn_lds)[wave_id()];
+}
+template <int NW>
+__device__ __forceinline__ int *wave_pushbuf() {
+  const int waves = blockDim.x / kWave;
+  return reinterpret_cast<int *>(gu_dyn_lds + (size_t)waves * sizeof(GroupLds<NW>)) + wave_id() * kPB;
+}
+
 // Bottom-up level: every vertex that some lane has not reached pulls from
-// its neighbours on the frontier.  Waves take precomputed blocks of at most
-// 64 consecutive vertices with a bounded edge count (hubs are left to the
-// segment pass), statically: the blocks are already edge-balanced.
+// its in-neighbours on the frontier.  Waves take precomputed blocks of at
+// most 64 consecutive vertices with a bounded in-edge count (hubs are left
+// to the segment pass), statically: the blocks are already edge-balanced.
 template <int NW, bool SIGMA>
-__device__ void bu_level(const Eng<NW> &e, int L, Pusher &pb, Acc &a) {
+__device__ void bu_level(const Eng<NW> &e, int L, Pusher &pb, GroupLds<NW> &g, Acc &a) {
   const int lane = lane_id();
   const u64 *fc = frs(e, L);
   u64 *fn = frs(e, L + 1);
@@ -603,10 +775,27 @@ __device__ void bu_level(const Eng<NW> &e, int L, Pusher &pb, Acc &a) {
   for (int bi = blockIdx.x * (blockDim.x / kWave) + wave_id(); bi < e.nblocks; bi += waves) {
     const int2 blk = e.blocks[bi];
     const int nblk = blk.y - blk.x;
+    if (SIGMA) {
+      const bool have = lane < nblk;
+      const int w = blk.x + (have ? lane : 0);
+      u64 tgt[NW];
+      bool any = false;
+#pragma unroll
+      for (int j = 0; j < NW; j++) {
+        tgt[j] = have ? (e.all[j] & ~e.visit[(size_t)w * NW + j]) : 0ull;
+        any |= tgt[j] != 0;
+      }
+      const int r0 = have ? e.RP[w] : 0;
+      const int deg = any ? e.RP[w + 1] - r0 : 0;
+      pull_group<NW, true>(e, L, w, tgt, r0, deg, g, pb, a);
+      continue;
+    }
+    // distances only: one vertex at a time, stopping as soon as every
+    // target lane has been found
     uint64_t disc = 0;
     for (int i = 0; i < nblk; i++) {
       const int w = blk.x + i;
-      const int r0 = e.R[w], r1 = e.R[w + 1];
+      const int r0 = e.RP[w], r1 = e.RP[w + 1];
       if (r0 == r1) continue;
       u64 vo[NW], tgt[NW];
       bool any = false;
@@ -618,58 +807,67 @@ __device__ void bu_level(const Eng<NW> &e, int L, Pusher &pb, Acc &a) {
       }
       if (!any) continue;
       u64 found[NW], acc[NW];
-      pull<NW, SIGMA>(e, fc, r0, r1, tgt, found, acc, &a.ovf, &a.dag);
+      pull<NW, false>(e, fc, r0, r1, tgt, found, acc, &a.ovf, &a.dag);
       bool anyn = false;
 #pragma unroll
       for (int j = 0; j < NW; j++) anyn |= found[j] != 0;
       if (!anyn) continue;
-      commit<NW, SIGMA>(e, L, w, vo, found, acc);
+      commit<NW, false>(e, L, w, vo, found, acc);
       if (lane == 0) {
 #pragma unroll
         for (int j = 0; j < NW; j++) fn[(size_t)w * NW + j] = found[j];
       }
       disc |= 1ull << i;
-      a.fe += (u64)(r1 - r0);
-      if (full_mask<NW>(e, vo, found)) a.ud += (u64)(r1 - r0);
+      const u64 deg = (u64)(e.R[w + 1] - e.R[w]);
+      a.fe += deg;
+      if (full_mask<NW>(e, vo, found)) a.ud += deg;
     }
     pb_push<NW>(e, L, pb, (disc >> lane) & 1ull, blk.x + lane);
   }
 }
 
 // Top-down finalize: for every non-hub vertex pushed this level, commit its
-// new lanes and (with path counts) pull sigma from its frontier neighbours.
-// Waves take chunks of the queue in a fixed interleave; hubs are committed
-// by the hub pass.
+// new lanes and (with path counts) pull sigma from its frontier
+// in-neighbours.  Waves take 32-entry chunks of the queue in a fixed
+// interleave; hubs are committed by the hub pass.
 template <int NW, bool SIGMA>
-__device__ void td_finalize(const Eng<NW> &e, int L, Acc &a) {
-  constexpr int CH = 8;
+__device__ void td_finalize(const Eng<NW> &e, int L, GroupLds<NW> &g, Acc &a) {
+  constexpr int CH = 32;
   const int lane = lane_id();
   const int *Qn = qptr(e, L + 1);
   const unsigned qn = e.ctrl->qlen[slot_of(L + 1)];
-  const u64 *fc = frs(e, L);
   const u64 *fn = frs(e, L + 1);
   const unsigned waves = gridDim.x * (blockDim.x / kWave);
+  Pusher none{nullptr, 0};
   for (unsigned base = (blockIdx.x * (blockDim.x / kWave) + wave_id()) * CH; base < qn; base += waves * CH) {
     const int cnt = qn - base < (unsigned)CH ? (int)(qn - base) : CH;
-    const int wl = lane < cnt ? Qn[base + lane] : 0;
-    for (int i = 0; i < cnt; i++) {
-      const int w = readlane(wl, i);
-      const int r0 = e.R[w], r1 = e.R[w + 1];
-      if (r1 - r0 > e.hub_deg) continue;  // committed by the hub pass
-      u64 vo[NW], nw[NW], acc[NW];
+    const bool have = lane < cnt;
+    const int w = have ? Qn[base + lane] : 0;
+    const int r0 = have ? e.RP[w] : 0;
+    const int deg = have ? e.RP[w + 1] - r0 : 0;
+    const bool hub = deg > e.hub_deg;  // committed by the hub pass
+    u64 nw[NW];
+#pragma unroll
+    for (int j = 0; j < NW; j++) nw[j] = (have && !hub) ? fn[(size_t)w * NW + j] : 0ull;
+    if (SIGMA) {
+      pull_group<NW, false>(e, L, w, nw, r0, hub ? 0 : deg, g, none, a);
+      continue;
+    }
+    uint64_t todo = ballot(have && !hub);
+    while (todo) {
+      const int k = __ffsll((unsigned long long)todo) - 1;
+      todo &= todo - 1;
+      const int wk = readlane(w, k);
+      u64 vo[NW], nk[NW];
 #pragma unroll
       for (int j = 0; j < NW; j++) {
-        vo[j] = e.visit[(size_t)w * NW + j];
-        nw[j] = fn[(size_t)w * NW + j];
-        acc[j] = 0;
+        vo[j] = e.visit[(size_t)wk * NW + j];
+        nk[j] = readlane64(nw[j], k);
       }
-      if (SIGMA) {
-        u64 found[NW];
-        pull<NW, true>(e, fc, r0, r1, nw, found, acc, &a.ovf, &a.dag);
-      }
-      commit<NW, SIGMA>(e, L, w, vo, nw, acc);
-      a.fe += (u64)(r1 - r0);
-      if (full_mask<NW>(e, vo, nw)) a.ud += (u64)(r1 - r0);
+      commit<NW, false>(e, L, wk, vo, nk, nullptr);
+      const u64 d = (u64)(e.R[wk + 1] - e.R[wk]);
+      a.fe += d;
+      if (full_mask<NW>(e, vo, nk)) a.ud += d;
     }
   }
 }

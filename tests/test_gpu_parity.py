@@ -212,6 +212,35 @@ def test_bc_grid_deep(gu, orc, gpu):
     _check_bc(gu, orc, hg, src, 64, "grid24")
 
 
+def _directed_graph(gu, n=3000, arcs=20000, seed=12):
+    """Random directed graph plus an in-hub (vertex 0, in-degree ~n/2) and an
+    out-hub (vertex 1): pulls must use in-edges, expansion out-edges."""
+    rng = np.random.default_rng(seed)
+    src = rng.integers(0, n, arcs)
+    dst = rng.integers(0, n, arcs)
+    extra_src = np.concatenate([np.arange(2, n, 2), np.full(n // 2, 1)])
+    extra_dst = np.concatenate([np.zeros(len(range(2, n, 2)), dtype=np.int64),
+                                rng.integers(2, n, n // 2)])
+    src = np.concatenate([src, extra_src])
+    dst = np.concatenate([dst, extra_dst])
+    keep = src != dst
+    pairs = np.unique(np.stack([src[keep], dst[keep]], 1), axis=0)
+    R = np.zeros(n + 1, dtype=np.int32)
+    np.add.at(R, pairs[:, 0] + 1, 1)
+    R = np.cumsum(R).astype(np.int32)
+    C = pairs[:, 1].astype(np.int32)  # rows sorted by (src, dst)
+    return gu.HostGraph(R, C, directed=True)
+
+
+@pytest.mark.parametrize("direction", ["auto", "topdown", "bottomup"])
+@pytest.mark.parametrize("variant", ["edge_parallel", "warp", "shuffle"])
+@pytest.mark.parametrize("width", [64, 128])
+def test_directed_graph(gu, orc, gpu, direction, variant, width):
+    hg = _directed_graph(gu)
+    src = np.concatenate([[0, 1, 2], hg.sample_sources(width - 3, seed=13)]).astype(np.int32)
+    _search_and_compare(gu, orc, hg, src, variant, direction, width, "directed")
+
+
 LBS_CASES = {
     "small": [0, 3, 0, 5, 1, 0, 2],
     "zeros_ends": [0, 0, 4, 0, 0, 0, 7, 1, 0, 0],
