@@ -47,6 +47,9 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-variants", action="store_true", help="skip the per-variant sweep")
+    ap.add_argument("--calib-bytes", type=int, default=0,
+                    help="first run one device-to-device copy of this many bytes (a kernel of known "
+                         "traffic, to scale HBM counters of a profiler run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: multi-GPU source sharding is not part of this "
@@ -55,6 +58,14 @@ def main():
 
     gu = __graft_entry__.load_package()
     props = gu.choose_device(0)
+    if args.calib_bytes:
+        a = gu.DeviceBuffer(args.calib_bytes)
+        b = gu.DeviceBuffer(args.calib_bytes)
+        a.zero()
+        b.copy_from(a)
+        gu.synchronize()
+        a.close()
+        b.close()
     t0 = time.time()
     hg = gu.HostGraph.rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=20)
     gen_s = time.time() - t0

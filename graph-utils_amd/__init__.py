@@ -51,7 +51,7 @@ INT_MAX = 2147483647
 # Names the C ABI must export (checked by the CPU tests against include/*.h).
 EXPORTS = (
     "gu_choose_device", "gu_device_count", "gu_timer_start", "gu_timer_stop",
-    "gu_malloc", "gu_free", "gu_memcpy_h2d", "gu_memcpy_d2h", "gu_memset_d",
+    "gu_malloc", "gu_free", "gu_memcpy_h2d", "gu_memcpy_d2h", "gu_memcpy_d2d", "gu_memset_d",
     "gu_synchronize", "gu_strerror", "gu_graph_create", "gu_graph_destroy",
     "gu_graph_arrays", "gu_graph_device_bytes", "gu_multi_search",
     "gu_betweenness", "gu_diameter", "gu_lbs", "gu_parse_metis",
@@ -142,6 +142,7 @@ def lib():
         "gu_memcpy_h2d": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
         "gu_memcpy_d2h": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
         "gu_memset_d": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_size_t]),
+        "gu_memcpy_d2d": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
         "gu_synchronize": (ctypes.c_int, []),
         "gu_graph_create": (ctypes.c_int, [_P, _P, _P, ctypes.c_int32, ctypes.c_int64,
                                            ctypes.c_int32, ctypes.POINTER(_P)]),
@@ -265,6 +266,10 @@ class DeviceBuffer:
             src = ctypes.c_void_p(self.ptr.value + offset_items * dtype.itemsize)
             _check(lib().gu_memcpy_d2h(_ptr(out), src, out.nbytes), "gu_memcpy_d2h")
         return out
+
+    def copy_from(self, other, nbytes=None):
+        nb = self.nbytes if nbytes is None else int(nbytes)
+        _check(lib().gu_memcpy_d2d(self.ptr, other.ptr, nb), "gu_memcpy_d2d")
 
     def zero(self):
         _check(lib().gu_memset_d(self.ptr, 0, self.nbytes), "gu_memset_d")

@@ -136,7 +136,7 @@ __global__ __launch_bounds__(1024) void k_finalize(Eng<NW> e) {
   const int mode = e.ctrl->mode;
   Acc a;
   if (mode == 0) td_finalize<NW, SIGMA>(e, L, wave_group<NW>(), a);
-  hub_segments<NW, SIGMA>(e, L, mode, a);
+  hub_segments<NW, SIGMA>(e, L, mode, wave_group<NW>(), a);
   write_acct<NW>(e, 1, a);
 }
 
@@ -408,36 +408,37 @@ __global__ __launch_bounds__(256) void k_lbs_expand(const int *scan, long long S
 }
 
 // ---- outputs ----------------------------------------------------------------
-// Transpose vertex-major distances / path counts of the batch into the
-// caller's source-major rows; lanes that did not reach v get INT_MAX / 0.
-template <int NW>
-__global__ __launch_bounds__(256) void k_output(Eng<NW> e, int *dout, u64 *sout) {
-  __shared__ int td[64][65];
-  __shared__ u64 ts[64][65];
-  const int tid = threadIdx.x, T = blockDim.x;
-  const int col = tid % 64;
-  for (long long v0 = (long long)blockIdx.x * 64; v0 < e.n; v0 += (long long)gridDim.x * 64) {
-    for (int j = 0; j < NW; j++) {
-      if (j * kWL >= e.nb) break;
-      for (int rr = tid / 64; rr < 64; rr += T / 64) {
-        long long v = v0 + rr;
-        bool ok = false;
-        if (v < e.n) ok = (e.visit[(size_t)v * NW + j] >> col) & 1ull;
-        size_t idx = (size_t)v * (kWL * NW) + j * kWL + col;
-        if (dout) td[rr][col] = ok ? e.dvm[idx] : INT_MAX;
-        if (sout) ts[rr][col] = ok ? e.svm[idx] : 0ull;
-      }
-      __syncthreads();
-      for (int bb = tid / 64; bb < 64; bb += T / 64) {
-        int b = j * kWL + bb;
-        long long v = v0 + col;
-        if (b < e.nb && v < e.n) {
-          if (dout) dout[(size_t)b * e.n + v] = td[col][bb];
-          if (sout) sout[(size_t)b * e.n + v] = ts[col][bb];
-        }
-      }
-      __syncthreads();
+// Transpose one vertex-major per-lane array of the batch ([n][64*NW]) into the
+// caller's source-major rows ([nb][n]); lanes that did not reach v get
+// `fill`.  A workgroup moves 64 vertices x 64 lanes through LDS; every
+// thread keeps 16 loads in flight.
+template <int NW, class T>
+__global__ __launch_bounds__(256) void k_transpose_out(Eng<NW> e, const T *src, T *dst, T fill) {
+  __shared__ T tile[64][65];
+  const int tid = threadIdx.x;
+  const int col = tid & 63, row0 = tid >> 6;
+  const int words = (e.nb + kWL - 1) / kWL;
+  const long long ntiles = (((long long)e.n + 63) / 64) * words;
+  for (long long t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const long long v0 = (t / words) * 64;
+    const int j = (int)(t % words);
+    T x[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const long long v = v0 + row0 + 4 * k;
+      bool ok = v < e.n && ((e.visit[(size_t)v * NW + j] >> col) & 1ull);
+      x[k] = ok ? src[(size_t)v * (kWL * NW) + j * kWL + col] : fill;
     }
+#pragma unroll
+    for (int k = 0; k < 16; k++) tile[row0 + 4 * k][col] = x[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const int b = j * kWL + row0 + 4 * k;
+      const long long v = v0 + col;
+      if (b < e.nb && v < e.n) dst[(size_t)b * e.n + v] = tile[col][row0 + 4 * k];
+    }
+    __syncthreads();
   }
 }
 
@@ -1079,7 +1080,12 @@ int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const g
   }
   const int last = g->ctrl_h->last_level;
   if (job == JOB_BFS && (out.dist || out.sigma))
-    k_output<NW><<<pick_grid(g, 8), 256, 0, st>>>(e, out.dist, out.sigma);
+  {
+    const long long tiles = ((long long)g->n + 63) / 64 * ((nb + kWL - 1) / kWL);
+    const int og = (int)std::max(1LL, std::min<long long>(tiles, pick_grid(g, 8)));
+    if (out.dist) k_transpose_out<NW, int><<<og, 256, 0, st>>>(e, w.dvm, out.dist, INT_MAX);
+    if (out.sigma) k_transpose_out<NW, u64><<<og, 256, 0, st>>>(e, w.svm, out.sigma, 0ull);
+  }
   if (job == JOB_BC) {
     k_fill_i32<<<fgrid, 256, 0, st>>>(w.btag, 2 * n, -1);
     k_fill_i32<<<1, 256, 0, st>>>((int *)w.bwork, (size_t)last + 2, 0);
@@ -1318,6 +1324,10 @@ int gu_memcpy_h2d(void *dst, const void *src, size_t bytes) {
 }
 int gu_memcpy_d2h(void *dst, const void *src, size_t bytes) {
   GU_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return GU_OK;
+}
+int gu_memcpy_d2d(void *dst, const void *src, size_t bytes) {
+  GU_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
   return GU_OK;
 }
 int gu_memset_d(void *dst, int value, size_t bytes) {

@@ -609,6 +609,15 @@ struct GroupLds {
   int off[kWave + 1];
   int w[kWave];
   int r0[kWave];
+  int aux[kWave];  // hub index of a segment owner
+};
+
+// What a group pull does with a finished owner.
+enum PullMode {
+  PM_TD = 0,   // top-down finalize: commit the lanes pushed this level
+  PM_BU = 1,   // bottom-up: commit the lanes found, push the vertex
+  PM_SEG = 2,  // hub segment: atomically add partial path counts (and, when
+               // bottom-up, OR the found lanes into the hub's mask)
 };
 
 __device__ __forceinline__ int owner_of(const int *off, int t) {
@@ -621,9 +630,12 @@ __device__ __forceinline__ int owner_of(const int *off, int t) {
   return a;
 }
 
-template <int NW, bool BU>
+template <int NW, int MODE>
 __device__ void pull_group(const Eng<NW> &e, int L, int w_l, const u64 *tgt_l, int r0_l, int deg_l,
-                           GroupLds<NW> &g, Pusher &pb, Acc &a) {
+                           GroupLds<NW> &g, Pusher &pb, Acc &a, int aux_l = 0, bool seg_bu = false) {
+  constexpr bool BU = MODE == PM_BU;
+  constexpr bool OR_FOUND = MODE != PM_TD;
+  constexpr int D = NW == 1 ? 8 : (NW == 2 ? 4 : 2);  // predecessor rows in flight
   const int lane = lane_id();
   const int inc = wave_inclusive_sum(deg_l);
   const int T = wave_total(inc);
@@ -632,6 +644,7 @@ __device__ void pull_group(const Eng<NW> &e, int L, int w_l, const u64 *tgt_l, i
   if (lane == kWave - 1) g.off[kWave] = T;
   g.w[lane] = w_l;
   g.r0[lane] = r0_l;
+  g.aux[lane] = aux_l;
 #pragma unroll
   for (int j = 0; j < NW; j++) {
     g.tgt[lane][j] = tgt_l[j];
@@ -648,6 +661,22 @@ __device__ void pull_group(const Eng<NW> &e, int L, int w_l, const u64 *tgt_l, i
 
   auto flush = [&](int slot) {
     const int w = g.w[slot];
+    if (MODE == PM_SEG) {
+      u64 *srow = e.svm + (size_t)w * (kWL * NW);
+#pragma unroll
+      for (int j = 0; j < NW; j++) {
+        if (acc[j] != 0) {
+          u64 old = atomicAdd(&srow[j * kWL + lane], acc[j]);
+          a.ovf |= old + acc[j] < acc[j];
+        }
+      }
+      if (seg_bu && lane == 0) {
+#pragma unroll
+        for (int j = 0; j < NW; j++)
+          if (g.found[slot][j]) atomicOr(&e.hub_found[(size_t)g.aux[slot] * NW + j], g.found[slot][j]);
+      }
+      return;
+    }
     u64 vo[NW], nw[NW];
 #pragma unroll
     for (int j = 0; j < NW; j++) {
@@ -682,7 +711,7 @@ __device__ void pull_group(const Eng<NW> &e, int L, int w_l, const u64 *tgt_l, i
     const bool validn = tn < T;
     const int ownn = validn ? owner_of(g.off, tn) : 0;
     const int vn = validn ? e.CP[g.r0[ownn] + tn - g.off[ownn]] : 0;
-    if (BU) {
+    if (OR_FOUND) {
 #pragma unroll
       for (int j = 0; j < NW; j++)
         if (m[j]) atomicOr(&g.found[own][j], m[j]);
@@ -690,9 +719,9 @@ __device__ void pull_group(const Eng<NW> &e, int L, int w_l, const u64 *tgt_l, i
     }
     uint64_t pm = ballot(has);
     while (pm) {
-      int k[4];
+      int k[D];
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
+      for (int u = 0; u < D; u++) {
         if (pm) {
           k[u] = __ffsll((unsigned long long)pm) - 1;
           pm &= pm - 1;
@@ -700,10 +729,10 @@ __device__ void pull_group(const Eng<NW> &e, int L, int w_l, const u64 *tgt_l, i
           k[u] = -1;
         }
       }
-      u64 x[4][NW];
-      int ow[4];
+      u64 x[D][NW];
+      int ow[D];
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
+      for (int u = 0; u < D; u++) {
         ow[u] = -1;
         if (k[u] >= 0) {
           ow[u] = readlane(own, k[u]);
@@ -718,7 +747,7 @@ __device__ void pull_group(const Eng<NW> &e, int L, int w_l, const u64 *tgt_l, i
         }
       }
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
+      for (int u = 0; u < D; u++) {
         if (ow[u] < 0) continue;
         if (ow[u] != cur) {
           if (cur >= 0) flush(cur);
@@ -787,7 +816,7 @@ __device__ void bu_level(const Eng<NW> &e, int L, Pusher &pb, GroupLds<NW> &g, A
       }
       const int r0 = have ? e.RP[w] : 0;
       const int deg = any ? e.RP[w + 1] - r0 : 0;
-      pull_group<NW, true>(e, L, w, tgt, r0, deg, g, pb, a);
+      pull_group<NW, PM_BU>(e, L, w, tgt, r0, deg, g, pb, a);
       continue;
     }
     // distances only: one vertex at a time, stopping as soon as every
@@ -850,7 +879,7 @@ __device__ void td_finalize(const Eng<NW> &e, int L, GroupLds<NW> &g, Acc &a) {
 #pragma unroll
     for (int j = 0; j < NW; j++) nw[j] = (have && !hub) ? fn[(size_t)w * NW + j] : 0ull;
     if (SIGMA) {
-      pull_group<NW, false>(e, L, w, nw, r0, hub ? 0 : deg, g, none, a);
+      pull_group<NW, PM_TD>(e, L, w, nw, r0, hub ? 0 : deg, g, none, a);
       continue;
     }
     uint64_t todo = ballot(have && !hub);
@@ -872,43 +901,51 @@ __device__ void td_finalize(const Eng<NW> &e, int L, GroupLds<NW> &g, Acc &a) {
   }
 }
 
-// Hub segments (both directions): each wave pulls one segment of a hub's
-// adjacency for the hub's target lanes (top-down: the lanes pushed this
+// Hub segments (both directions): each wave pulls a pair of segments of hub
+// adjacency lists for the hubs' target lanes (top-down: the lanes pushed this
 // level; bottom-up: the lanes not yet reached) and adds its partial path
 // counts with 64-bit atomics -- exact, so the sum does not depend on the
 // order of the segments.  Bottom-up also ORs the lanes it found.
 template <int NW, bool SIGMA>
-__device__ void hub_segments(const Eng<NW> &e, int L, int mode, Acc &a) {
+__device__ void hub_segments(const Eng<NW> &e, int L, int mode, GroupLds<NW> &g, Acc &a) {
+  constexpr int G = 2;  // segments per wave step
   const int lane = lane_id();
   const u64 *fc = frs(e, L);
   const u64 *fn = frs(e, L + 1);
   const int waves = gridDim.x * (blockDim.x / kWave);
-  for (int si = blockIdx.x * (blockDim.x / kWave) + wave_id(); si < e.nsegs; si += waves) {
-    const int4 sg = e.segs[si];
-    const int w = sg.x;
-    u64 tgt[NW];
-    bool any = false;
-#pragma unroll
-    for (int j = 0; j < NW; j++) {
-      tgt[j] = mode ? (e.all[j] & ~e.visit[(size_t)w * NW + j]) : fn[(size_t)w * NW + j];
-      any |= tgt[j] != 0;
-    }
-    if (!any) continue;
-    u64 found[NW], acc[NW];
-    pull<NW, SIGMA>(e, fc, sg.y, sg.z, tgt, found, acc, &a.ovf, &a.dag);
-    if (mode && lane == 0) {
-#pragma unroll
-      for (int j = 0; j < NW; j++)
-        if (found[j]) atomicOr(&e.hub_found[(size_t)sg.w * NW + j], found[j]);
-    }
+  Pusher none{nullptr, 0};
+  for (int s0 = (blockIdx.x * (blockDim.x / kWave) + wave_id()) * G; s0 < e.nsegs; s0 += waves * G) {
     if (SIGMA) {
-      u64 *srow = e.svm + (size_t)w * (kWL * NW);
+      const bool have = lane < G && s0 + lane < e.nsegs;
+      const int4 sg = have ? e.segs[s0 + lane] : make_int4(0, 0, 0, 0);
+      u64 tgt[NW];
+      bool any = false;
 #pragma unroll
       for (int j = 0; j < NW; j++) {
-        if (acc[j] != 0) {
-          u64 old = atomicAdd(&srow[j * kWL + lane], acc[j]);
-          a.ovf |= old + acc[j] < acc[j];
-        }
+        tgt[j] = !have ? 0ull
+                       : (mode ? (e.all[j] & ~e.visit[(size_t)sg.x * NW + j]) : fn[(size_t)sg.x * NW + j]);
+        any |= tgt[j] != 0;
+      }
+      pull_group<NW, PM_SEG>(e, L, sg.x, tgt, sg.y, any ? sg.z - sg.y : 0, g, none, a, sg.w, mode != 0);
+      continue;
+    }
+    for (int si = s0; si < s0 + G && si < e.nsegs; si++) {
+      const int4 sg = e.segs[si];
+      const int w = sg.x;
+      u64 tgt[NW];
+      bool any = false;
+#pragma unroll
+      for (int j = 0; j < NW; j++) {
+        tgt[j] = mode ? (e.all[j] & ~e.visit[(size_t)w * NW + j]) : fn[(size_t)w * NW + j];
+        any |= tgt[j] != 0;
+      }
+      if (!any) continue;
+      u64 found[NW], acc[NW];
+      pull<NW, false>(e, fc, sg.y, sg.z, tgt, found, acc, &a.ovf, &a.dag);
+      if (mode && lane == 0) {
+#pragma unroll
+        for (int j = 0; j < NW; j++)
+          if (found[j]) atomicOr(&e.hub_found[(size_t)sg.w * NW + j], found[j]);
       }
     }
   }

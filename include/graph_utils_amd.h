@@ -118,6 +118,7 @@ int gu_free(void *p_dev);
 int gu_memcpy_h2d(void *dst_dev, const void *src, size_t bytes);
 int gu_memcpy_d2h(void *dst, const void *src_dev, size_t bytes);
 int gu_memset_d(void *dst_dev, int value, size_t bytes);
+int gu_memcpy_d2d(void *dst_dev, const void *src_dev, size_t bytes);
 int gu_synchronize(void);
 const char *gu_strerror(int status);
 
