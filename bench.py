@@ -43,6 +43,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--variant", default="auto")
+    ap.add_argument("--alpha", type=int, default=0)
+    ap.add_argument("--beta", type=int, default=0)
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -75,7 +77,7 @@ def main():
     batches = [hg.sample_sources(k, seed=1000 + i) for i in range(nb)]
     dist = gu.DeviceBuffer(4 * k * n)
     sig = gu.DeviceBuffer(8 * k * n)
-    cfg = gu.make_config(args.variant)
+    cfg = gu.make_config(args.variant, alpha=args.alpha, beta=args.beta)
     with gu.DeviceGraph(hg) as dg:
         for i in range(args.warmup):
             dg.multi_search_device(batches[i], dist, sig, cfg)

@@ -1011,7 +1011,7 @@ void launch_level(gu_graph *g, const Eng<NW> &e, int variant, int block, int gri
   k_control<NW><<<1, 256, 0, st>>>(e);
 }
 
-int default_alpha(const gu_config &c) { return c.alpha > 0 ? c.alpha : 4; }
+int default_alpha(const gu_config &c) { return c.alpha > 0 ? c.alpha : 16; }
 int default_beta(const gu_config &c) { return c.beta > 0 ? c.beta : 64; }
 
 // One batch of nb <= 64*NW sources through the level loop.

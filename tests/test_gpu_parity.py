@@ -115,6 +115,26 @@ def test_grid_256_sources_and_diameter(gu, orc, gpu):
     assert diam == max(orc.eccentricity(hg.R, hg.C, int(s)) for s in src[:64])
 
 
+@pytest.mark.parametrize("width", [64, 256])
+def test_intact_grid_binomials_wrap(gu, gpu, width):
+    """40x40 grid from a corner: sigma = C(r+c, r) mod 2^64 (wraps past
+    2^64, which the overflow flag reports)."""
+    from math import comb
+    W = 40
+    hg = gu.HostGraph.grid(W, 0.0, seed=0, lcc_only=False)
+    src = np.array([0, W - 1, W * W - 1, 5 * W + 7] * (width // 4), dtype=np.int32)
+    announce("intact grid 40x40 width=%d" % width)
+    with gu.DeviceGraph(hg) as dg:
+        d, s, st = dg.multi_search(src, True, True, gu.make_config("auto", 0, width))
+    r, c = np.divmod(np.arange(W * W), W)
+    exp = np.array([comb(int(a + b), int(a)) % (1 << 64) for a, b in zip(r, c)], dtype=np.uint64)
+    assert np.array_equal(d[0], r + c)
+    assert np.array_equal(s[0], exp)
+    assert st.sigma_overflow == 1
+    # corner W-1: distance r + (W-1-c)
+    assert np.array_equal(d[1], r + (W - 1 - c))
+
+
 @pytest.mark.parametrize("name", FIXTURES + ["er198"])
 def test_diameter_fixtures(gu, orc, gpu, name):
     hg = _load(gu, name)
