@@ -45,6 +45,7 @@ VARIANTS = {
     "auto": VARIANT_AUTO,
 }
 DIR_AUTO, DIR_TOPDOWN, DIR_BOTTOMUP = 0, 1, 2
+ENGINE_AUTO, ENGINE_BATCHED, ENGINE_PER_SOURCE = 0, 1, 2
 
 INT_MAX = 2147483647
 
@@ -75,7 +76,7 @@ class Config(ctypes.Structure):
         ("alpha", ctypes.c_int32),
         ("beta", ctypes.c_int32),
         ("sync_every", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("engine", ctypes.c_int32),
         ("stream", ctypes.c_void_p),
     ]
 
@@ -93,11 +94,11 @@ class Stats(ctypes.Structure):
         ("sigma_overflow", ctypes.c_int32),
         ("variant_used", ctypes.c_int32),
         ("batch_width_used", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("engine_used", ctypes.c_int32),
     ]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class DeviceProps(ctypes.Structure):
@@ -208,10 +209,10 @@ def _take(p, count, dtype=np.int32):
 
 
 def make_config(variant="auto", direction=DIR_AUTO, batch_width=0, wg_size=0, alpha=0,
-                beta=0, sync_every=0):
+                beta=0, sync_every=0, engine=ENGINE_AUTO):
     v = VARIANTS[variant] if isinstance(variant, str) else int(variant)
     return Config(v, int(direction), int(batch_width), int(wg_size), int(alpha), int(beta),
-                  int(sync_every), 0, None)
+                  int(sync_every), int(engine), None)
 
 
 # --------------------------------------------------------------------------

@@ -15,6 +15,7 @@
 
 #include "../../include/graph_utils_amd.h"
 #include "gu_kernels.hpp"
+#include "gu_persource.hpp"
 
 namespace gu {
 
@@ -770,6 +771,23 @@ struct Workspace {
 
 }  // namespace
 
+// Per-source engine buffers (allocated on first use).
+struct PsWorkspace {
+  int cap = 0;              // sources per launch
+  bool dist = false;        // has a distance scratch of cap rows
+  int *queues = nullptr;    // [cap][2][n]
+  int *dscratch = nullptr;  // [cap][n] distances when the caller keeps none
+  int *depth = nullptr, *ovf = nullptr;
+  u64 *part = nullptr;
+  size_t bytes = 0;
+  void release() {
+    void *ps[] = {queues, dscratch, depth, ovf, part};
+    for (void *p : ps)
+      if (p) (void)hipFree(p);
+    *this = PsWorkspace();
+  }
+};
+
 // Degree-balanced work units of one adjacency (fixed per graph).
 struct Units {
   int2 *blocks = nullptr;  // vertex blocks
@@ -798,10 +816,12 @@ struct gu_graph {
   int cus = 0;
   int device = 0;
   Workspace ws;
+  PsWorkspace pws;
   Ctrl *ctrl = nullptr;       // device
   Ctrl *ctrl_h = nullptr;     // pinned host mirror
   int *srcs_h = nullptr;      // pinned staging for batch sources
   int max_deg = 0;
+  double avg_deg = 0;         // over vertices with out-degree > 0
   int *RT = nullptr, *CT = nullptr;  // in-edge CSR of a directed graph
   Units pu;                   // work units over in-edges (pulls)
   Units ou;                   // work units over out-edges (Brandes backward)
@@ -1120,6 +1140,78 @@ int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const g
   return GU_OK;
 }
 
+constexpr int kPsCap = 256;  // sources per per-source launch
+
+int ensure_pws(gu_graph *g, bool need_dist) {
+  PsWorkspace &p = g->pws;
+  const size_t n = (size_t)g->n;
+  if (p.cap == 0) {
+    GU_HIP(dalloc(&p.queues, (size_t)kPsCap * 2 * n));
+    GU_HIP(dalloc(&p.depth, kPsCap));
+    GU_HIP(dalloc(&p.ovf, 1));
+    GU_HIP(dalloc(&p.part, 3 * kPsCap));
+    p.cap = kPsCap;
+    p.bytes = (size_t)kPsCap * 2 * n * 4 + kPsCap * 20 + 4;
+  }
+  if (need_dist && !p.dist) {
+    GU_HIP(dalloc(&p.dscratch, (size_t)kPsCap * n));
+    p.dist = true;
+    p.bytes += (size_t)kPsCap * n * 4;
+  }
+  return GU_OK;
+}
+
+// Per-source engine over nsrc sources: rows go to dist / sigma (caller rows,
+// may be null: distances then use a scratch of kPsCap rows).
+int run_persource(gu_graph *g, const int *sources, int nsrc, int *dist, u64 *sigma, gu_stats &s,
+                  int &dmax, hipStream_t st) {
+  int rc = ensure_pws(g, dist == nullptr);
+  if (rc) return rc;
+  PsWorkspace &p = g->pws;
+  const size_t n = (size_t)g->n;
+  std::vector<u64> part(3 * kPsCap);
+  std::vector<int> depth(kPsCap);
+  int *srcs_d = nullptr;
+  GU_HIP(dalloc(&srcs_d, (size_t)std::max(nsrc, 1)));
+  GU_HIP(hipMemcpyAsync(srcs_d, sources, sizeof(int) * nsrc, hipMemcpyHostToDevice, st));
+  GU_HIP(hipMemsetAsync(p.ovf, 0, sizeof(int), st));
+  for (int b0 = 0; b0 < nsrc; b0 += kPsCap) {
+    const int nb = std::min(kPsCap, nsrc - b0);
+    PsArgs a{g->R, g->C, g->n, srcs_d + b0,
+             dist ? dist + (size_t)b0 * n : p.dscratch,
+             sigma ? sigma + (size_t)b0 * n : nullptr,
+             p.queues, p.depth, p.part, p.ovf};
+    if (sigma)
+      k_persource<true><<<nb, 1024, 0, st>>>(a);
+    else
+      k_persource<false><<<nb, 1024, 0, st>>>(a);
+    GU_HIP(hipGetLastError());
+    GU_HIP(hipMemcpyAsync(part.data(), p.part, sizeof(u64) * 3 * nb, hipMemcpyDeviceToHost, st));
+    GU_HIP(hipMemcpyAsync(depth.data(), p.depth, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
+    GU_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < nb; i++) {
+      s.teps_units += (int64_t)part[3 * i];
+      s.reached += (int64_t)part[3 * i + 1];
+      s.dag_edges += (int64_t)part[3 * i + 2];
+      dmax = std::max(dmax, depth[i]);
+    }
+    s.batches++;
+  }
+  int ovf = 0;
+  GU_HIP(hipMemcpy(&ovf, p.ovf, sizeof(int), hipMemcpyDeviceToHost));
+  s.sigma_overflow |= ovf;
+  (void)hipFree(srcs_d);
+  return GU_OK;
+}
+
+// Engine choice from the degree profile: bounded-degree, sparse graphs are
+// the high-diameter ones whose sources do not share levels.
+int resolve_engine(gu_graph *g, const gu_config &cfg, int job) {
+  if (job == 1) return GU_ENGINE_BATCHED;  // Brandes needs the batched stacks
+  if (cfg.engine == GU_ENGINE_BATCHED || cfg.engine == GU_ENGINE_PER_SOURCE) return cfg.engine;
+  return (g->max_deg <= 64 && g->avg_deg <= 8.0) ? GU_ENGINE_PER_SOURCE : GU_ENGINE_BATCHED;
+}
+
 int resolve_variant(gu_graph *g, const gu_config &cfg) {
   int v = cfg.variant;
   if (v >= V_LINEAR && v <= V_SHUFFLE) return v;
@@ -1164,9 +1256,12 @@ int run_job(gu_graph *g, const int32_t *sources, int32_t nsrc, Job job, int32_t 
   gu_stats s{};
   s.variant_used = variant;
   s.batch_width_used = width;
+  s.engine_used = resolve_engine(g, cfg, job);
   int rc = GU_OK;
   int dmax = 0;
-  for (int b0 = 0; b0 < nsrc && rc == GU_OK; b0 += width) {
+  if (s.engine_used == GU_ENGINE_PER_SOURCE && nsrc > 0)
+    rc = run_persource(g, sources, nsrc, dist, want_sigma ? (u64 *)sigma : nullptr, s, dmax, st);
+  for (int b0 = 0; s.engine_used == GU_ENGINE_BATCHED && b0 < nsrc && rc == GU_OK; b0 += width) {
     int nb = std::min(width, nsrc - b0);
     BatchOut o;
     const size_t row = (size_t)b0 * (size_t)g->n;
@@ -1348,9 +1443,11 @@ int gu_graph_create(const int32_t *R, const int32_t *C, const int32_t *F, int32_
   // host-side checks of the CSR shape (the reference trusts its parser)
   if (R[0] != 0 || (int64_t)R[n] != m) return GU_ERR_INVALID;
   int maxd = 0;
+  long long nonzero = 0;
   for (int32_t v = 0; v < n; v++) {
     if (R[v + 1] < R[v]) return GU_ERR_INVALID;
     maxd = std::max(maxd, R[v + 1] - R[v]);
+    nonzero += R[v + 1] > R[v];
   }
   for (int64_t k = 0; k < m; k++)
     if (C[k] < 0 || C[k] >= n) return GU_ERR_RANGE;
@@ -1360,6 +1457,7 @@ int gu_graph_create(const int32_t *R, const int32_t *C, const int32_t *F, int32_
   g->m = m;
   g->directed = directed;
   g->max_deg = maxd;
+  g->avg_deg = nonzero ? (double)m / (double)nonzero : 0.0;
   int rc = GU_OK;
   do {
     if (hipGetDevice(&g->device) != hipSuccess) { rc = GU_ERR_HIP; break; }
@@ -1449,7 +1547,7 @@ int gu_graph_arrays(gu_graph_t g, const int32_t **R, const int32_t **C, const in
 
 int gu_graph_device_bytes(gu_graph_t g, int64_t *bytes) {
   if (!g || !bytes) return GU_ERR_INVALID;
-  *bytes = (int64_t)(((size_t)g->n + 1) * 4 + (size_t)g->m * 8 + g->ws.bytes);
+  *bytes = (int64_t)(((size_t)g->n + 1) * 4 + (size_t)g->m * 8 + g->ws.bytes + g->pws.bytes);
   return GU_OK;
 }
 

@@ -55,6 +55,16 @@ extern "C" {
 #define GU_DIR_TOPDOWN 1   /* frontier expansion only (the reference's form) */
 #define GU_DIR_BOTTOMUP 2  /* pull from the frontier only */
 
+/* Search engines.  BATCHED runs up to 256 sources per launch sequence as
+ * lanes of per-vertex bit masks, one pass over the graph per level for all of
+ * them (best when sources share levels: low-diameter graphs).  PER_SOURCE
+ * runs one workgroup per source with the whole level loop inside one launch
+ * (best for high-diameter, bounded-degree graphs).  AUTO picks from the
+ * degree profile.  Brandes betweenness always uses BATCHED. */
+#define GU_ENGINE_AUTO 0
+#define GU_ENGINE_BATCHED 1
+#define GU_ENGINE_PER_SOURCE 2
+
 typedef struct gu_graph *gu_graph_t;
 
 typedef struct {
@@ -67,7 +77,7 @@ typedef struct {
   int32_t beta;         /* bottom-up -> top-down when the frontier has fewer
                            than n / beta vertices (0 = default) */
   int32_t sync_every;   /* levels launched between host checks (0 = adaptive) */
-  int32_t reserved;
+  int32_t engine;       /* GU_ENGINE_* (default AUTO) */
   void *stream;         /* hipStream_t; NULL = the default stream */
 } gu_config;
 
@@ -86,7 +96,7 @@ typedef struct {
   int32_t sigma_overflow;  /* 1 if any path count wrapped modulo 2^64 */
   int32_t variant_used;    /* GU_VARIANT_* actually run */
   int32_t batch_width_used;
-  int32_t reserved;
+  int32_t engine_used;     /* GU_ENGINE_* actually run */
 } gu_stats;
 
 typedef struct {

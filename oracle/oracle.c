@@ -134,6 +134,51 @@ void or_bc_f64(const int32_t *R, const int32_t *C, int32_t n,
   free(delta);
 }
 
+typedef struct {
+  const int32_t *R, *C, *sources;
+  int32_t n, lo, hi;
+  double *bc;
+} bc_arg;
+
+static void *bc_worker(void *p) {
+  bc_arg *a = (bc_arg *)p;
+  double *delta = (double *)malloc(sizeof(double) * (size_t)(a->n > 0 ? a->n : 1));
+  for (int32_t i = 0; i < a->n; i++) a->bc[i] = 0.0;
+  for (int32_t k = a->lo; k < a->hi; k++) {
+    or_brandes_f64(a->R, a->C, a->n, a->sources[k], NULL, NULL, delta);
+    for (int32_t i = 0; i < a->n; i++) a->bc[i] += delta[i];
+  }
+  free(delta);
+  return NULL;
+}
+
+void or_bc_f64_mt(const int32_t *R, const int32_t *C, int32_t n,
+                  const int32_t *sources, int32_t nsrc, int32_t nthreads,
+                  double *bc) {
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > nsrc) nthreads = nsrc > 0 ? nsrc : 1;
+  pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nthreads);
+  bc_arg *args = (bc_arg *)malloc(sizeof(bc_arg) * (size_t)nthreads);
+  for (int32_t t = 0; t < nthreads; t++) {
+    args[t].R = R;
+    args[t].C = C;
+    args[t].sources = sources;
+    args[t].n = n;
+    args[t].lo = (int32_t)((int64_t)nsrc * t / nthreads);
+    args[t].hi = (int32_t)((int64_t)nsrc * (t + 1) / nthreads);
+    args[t].bc = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    pthread_create(&th[t], NULL, bc_worker, &args[t]);
+  }
+  for (int32_t i = 0; i < n; i++) bc[i] = 0.0;
+  for (int32_t t = 0; t < nthreads; t++) {
+    pthread_join(th[t], NULL);
+    for (int32_t i = 0; i < n; i++) bc[i] += args[t].bc[i];
+    free(args[t].bc);
+  }
+  free(th);
+  free(args);
+}
+
 /* diameter_sampling_test.cu:9-38: BFS, then the largest finite distance. */
 int32_t or_eccentricity(const int32_t *R, const int32_t *C, int32_t n,
                         int32_t src) {

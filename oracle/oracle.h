@@ -51,6 +51,12 @@ void or_brandes_f64(const int32_t *R, const int32_t *C, int32_t n, int32_t src,
 void or_bc_f64(const int32_t *R, const int32_t *C, int32_t n,
                const int32_t *sources, int32_t nsrc, double *bc);
 
+/* or_bc_f64 on nthreads threads: static partition of the source list,
+ * per-thread FP64 sums added in thread order (deterministic). */
+void or_bc_f64_mt(const int32_t *R, const int32_t *C, int32_t n,
+                  const int32_t *sources, int32_t nsrc, int32_t nthreads,
+                  double *bc);
+
 /* Max finite distance from src (eccentricity inside src's component).
  * examples/diameter_sampling/diameter_sampling_test.cu:9-38 */
 int32_t or_eccentricity(const int32_t *R, const int32_t *C, int32_t n,

@@ -31,6 +31,7 @@ def lib():
         L.or_brandes_f32.argtypes = [_P, _P, i32, i32, _P, _P, _P]
         L.or_brandes_f64.argtypes = [_P, _P, i32, i32, _P, _P, _P]
         L.or_bc_f64.argtypes = [_P, _P, i32, _P, i32, _P]
+        L.or_bc_f64_mt.argtypes = [_P, _P, i32, _P, i32, i32, _P]
         L.or_eccentricity.argtypes = [_P, _P, i32, i32]
         L.or_eccentricity.restype = i32
         L.or_exclusive_scan.argtypes = [_P, i64, _P]
@@ -95,6 +96,15 @@ def bc_f64(R, C, sources):
     src = np.ascontiguousarray(sources, dtype=np.int32)
     bc = np.empty(n, dtype=np.float64)
     lib().or_bc_f64(_p(R), _p(C), n, _p(src), len(src), _p(bc))
+    return bc
+
+
+def bc_f64_mt(R, C, sources, nthreads):
+    R, C = _csr(R, C)
+    n = len(R) - 1
+    src = np.ascontiguousarray(sources, dtype=np.int32)
+    bc = np.empty(n, dtype=np.float64)
+    lib().or_bc_f64_mt(_p(R), _p(C), n, _p(src), len(src), int(nthreads), _p(bc))
     return bc
 
 

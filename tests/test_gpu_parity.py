@@ -337,3 +337,81 @@ def test_repeatable(gu, gpu, rmat14):
         c2 = dg.betweenness(src[:32])
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
     assert np.array_equal(c1[0], c2[0]) and np.array_equal(c1[1], c2[1])
+
+
+# ---- per-source engine (one workgroup per source, level loop in one launch)
+
+PS = 2  # GU_ENGINE_PER_SOURCE
+
+
+def _ps_compare(gu, orc, hg, sources, label, sigma=True):
+    announce("per-source %s nsrc=%d n=%d" % (label, len(sources), hg.n))
+    cfg = gu.make_config(engine=PS)
+    with gu.DeviceGraph(hg) as dg:
+        d, s, st = dg.multi_search(sources, dist=True, sigma=sigma, config=cfg)
+    assert st.engine_used == PS
+    od, os_ = _oracle_rows(orc, hg, sources)
+    bad = np.argwhere(d != od)
+    assert bad.size == 0, "distance mismatch at %s" % (bad[:5].tolist(),)
+    if sigma:
+        bad = np.argwhere(s != os_)
+        assert bad.size == 0, "path count mismatch at %s" % (bad[:5].tolist(),)
+    return st
+
+
+@pytest.mark.parametrize("name", FIXTURES + ["er198"])
+def test_persource_fixtures(gu, orc, gpu, name):
+    hg = _load(gu, name)
+    _ps_compare(gu, orc, hg, np.arange(hg.n, dtype=np.int32), name)
+
+
+def test_persource_rmat14_and_hubs(gu, orc, gpu, rmat14):
+    hg, src, _ = rmat14
+    _ps_compare(gu, orc, hg, src[:32], "rmat14")
+    _ps_compare(gu, orc, _hub_graph(gu), np.array([0, 4000, 4001, 3999], dtype=np.int32), "hubgraph")
+
+
+def test_persource_grid_and_diameter(gu, orc, gpu):
+    hg = gu.HostGraph.grid(48, 0.10, seed=3)
+    src = hg.sample_sources(300, seed=4)  # more than one 256-source launch
+    st = _ps_compare(gu, orc, hg, src, "grid48")
+    assert st.batches == 2
+    with gu.DeviceGraph(hg) as dg:
+        diam, st = dg.diameter(src[:64], gu.make_config(engine=PS))
+    assert st.engine_used == PS
+    assert diam == max(orc.eccentricity(hg.R, hg.C, int(s)) for s in src[:64])
+
+
+def test_persource_directed(gu, orc, gpu):
+    hg = _directed_graph(gu)
+    _ps_compare(gu, orc, hg, np.concatenate([[0, 1, 2], hg.sample_sources(29, seed=5)]).astype(np.int32),
+                "directed")
+
+
+def test_persource_distances_only(gu, orc, gpu):
+    hg = gu.HostGraph.grid(32, 0.10, seed=9)
+    _ps_compare(gu, orc, hg, hg.sample_sources(40, seed=10), "grid32-d", sigma=False)
+
+
+def test_persource_intact_grid_wrap(gu, gpu):
+    from math import comb
+    W = 40
+    hg = gu.HostGraph.grid(W, 0.0, seed=0, lcc_only=False)
+    with gu.DeviceGraph(hg) as dg:
+        d, s, st = dg.multi_search(np.array([0], dtype=np.int32), True, True, gu.make_config(engine=PS))
+    r, c = np.divmod(np.arange(W * W), W)
+    assert np.array_equal(d[0], r + c)
+    assert np.array_equal(s[0], np.array([comb(int(a + b), int(a)) % (1 << 64) for a, b in zip(r, c)],
+                                         dtype=np.uint64))
+    assert st.sigma_overflow == 1
+
+
+def test_engine_auto_choice(gu, gpu, rmat14):
+    hg, src, _ = rmat14
+    with gu.DeviceGraph(hg) as dg:
+        _, _, st = dg.multi_search(src[:8], True, False)
+    assert st.engine_used == 1  # batched on a power-law graph
+    hg = gu.HostGraph.grid(32, 0.10, seed=9)
+    with gu.DeviceGraph(hg) as dg:
+        _, _, st = dg.multi_search(hg.sample_sources(8, seed=1), True, False)
+    assert st.engine_used == PS

@@ -328,3 +328,9 @@ def test_planted_mistake_is_caught(orc):
         d[s] += sum(1 for v in range(6) if v != s)  # what the unskipped form adds
         wrong += d
     assert not np.allclose(wrong, [20, 0, 0, 0, 0, 0])
+
+
+def test_bc_f64_multithreaded_matches_serial(orc):
+    R, C = load_ref_graph(orc, "jazz")
+    src = np.arange(0, 198, 3)
+    assert np.allclose(orc.bc_f64_mt(R, C, src, 4), orc.bc_f64(R, C, src), rtol=1e-13, atol=1e-10)
