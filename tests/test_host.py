@@ -1,0 +1,274 @@
+"""CPU tests of the host side of the C ABI: symbol exports against
+include/graph_utils_amd.h, struct layouts, the METIS reader (against the
+reference's own parse_metis compiled into oracle/_ref where available), the
+writer, the seeded generators and the [start,end) row mapping.  None of these
+calls reaches a GPU."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+from scipy.sparse.csgraph import connected_components
+
+from conftest import FIXTURES, ROOT, graph_path
+
+HEADER = os.path.join(ROOT, "include", "graph_utils_amd.h")
+
+
+def _declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^[A-Za-z_][\w \*]*?\b(gu_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol(gu):
+    declared = _declared_functions()
+    assert len(declared) >= 28
+    L = gu.lib()
+    missing = [s for s in declared if not hasattr(L, s)]
+    assert not missing, missing
+    assert set(declared) == set(gu.EXPORTS)
+
+
+def test_status_strings(gu):
+    codes = [0, -1, -2, -3, -4, -5, -6, -7, -8, -9]
+    msgs = [gu.lib().gu_strerror(c) for c in codes]
+    assert all(isinstance(m, bytes) and m for m in msgs)
+    assert len(set(msgs)) == len(msgs)
+    assert gu.lib().gu_strerror(-1000) == b"unknown status"
+
+
+def test_struct_layouts_match_header(gu, tmp_path):
+    """ctypes mirrors of gu_config / gu_stats / gu_device_props have the C
+    sizes and field offsets (compiled here with gcc against the header)."""
+    fields = {
+        "gu_config": [f[0] for f in gu.Config._fields_],
+        "gu_stats": [f[0] for f in gu.Stats._fields_],
+        "gu_device_props": [f[0] for f in gu.DeviceProps._fields_],
+    }
+    src = ['#include <stdio.h>', '#include <stddef.h>', '#include "graph_utils_amd.h"',
+           "int main(void) {"]
+    for st, fl in fields.items():
+        src.append('printf("%s %%zu\\n", sizeof(%s));' % (st, st))
+        for f in fl:
+            src.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (st, f, st, f))
+    src.append("return 0; }")
+    c = tmp_path / "layout.c"
+    c.write_text("\n".join(src))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.dirname(HEADER), str(c), "-o", str(exe)],
+                   check=True)
+    out = dict(line.split() for line in subprocess.run([str(exe)], check=True,
+                                                         capture_output=True, text=True)
+               .stdout.splitlines())
+    for st, cls in (("gu_config", gu.Config), ("gu_stats", gu.Stats),
+                    ("gu_device_props", gu.DeviceProps)):
+        assert int(out[st]) == ctypes.sizeof(cls), st
+        for f in fields[st]:
+            assert int(out[st + "." + f]) == getattr(cls, f).offset, (st, f)
+
+
+def test_calls_without_graph_are_rejected(gu):
+    L = gu.lib()
+    src = np.zeros(1, np.int32)
+    assert L.gu_multi_search(None, src.ctypes.data, 1, None, None, None, None) == -1
+    assert L.gu_betweenness(None, src.ctypes.data, 1, None, None, None, None) == -1
+    assert L.gu_diameter(None, src.ctypes.data, 1, None, None, None) == -1
+    assert L.gu_graph_destroy(None) == -1
+
+
+# --------------------------------------------------------------------------
+# METIS reader (parse.cpp:115-208)
+# --------------------------------------------------------------------------
+def _ref_or_skip(orc):
+    if not orc.ref_parse_available():
+        pytest.skip("oracle/_ref/libref_parse.so not built (needs the reference checkout)")
+
+
+QUIRK_FILES = {
+    # a leading blank makes the first id " 2" non-numeric: dropped (parse.cpp:143-154,188-193)
+    "leading_blank": "3 2\n 2 3\n1\n1\n",
+    # a doubled blank glues a blank onto the next id: dropped
+    "double_blank": "3 2\n2  3\n1\n1\n",
+    # CRLF: the last id of every line carries '\r': dropped
+    "crlf": "3 2\r\n2 3\r\n1\r\n1\r\n",
+    # tabs separate like blanks; comment lines are skipped
+    "tabs_comments": "% c\n4 3\n2\t3\n% mid\n1\t4\n1\n2\n",
+    # an empty adjacency line is a vertex without neighbours
+    "empty_line": "4 1\n\n3\n2\n\n",
+    # header with a zero weight flag is unweighted (parse.cpp:169)
+    "flag_zero": "2 1 0\n2\n1\n",
+}
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_parse_fixture_matches_reference_reader(gu, orc, name):
+    _ref_or_skip(orc)
+    hg = gu.HostGraph.parse_metis(graph_path(name))
+    R, C, F, n, m, directed = orc.ref_parse_metis(graph_path(name))
+    assert (hg.n, hg.m, hg.directed) == (n, m, directed)
+    assert np.array_equal(hg.R, R) and np.array_equal(hg.C, C) and np.array_equal(hg.F, F)
+
+
+@pytest.mark.parametrize("name", sorted(QUIRK_FILES))
+def test_parse_quirks_match_reference_reader(gu, orc, name, tmp_path):
+    _ref_or_skip(orc)
+    p = tmp_path / (name + ".graph")
+    p.write_bytes(QUIRK_FILES[name].encode())
+    hg = gu.HostGraph.parse_metis(str(p))
+    R, C, F, n, m, _ = orc.ref_parse_metis(str(p))
+    assert (hg.n, hg.m) == (n, m)
+    assert np.array_equal(hg.R, R) and np.array_equal(hg.C, C) and np.array_equal(hg.F, F)
+
+
+def test_parse_quirks_expected_values(gu, tmp_path):
+    """Known answers for the quirk files, independent of the reference build."""
+    def parse(text):
+        p = tmp_path / "q.graph"
+        p.write_bytes(text.encode())
+        return gu.HostGraph.parse_metis(str(p))
+
+    g = parse(QUIRK_FILES["leading_blank"])
+    assert g.R.tolist() == [0, 1, 2, 3] and g.C.tolist() == [2, 0, 0, 0]
+    g = parse(QUIRK_FILES["crlf"])
+    assert g.R.tolist() == [0, 1, 1, 1] and g.C.tolist()[:1] == [1]
+    g = parse(QUIRK_FILES["empty_line"])
+    assert g.R.tolist() == [0, 0, 1, 2, 2] and g.C.tolist() == [2, 1]
+    g = parse(QUIRK_FILES["tabs_comments"])
+    assert g.R.tolist() == [0, 2, 4, 5, 6] and g.C.tolist() == [1, 2, 0, 3, 0, 1]
+
+
+@pytest.mark.parametrize("text,code", [
+    ("3 2 1\n2 3\n1\n1\n", -6),        # weighted flag (parse.cpp:169-173 exits -2)
+    ("3 2 0 1\n2 3\n1\n1\n", -6),      # four header fields (parse.cpp:164-168)
+    ("x y\n", -5),                     # non-numeric header
+    ("", -5),                          # no header at all
+    ("3 1\n2 9\n1\n\n", -4),           # id beyond n
+    ("2 1\n2\n1\n1\n", -5),            # ids on a line past the n-th
+    ("2 1\n2 2 2\n1\n", -5),           # more entries than the header's m
+])
+def test_parse_errors_return_status(gu, tmp_path, text, code):
+    p = tmp_path / "bad.graph"
+    p.write_bytes(text.encode())
+    with pytest.raises(gu.GuError) as e:
+        gu.HostGraph.parse_metis(str(p))
+    assert e.value.status == code
+
+
+def test_parse_missing_file(gu, tmp_path):
+    with pytest.raises(gu.GuError) as e:
+        gu.HostGraph.parse_metis(str(tmp_path / "nope.graph"))
+    assert e.value.status == -7
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_write_parse_round_trip(gu, name, tmp_path):
+    hg = gu.HostGraph.parse_metis(graph_path(name))
+    out = tmp_path / "rt.graph"
+    hg.write_metis(str(out))
+    text = out.read_bytes()
+    assert b"\r" not in text and b"  " not in text and not re.search(rb"^ ", text, re.M)
+    g2 = gu.HostGraph.parse_metis(str(out))
+    assert np.array_equal(g2.R, hg.R) and np.array_equal(g2.C, hg.C)
+    assert np.array_equal(g2.F, hg.F) and g2.m == hg.m
+
+
+# --------------------------------------------------------------------------
+# Seeded generators
+# --------------------------------------------------------------------------
+def _simple_symmetric(g):
+    n = g.n
+    rows = np.repeat(np.arange(n), np.diff(g.R))
+    assert np.all(g.C >= 0) and np.all(g.C < n)
+    assert not np.any(rows == g.C), "self-loop"
+    A = sp.csr_matrix((np.ones(len(g.C)), (rows, g.C)), shape=(n, n))
+    assert A.nnz == len(g.C), "duplicate edge"
+    assert (A != A.T).nnz == 0, "not symmetric"
+    assert np.array_equal(g.F, rows)
+    for v in range(0, n, max(1, n // 97)):
+        row = g.C[g.R[v]:g.R[v + 1]]
+        assert np.all(np.diff(row) > 0), "row not sorted"
+    return A
+
+
+def test_er_matches_committed_fixture(gu):
+    """C1's graph: ER(198, 2742), seed 0, as committed with the golden vectors."""
+    z = np.load(os.path.join(ROOT, "tests", "golden", "er198.npz"))
+    g = gu.HostGraph.er(198, 2742, seed=0)
+    assert g.n == 198 and g.m == 5484
+    assert np.array_equal(g.R, z["R"]) and np.array_equal(g.C, z["C"])
+    _simple_symmetric(g)
+
+
+def test_er_seeds(gu):
+    a = gu.HostGraph.er(500, 4000, seed=1)
+    b = gu.HostGraph.er(500, 4000, seed=1)
+    c = gu.HostGraph.er(500, 4000, seed=2)
+    assert np.array_equal(a.C, b.C) and not np.array_equal(a.C, c.C)
+    _simple_symmetric(a)
+    full = gu.HostGraph.er(20, 190, seed=3)          # complete graph K20
+    assert np.array_equal(np.diff(full.R), np.full(20, 19))
+    with pytest.raises(gu.GuError):
+        gu.HostGraph.er(20, 191, seed=3)
+
+
+def test_rmat_properties(gu):
+    a = gu.HostGraph.rmat(12, 16, seed=5)
+    b = gu.HostGraph.rmat(12, 16, seed=5)
+    c = gu.HostGraph.rmat(12, 16, seed=6)
+    assert a.n == 4096
+    assert np.array_equal(a.R, b.R) and np.array_equal(a.C, b.C)
+    assert not np.array_equal(a.C, c.C)
+    _simple_symmetric(a)
+    deg = np.diff(a.R)
+    # skewed: vertex 0 gets the a-quadrant mass, isolated vertices exist
+    assert deg[0] == deg.max() and np.count_nonzero(deg == 0) > 0
+    assert a.m <= 2 * 16 * 4096
+
+
+def test_grid_intact_and_lcc(gu):
+    g = gu.HostGraph.grid(16, 0.0, seed=0)
+    assert g.n == 256 and g.m == 2 * 2 * 16 * 15
+    _simple_symmetric(g)
+    deg = np.diff(g.R)
+    assert deg.min() == 2 and deg.max() == 4
+    h = gu.HostGraph.grid(64, 0.10, seed=4)
+    A = _simple_symmetric(h)
+    assert connected_components(A, directed=False)[0] == 1
+    assert h.n <= 64 * 64 and h.m < 2 * 2 * 64 * 63
+    sparse = gu.HostGraph.grid(64, 0.45, seed=4)      # near the percolation point
+    assert connected_components(_simple_symmetric(sparse), directed=False)[0] == 1
+    assert sparse.n < 64 * 64
+    full = gu.HostGraph.grid(64, 0.45, seed=4, lcc_only=False)
+    assert full.n == 64 * 64
+    assert connected_components(_simple_symmetric(full), directed=False)[0] > 1
+    assert np.array_equal(gu.HostGraph.grid(64, 0.10, seed=4).C, h.C)
+
+
+def test_zipf_counts(gu):
+    W = 1 << 16
+    c = gu.zipf_counts(1.1, 1 << 12, W, seed=7)
+    assert c.dtype == np.int32 and int(c.astype(np.int64).sum()) == W
+    assert c.min() >= 1 and c.max() <= (1 << 12)
+    assert np.array_equal(c, gu.zipf_counts(1.1, 1 << 12, W, seed=7))
+    # heavy head: count 1 is the most frequent value
+    vals, cnt = np.unique(c, return_counts=True)
+    assert vals[np.argmax(cnt)] == 1
+
+
+def test_sample_sources(gu):
+    g = gu.HostGraph.rmat(12, 16, seed=5)
+    s = g.sample_sources(256, seed=9)
+    assert np.all(np.diff(g.R)[s] > 0)
+    assert np.array_equal(s, g.sample_sources(256, seed=9))
+    assert not np.array_equal(s, g.sample_sources(256, seed=10))
+
+
+@pytest.mark.parametrize("start,end,G", [(0, 5, 256), (0, 198, 256), (0, 300, 256),
+                                         (10, 700, 256), (3, 4, 1), (0, 1000, 7),
+                                         (5, 5, 256)])
+def test_row_sources_match_oracle(gu, orc, start, end, G):
+    assert np.array_equal(gu.row_sources(start, end, G), orc.rows_for_range(start, end, G))
