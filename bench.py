@@ -60,6 +60,31 @@ def profile_traffic(name):
     return None
 
 
+def copy_gbs(gu, nbytes=1 << 30, reps=5):
+    """Measured device-to-device copy bandwidth (read + write bytes / time),
+    the box's STREAM-like ceiling reported beside the 8 TB/s peak."""
+    a, b = gu.DeviceBuffer(nbytes), gu.DeviceBuffer(nbytes)
+    try:
+        a.zero()
+        b.copy_from(a)
+        gu.synchronize()
+        with gu.Timer() as t:
+            for _ in range(reps):
+                b.copy_from(a)
+        return round(2 * nbytes * reps / (t.ms / 1e3) / 1e9, 1)
+    finally:
+        a.close()
+        b.close()
+
+
+def cpu_threads():
+    try:
+        c = len(os.sched_getaffinity(0))
+    except AttributeError:
+        c = os.cpu_count() or 1
+    return max(1, min(16, c))
+
+
 def _line(metric, value, unit, args, ms, dtype, config, extra):
     out = {
         "metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
@@ -115,7 +140,8 @@ def bench_grid(args, gu, props):
           {"levels_per_step": levels, "us_per_level": round(dev_ms / args.steps / levels * 1e3, 2),
            "roofline": {"bound": "hbm", "achieved": round(alg / (dev_ms / 1e3) / 1e9, 1),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(alg / (dev_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None},
+                        "frac": round(alg / (dev_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                        "traffic": profile_traffic("grid2048_256src_traffic.txt")},
            "diameter_sampling": {"sources": 64, "diameter": diam, "ms": round(dwall * 1e3, 3),
                                  "levels": dst.max_level + 1}})
 
@@ -197,7 +223,7 @@ def bench_lbs(args, gu, props):
     extra = {"roofline": {"bound": "hbm", "achieved": round(alg / (dev / args.steps / 1e3) / 1e9, 1),
                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": round(alg / (dev / args.steps / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
-                          "traffic": None},
+                          "traffic": profile_traffic("lbs_zipf_2p26_traffic.txt")},
              "device_ms_per_step": round(dev / args.steps, 4), "segments": S}
     if not args.no_cpu:
         from oracle import oracle as orc
@@ -241,9 +267,6 @@ def main():
 
     gu = __graft_entry__.load_package()
     props = gu.choose_device(0)
-    if args.workload != "bfs":
-        {"grid": bench_grid, "bc": bench_bc, "lbs": bench_lbs}[args.workload](args, gu, props)
-        return
     if args.calib_bytes:
         a = gu.DeviceBuffer(args.calib_bytes)
         b = gu.DeviceBuffer(args.calib_bytes)
@@ -252,6 +275,9 @@ def main():
         gu.synchronize()
         a.close()
         b.close()
+    if args.workload != "bfs":
+        {"grid": bench_grid, "bc": bench_bc, "lbs": bench_lbs}[args.workload](args, gu, props)
+        return
     t0 = time.time()
     hg = gu.HostGraph.rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=20)
     gen_s = time.time() - t0
@@ -329,13 +355,14 @@ def main():
                      "bytes = per-source model summed over the batch",
             "algorithmic_bytes_per_step": int(alg_bytes / args.steps),
             "pass_ms": round(dev_ms / args.steps, 4),
+            "measured_copy_gbs": copy_gbs(gu),
         },
         "variants_gteps": variants,
     }
     if not args.no_cpu:
         from oracle import oracle as orc
 
-        sample = batches[args.warmup][:8]
+        sample = np.concatenate(batches[args.warmup:args.warmup + 2])
         orc.sigma_batch_timed(hg.R, hg.C, sample[:1], 1)  # warm-up
         u, secs = orc.sigma_batch_timed(hg.R, hg.C, sample, 1)
         out["cpu_baseline"] = {
@@ -343,8 +370,16 @@ def main():
             "unit": "GTEPS",
             "cores": 1,
             "kind": "port",
-            "sample": "8 of the 64 sources of one batch, sequential BFS + path counts "
-                      "(oracle restating shortest_paths.cu:11-39), %.1f s" % secs,
+            "sample": "%d sources (the first two timed batches), sequential BFS + path counts, "
+                      "one source after another (oracle restating shortest_paths.cu:11-39), "
+                      "%.1f s" % (len(sample), secs),
+        }
+        nt = cpu_threads()
+        sample = np.concatenate(batches[args.warmup:args.warmup + min(4, args.steps)])
+        u, secs = orc.sigma_batch_timed(hg.R, hg.C, sample, nt)
+        out["cpu_baseline_threads"] = {
+            "value": round(u / secs / 1e9, 5), "unit": "GTEPS", "cores": nt, "kind": "port",
+            "sample": "%d sources, static partition over %d threads, %.1f s" % (len(sample), nt, secs),
         }
     print(json.dumps(out), flush=True)
 
