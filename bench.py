@@ -15,6 +15,8 @@ Other workloads (one JSON line each, same fields):
     --workload bc     sampled Brandes BC on R-MAT scale 22: 1024 sources per
                       step in batches of 64/128/256 (best reported)
     --workload lbs    load-balanced search over 2^26 items, Zipf(1.1) counts
+    --workload serve  query server over R-MAT s18 / s20 and a 1024^2 grid:
+                      served vs plain queries/s and p50/p99 latency
 """
 import argparse
 import json
@@ -238,9 +240,190 @@ def bench_lbs(args, gu, props):
            "device": props["name"], "arch": props["arch"]}, extra)
 
 
+def _serve_queries(gu, hgs, nq, seed):
+    """Seeded query stream: graph uniform over the resident graphs, kind
+    70% BFS (distances + path counts) / 20% sampled BC / 10% diameter
+    sampling, sources per query log-uniform integers in [1, 512]."""
+    rng = np.random.default_rng(seed)
+    qs = []
+    for _ in range(nq):
+        g = int(rng.integers(len(hgs)))
+        u = rng.random()
+        kind = gu.QUERY_BFS if u < 0.7 else (gu.QUERY_BC if u < 0.9 else gu.QUERY_DIAMETER)
+        k = int(min(512, max(1, np.floor(np.exp(rng.random() * np.log(513))))))
+        src = hgs[g].sample_sources(k, seed=int(rng.integers(1 << 31)))
+        qs.append((g, kind, src, gu.QF_SIGMA if kind == gu.QUERY_BFS else 0))
+    return qs
+
+
+def _pct(x, p):
+    return round(float(np.percentile(np.asarray(x), p)), 2) if len(x) else None
+
+
+def _served(gu, dgs, qs, arrivals_ms, budget, slots, max_pack):
+    """Runs the stream through the server; returns (latencies ms, makespan ms,
+    stats).  A collector thread takes finished queries as they come and
+    releases them; latency = done - scheduled arrival on the server clock."""
+    import threading
+
+    with gu.Server(hbm_budget=budget, slots=slots, max_pack=max_pack) as srv:
+        ids = [srv.add_graph(dg) for dg in dgs]
+        # warm-up: every slot sees every kind at full width on every graph
+        warm = []
+        for g, dg in enumerate(dgs):
+            src = np.arange(256, dtype=np.int32) % dg.n
+            for kind, fl in ((gu.QUERY_BFS, gu.QF_SIGMA), (gu.QUERY_BC, 0), (gu.QUERY_DIAMETER, 0)):
+                for _ in range(slots):
+                    warm.append(srv.submit(ids[g], kind, src, fl))
+        for t in warm:
+            srv.wait(t)
+            srv.release(t)
+        done = {}
+        nq = len(qs)
+
+        def collect():
+            for _ in range(nq):
+                t, r = srv.next_done(600000)
+                done[t] = (r.done_ms, r.status, r.launches)
+                srv.release(t)
+
+        th = threading.Thread(target=collect)
+        th.start()
+        tick = [0] * nq
+        start = srv.now_ms() + 2.0
+        for i, (g, kind, src, fl) in enumerate(qs):
+            wait = start + arrivals_ms[i] - srv.now_ms()
+            if wait > 0.2:
+                time.sleep(wait / 1e3)
+            tick[i] = srv.submit(ids[g], kind, src, fl)
+        th.join()
+        st = srv.stats()
+    lat = [done[tick[i]][0] - (start + arrivals_ms[i]) for i in range(nq)]
+    bad = sum(1 for i in range(nq) if done[tick[i]][1] != 0)
+    makespan = max(done[t][0] for t in tick) - start
+    st["failed"] = bad
+    st["launches_per_query"] = round(st["launches"] / max(1, nq), 3)
+    return lat, makespan, st
+
+
+def _plain(gu, dgs, qs, arrivals_ms, to_host):
+    """The plain path: one query per call on the graph's own handle, serial,
+    in arrival order; outputs allocated per query, copied to host memory when
+    to_host (the drop-in *_setup pattern: alloc, launch, D2H, free)."""
+    lat = []
+    t0 = time.perf_counter() * 1e3 + 2.0
+    for i, (g, kind, src, fl) in enumerate(qs):
+        wait = t0 + arrivals_ms[i] - time.perf_counter() * 1e3
+        if wait > 0.2:
+            time.sleep(wait / 1e3)
+        dg = dgs[g]
+        k, n = len(src), dg.n
+        if kind == gu.QUERY_BFS:
+            db, sb = gu.DeviceBuffer(4 * k * n), gu.DeviceBuffer(8 * k * n)
+            dg.multi_search_device(src, db, sb)
+            if to_host:
+                db.to_host(np.int32)
+                sb.to_host(np.uint64)
+            db.close()
+            sb.close()
+        elif kind == gu.QUERY_BC:
+            bb = gu.DeviceBuffer(8 * n)
+            bb.zero()
+            dg.betweenness_device(src, None, bb)
+            if to_host:
+                bb.to_host(np.float64)
+            bb.close()
+        else:
+            dg.diameter(src)
+        lat.append(time.perf_counter() * 1e3 - (t0 + arrivals_ms[i]))
+    makespan = time.perf_counter() * 1e3 - t0
+    return lat, makespan
+
+
+def bench_serve(args, gu, props):
+    """Serving: R-MAT s18 ef16, R-MAT s20 ef16 and a 1024^2 grid (-10% edges,
+    largest component) resident in HBM; a seeded stream of BFS / BC /
+    diameter queries of 1-512 sources.  Measures the served form's saturated
+    throughput (every query queued at t=0), then runs the stream with Poisson
+    arrivals at 70% of that rate through the server and through the plain
+    path (one query per call, serial), reporting queries/s and p50/p99
+    latency (scheduled arrival -> result ready) for each."""
+    t0 = time.time()
+    hgs = [gu.HostGraph.rmat(18, 16, 0.57, 0.19, 0.19, seed=18),
+           gu.HostGraph.rmat(20, 16, 0.57, 0.19, 0.19, seed=20),
+           gu.HostGraph.grid(1024, 0.10, seed=10)]
+    gen_s = time.time() - t0
+    nq = args.queries
+    qs = _serve_queries(gu, hgs, nq, seed=4242)
+    budget = args.budget_gb << 30
+    dgs = [gu.DeviceGraph(h) for h in hgs]
+    unit = {}
+    try:
+        # device time of one launch per graph and kind, 1 and 256 sources
+        for name, h, dg in zip(("rmat18", "rmat20", "grid1024"), hgs, dgs):
+            src = h.sample_sources(256, seed=7)
+            db, sb, bb = gu.DeviceBuffer(4 * 256 * h.n), gu.DeviceBuffer(8 * 256 * h.n), gu.DeviceBuffer(8 * h.n)
+            row = {}
+            for k in (1, 256):
+                dg.multi_search_device(src[:k], db, sb)
+                row["bfs_sigma_%d" % k] = round(dg.multi_search_device(src[:k], db, sb).device_ms, 3)
+                dg.betweenness_device(src[:k], None, bb)
+                row["bc_%d" % k] = round(dg.betweenness_device(src[:k], None, bb).device_ms, 3)
+                row["diameter_%d" % k] = round(dg.diameter(src[:k])[1].device_ms, 3)
+            for b in (db, sb, bb):
+                b.close()
+            unit[name] = row
+        print("serve: unit costs %s" % json.dumps(unit), file=sys.stderr, flush=True)
+        zero = np.zeros(nq)
+        lat_s, mk_s, st_s = _served(gu, dgs, qs, zero, budget, args.slots, 256)
+        sat_qps = nq / (mk_s / 1e3)
+        qps = args.qps if args.qps > 0 else 0.7 * sat_qps
+        gaps = np.random.default_rng(99).exponential(1e3 / qps, nq)
+        arr = np.cumsum(gaps) - gaps[0]
+        print("serve: saturated %.1f q/s; offered %.1f q/s" % (sat_qps, qps), file=sys.stderr, flush=True)
+        lat_l, mk_l, st_l = _served(gu, dgs, qs, arr, budget, args.slots, 256)
+        print("serve: served run done", file=sys.stderr, flush=True)
+        lat_pd, mk_pd = _plain(gu, dgs, qs, arr, False)
+        print("serve: plain (device results) done", file=sys.stderr, flush=True)
+        lat_ph, mk_ph = _plain(gu, dgs, qs, arr, True)
+        print("serve: plain (host results) done", file=sys.stderr, flush=True)
+    finally:
+        for dg in dgs:
+            dg.close()
+    served_qps = nq / (mk_l / 1e3)
+    res = {
+        "launch_ms": unit,
+        "saturated": {"served_qps": round(sat_qps, 2), "p50_ms": _pct(lat_s, 50), "p99_ms": _pct(lat_s, 99),
+                      "server": st_s},
+        "offered_qps": round(qps, 2),
+        "served": {"qps": round(served_qps, 2), "p50_ms": _pct(lat_l, 50), "p99_ms": _pct(lat_l, 99),
+                   "server": st_l},
+        "plain_device_results": {"qps": round(nq / (mk_pd / 1e3), 2), "p50_ms": _pct(lat_pd, 50),
+                                 "p99_ms": _pct(lat_pd, 99)},
+        "plain_host_results": {"qps": round(nq / (mk_ph / 1e3), 2), "p50_ms": _pct(lat_ph, 50),
+                               "p99_ms": _pct(lat_ph, 99)},
+    }
+    res["qps_ratio_vs_plain_device"] = round(served_qps / res["plain_device_results"]["qps"], 2)
+    res["qps_ratio_vs_plain_host"] = round(served_qps / res["plain_host_results"]["qps"], 2)
+    _line("served queries/s, mixed BFS/BC/diameter stream over 3 resident graphs", round(served_qps, 2),
+          "queries/s", args, mk_l / nq, "int32+u64 / f32+f64 BC",
+          {"workload": "R-MAT s18 ef16 + R-MAT s20 ef16 + 1024^2 grid -10%% LCC resident; %d queries, "
+                       "70%% BFS (d + sigma) / 20%% BC / 10%% diameter, 1-512 sources log-uniform, "
+                       "Poisson arrivals" % nq,
+           "hbm_budget_gb": args.budget_gb, "slots": args.slots, "max_pack": 256,
+           "generator_seconds": round(gen_s, 2), "device": props["name"], "arch": props["arch"],
+           "graphs": [{"n": h.n, "m_directed": h.m} for h in hgs]},
+          res)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="bfs", choices=["bfs", "grid", "bc", "lbs"])
+    ap.add_argument("--workload", default="bfs", choices=["bfs", "grid", "bc", "lbs", "serve"])
+    ap.add_argument("--queries", type=int, default=200, help="serve workload: queries in the stream")
+    ap.add_argument("--qps", type=float, default=0.0, help="serve workload: offered load (0 = 70%% of "
+                    "the measured saturated rate)")
+    ap.add_argument("--slots", type=int, default=4, help="serve workload: launches in flight")
+    ap.add_argument("--budget-gb", type=int, default=64, help="serve workload: HBM budget for outputs")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -276,7 +459,8 @@ def main():
         a.close()
         b.close()
     if args.workload != "bfs":
-        {"grid": bench_grid, "bc": bench_bc, "lbs": bench_lbs}[args.workload](args, gu, props)
+        {"grid": bench_grid, "bc": bench_bc, "lbs": bench_lbs,
+         "serve": bench_serve}[args.workload](args, gu, props)
         return
     t0 = time.time()
     hg = gu.HostGraph.rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=20)

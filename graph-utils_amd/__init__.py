@@ -57,8 +57,15 @@ EXPORTS = (
     "gu_graph_arrays", "gu_graph_device_bytes", "gu_multi_search",
     "gu_betweenness", "gu_diameter", "gu_lbs", "gu_parse_metis",
     "gu_write_metis", "gu_gen_er", "gu_gen_rmat", "gu_gen_grid",
-    "gu_gen_zipf_counts", "gu_sample_sources", "gu_host_free",
+    "gu_gen_zipf_counts", "gu_sample_sources", "gu_host_free", "gu_graph_share",
+    "gu_multi_search_rows", "gu_betweenness_rows", "gu_server_create", "gu_server_add_graph",
+    "gu_server_submit", "gu_server_wait", "gu_server_next_done", "gu_server_poll",
+    "gu_server_release", "gu_server_now_ms", "gu_server_get_stats", "gu_server_destroy",
 )
+
+QUERY_BFS, QUERY_BC, QUERY_DIAMETER = 0, 1, 2
+QF_SIGMA, QF_DELTA = 1, 2
+ERR_CAPACITY, ERR_TIMEOUT = -9, -10
 
 
 class GuError(RuntimeError):
@@ -119,6 +126,38 @@ class DeviceProps(ctypes.Structure):
 
 _lib = None
 _P = ctypes.c_void_p
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_int64), ("slots", ctypes.c_int32),
+                ("max_pack", ctypes.c_int32), ("variant", ctypes.c_int32),
+                ("engine", ctypes.c_int32)]
+
+
+class Query(ctypes.Structure):
+    _fields_ = [("graph", ctypes.c_int32), ("kind", ctypes.c_int32), ("nsrc", ctypes.c_int32),
+                ("flags", ctypes.c_int32), ("sources", ctypes.c_void_p)]
+
+
+class QueryResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("kind", ctypes.c_int32), ("nsrc", ctypes.c_int32),
+                ("diameter", ctypes.c_int32), ("launches", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("dist_dev", ctypes.c_void_p),
+                ("sigma_dev", ctypes.c_void_p), ("delta_dev", ctypes.c_void_p),
+                ("bc_dev", ctypes.c_void_p), ("submit_ms", ctypes.c_double),
+                ("admit_ms", ctypes.c_double), ("first_launch_ms", ctypes.c_double),
+                ("done_ms", ctypes.c_double), ("hbm_bytes", ctypes.c_int64)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64),
+                ("rejected", ctypes.c_int64), ("launches", ctypes.c_int64),
+                ("sources", ctypes.c_int64), ("hbm_in_use", ctypes.c_int64),
+                ("hbm_peak", ctypes.c_int64), ("hbm_budget", ctypes.c_int64),
+                ("waiting", ctypes.c_int32), ("active", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
 _I32P = ctypes.POINTER(ctypes.c_int32)
 _I32PP = ctypes.POINTER(_I32P)
 _I64P = ctypes.POINTER(ctypes.c_int64)
@@ -173,6 +212,23 @@ def lib():
         "gu_sample_sources": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32,
                                              ctypes.c_uint64, _P]),
         "gu_host_free": (None, [_P]),
+        "gu_graph_share": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
+        "gu_multi_search_rows": (ctypes.c_int, [_P, _P, ctypes.c_int32, _P, _P, _P,
+                                                ctypes.POINTER(Config), ctypes.POINTER(Stats)]),
+        "gu_betweenness_rows": (ctypes.c_int, [_P, _P, ctypes.c_int32, _P, _P,
+                                               ctypes.POINTER(Config), ctypes.POINTER(Stats)]),
+        "gu_server_create": (ctypes.c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_P)]),
+        "gu_server_add_graph": (ctypes.c_int, [_P, _P, _I32P]),
+        "gu_server_submit": (ctypes.c_int, [_P, ctypes.POINTER(Query), _I64P]),
+        "gu_server_wait": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32,
+                                          ctypes.POINTER(QueryResult)]),
+        "gu_server_next_done": (ctypes.c_int, [_P, ctypes.c_int32, _I64P,
+                                               ctypes.POINTER(QueryResult)]),
+        "gu_server_poll": (ctypes.c_int, [_P, ctypes.c_int64, _I32P]),
+        "gu_server_release": (ctypes.c_int, [_P, ctypes.c_int64]),
+        "gu_server_now_ms": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double)]),
+        "gu_server_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(ServerStats)]),
+        "gu_server_destroy": (ctypes.c_int, [_P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -483,6 +539,132 @@ class DeviceGraph:
         _check(lib().gu_diameter(self.h, _ptr(src), len(src), ctypes.byref(out),
                                  ctypes.byref(cfg), ctypes.byref(st)), "gu_diameter")
         return out.value, st
+
+    # -- per-source output rows (gu_multi_search_rows / gu_betweenness_rows) --
+    def multi_search_rows(self, sources, dist_rows=None, sigma_rows=None, want_ecc=False,
+                          config=None):
+        """dist_rows / sigma_rows: sequences of device addresses (int, 0 = skip)
+        of n-entry rows, one per source.  Returns (ecc int32[k] or None, stats)."""
+        src = np.ascontiguousarray(sources, dtype=np.int32)
+        k = len(src)
+        dr = np.ascontiguousarray(dist_rows, dtype=np.uint64) if dist_rows is not None else None
+        sr = np.ascontiguousarray(sigma_rows, dtype=np.uint64) if sigma_rows is not None else None
+        ecc = np.zeros(k, dtype=np.int32) if want_ecc else None
+        st = Stats()
+        cfg = config if config is not None else make_config()
+        _check(lib().gu_multi_search_rows(self.h, _ptr(src), k,
+                                          _ptr(dr) if dr is not None else None,
+                                          _ptr(sr) if sr is not None else None,
+                                          _ptr(ecc) if ecc is not None else None,
+                                          ctypes.byref(cfg), ctypes.byref(st)),
+               "gu_multi_search_rows")
+        return ecc, st
+
+    def betweenness_rows(self, sources, delta_rows=None, bc_targets=None, config=None):
+        src = np.ascontiguousarray(sources, dtype=np.int32)
+        xr = np.ascontiguousarray(delta_rows, dtype=np.uint64) if delta_rows is not None else None
+        bt = np.ascontiguousarray(bc_targets, dtype=np.uint64) if bc_targets is not None else None
+        st = Stats()
+        cfg = config if config is not None else make_config()
+        _check(lib().gu_betweenness_rows(self.h, _ptr(src), len(src),
+                                         _ptr(xr) if xr is not None else None,
+                                         _ptr(bt) if bt is not None else None,
+                                         ctypes.byref(cfg), ctypes.byref(st)),
+               "gu_betweenness_rows")
+        return st
+
+    def share(self):
+        """A second handle on this resident graph with its own workspace
+        (gu_graph_share); close it before this one."""
+        h = ctypes.c_void_p()
+        _check(lib().gu_graph_share(self.h, ctypes.byref(h)), "gu_graph_share")
+        v = DeviceGraph.__new__(DeviceGraph)
+        v.n, v.m, v.h = self.n, self.m, h
+        return v
+
+
+class Server:
+    """Continuous-batching query server (gu_server_*): resident graphs,
+    HBM-budget admission, shared multi-source launches on `slots` streams."""
+
+    def __init__(self, hbm_budget=0, slots=0, max_pack=0, variant="auto", engine=ENGINE_AUTO):
+        c = ServerConfig(hbm_budget, slots, max_pack,
+                         VARIANTS[variant] if isinstance(variant, str) else int(variant), engine)
+        h = ctypes.c_void_p()
+        _check(lib().gu_server_create(ctypes.byref(c), ctypes.byref(h)), "gu_server_create")
+        self.h = h
+        self.n = {}
+
+    def add_graph(self, dg):
+        gid = ctypes.c_int32()
+        _check(lib().gu_server_add_graph(self.h, dg.h, ctypes.byref(gid)), "gu_server_add_graph")
+        self.n[gid.value] = dg.n
+        return gid.value
+
+    def submit(self, graph, kind, sources, flags=0):
+        src = np.ascontiguousarray(sources, dtype=np.int32)
+        q = Query(graph, kind, len(src), flags, src.ctypes.data)
+        t = ctypes.c_int64()
+        _check(lib().gu_server_submit(self.h, ctypes.byref(q), ctypes.byref(t)), "gu_server_submit")
+        return t.value
+
+    def wait(self, ticket, timeout_ms=-1):
+        r = QueryResult()
+        _check(lib().gu_server_wait(self.h, ticket, timeout_ms, ctypes.byref(r)), "gu_server_wait")
+        return r
+
+    def next_done(self, timeout_ms=-1):
+        r = QueryResult()
+        t = ctypes.c_int64()
+        _check(lib().gu_server_next_done(self.h, timeout_ms, ctypes.byref(t), ctypes.byref(r)),
+               "gu_server_next_done")
+        return t.value, r
+
+    def release(self, ticket):
+        _check(lib().gu_server_release(self.h, ticket), "gu_server_release")
+
+    def now_ms(self):
+        x = ctypes.c_double()
+        _check(lib().gu_server_now_ms(self.h, ctypes.byref(x)), "gu_server_now_ms")
+        return x.value
+
+    def stats(self):
+        s = ServerStats()
+        _check(lib().gu_server_get_stats(self.h, ctypes.byref(s)), "gu_server_get_stats")
+        return s.as_dict()
+
+    def fetch(self, res, graph):
+        """Host copies of a finished query's outputs: dict of arrays."""
+        n, k = self.n[graph], res.nsrc
+        out = {"status": res.status, "diameter": res.diameter}
+
+        def get(addr, dtype, count):
+            a = np.empty(count, dtype=dtype)
+            if count:
+                _check(lib().gu_memcpy_d2h(_ptr(a), ctypes.c_void_p(addr), a.nbytes), "d2h")
+            return a
+
+        if res.dist_dev:
+            out["dist"] = get(res.dist_dev, np.int32, k * n).reshape(k, n)
+        if res.sigma_dev:
+            out["sigma"] = get(res.sigma_dev, np.uint64, k * n).reshape(k, n)
+        if res.delta_dev:
+            out["delta"] = get(res.delta_dev, np.float32, k * n).reshape(k, n)
+        if res.bc_dev:
+            out["bc"] = get(res.bc_dev, np.float64, n)
+        return out
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().gu_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
 
 
 def lbs_device(counts_buf, S, scan_buf, result_buf, result_cap):

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <climits>
 #include <cstdint>
@@ -414,7 +415,7 @@ __global__ __launch_bounds__(256) void k_lbs_expand(const int *scan, long long S
 // `fill`.  A workgroup moves 64 vertices x 64 lanes through LDS; every
 // thread keeps 16 loads in flight.
 template <int NW, class T>
-__global__ __launch_bounds__(256) void k_transpose_out(Eng<NW> e, const T *src, T *dst, T fill) {
+__global__ __launch_bounds__(256) void k_transpose_out(Eng<NW> e, const T *src, T *const *rows, T fill) {
   __shared__ T tile[64][65];
   const int tid = threadIdx.x;
   const int col = tid & 63, row0 = tid >> 6;
@@ -437,9 +438,39 @@ __global__ __launch_bounds__(256) void k_transpose_out(Eng<NW> e, const T *src, 
     for (int k = 0; k < 16; k++) {
       const int b = j * kWL + row0 + 4 * k;
       const long long v = v0 + col;
-      if (b < e.nb && v < e.n) dst[(size_t)b * e.n + v] = tile[col][row0 + 4 * k];
+      if (b < e.nb && v < e.n) {
+        T *r = rows[b];
+        if (r) r[v] = tile[col][row0 + 4 * k];
+      }
     }
     __syncthreads();
+  }
+}
+
+// Eccentricity of every lane: the largest distance among its reached
+// vertices (diameter_sampling's per-source atomicMax over finite d,
+// shuffle_based.cu:82-111).  Block partial maxima go to ecc with one atomic
+// per lane per block; ecc starts at zero.
+template <int NW>
+__global__ __launch_bounds__(256) void k_lane_ecc(Eng<NW> e, int *ecc) {
+  __shared__ int s_m[4][kWL * NW];
+  const int tid = threadIdx.x, col = tid & 63, r0 = tid >> 6;
+  int mx[NW];
+#pragma unroll
+  for (int j = 0; j < NW; j++) mx[j] = 0;
+  for (long long v = (long long)blockIdx.x * 4 + r0; v < e.n; v += (long long)gridDim.x * 4) {
+#pragma unroll
+    for (int j = 0; j < NW; j++) {
+      if ((e.visit[(size_t)v * NW + j] >> col) & 1ull)
+        mx[j] = max(mx[j], e.dvm[(size_t)v * (kWL * NW) + j * kWL + col]);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NW; j++) s_m[r0][j * kWL + col] = mx[j];
+  __syncthreads();
+  for (int b = tid; b < e.nb; b += blockDim.x) {
+    const int m = max(max(s_m[0][b], s_m[1][b]), max(s_m[2][b], s_m[3][b]));
+    if (m > 0) atomicMax(&ecc[b], m);
   }
 }
 
@@ -684,7 +715,8 @@ __global__ __launch_bounds__(256) void k_bw_hub_combine(Eng<NW> e, Bw b, int L) 
 // bc[v] += sum over lanes of delta_b(v), summed lane by lane in a fixed
 // order so the result is deterministic.
 template <int NW>
-__global__ __launch_bounds__(256) void k_bc_out(Eng<NW> e, const float *dl, float *dout, double *bc) {
+__global__ __launch_bounds__(256) void k_bc_out(Eng<NW> e, const float *dl, float *const *drows,
+                                                double *const *bct) {
   __shared__ float tl[64][65];
   const int tid = threadIdx.x, T = blockDim.x;
   const int col = tid % 64;
@@ -702,19 +734,32 @@ __global__ __launch_bounds__(256) void k_bc_out(Eng<NW> e, const float *dl, floa
         tl[rr][col] = x;
       }
       __syncthreads();
-      if (dout) {
+      if (drows) {
         for (int bb = tid / 64; bb < 64; bb += T / 64) {
           int b = j * kWL + bb;
           long long v = v0 + col;
-          if (b < e.nb && v < e.n) dout[(size_t)b * e.n + v] = tl[col][bb];
+          if (b < e.nb && v < e.n) {
+            float *r = drows[b];
+            if (r) r[v] = tl[col][bb];
+          }
         }
       }
-      if (tid < 64) {
-        for (int c = 0; c < 64; c++) acc += (double)tl[tid][c];
+      if (bct && tid < 64) {
+        // lanes in order; a run of lanes with one target is summed, then
+        // added to that target once
+        for (int c = 0; c < 64; c++) {
+          const int b = j * kWL + c;
+          if (b >= e.nb) break;
+          acc += (double)tl[tid][c];
+          double *t = bct[b];
+          if (b + 1 == e.nb || bct[b + 1] != t) {
+            if (t && v0 + tid < e.n) t[v0 + tid] += acc;
+            acc = 0.0;
+          }
+        }
       }
       __syncthreads();
     }
-    if (bc && tid < 64 && v0 + tid < e.n) bc[v0 + tid] += acc;
   }
 }
 
@@ -775,13 +820,16 @@ struct Workspace {
 struct PsWorkspace {
   int cap = 0;              // sources per launch
   bool dist = false;        // has a distance scratch of cap rows
-  int *queues = nullptr;    // [cap][2][n]
+  int *queues = nullptr;    // [cap][2n+2] frontier queues / Brandes stack
   int *dscratch = nullptr;  // [cap][n] distances when the caller keeps none
+  u64 *sscratch = nullptr;  // [cap][n] path counts of a Brandes launch
+  float *xscratch = nullptr;  // [cap][n] dependencies when the caller keeps none
   int *depth = nullptr, *ovf = nullptr;
+  int *srcs = nullptr;      // [cap] sources of the launch
   u64 *part = nullptr;
   size_t bytes = 0;
   void release() {
-    void *ps[] = {queues, dscratch, depth, ovf, part};
+    void *ps[] = {queues, dscratch, sscratch, xscratch, depth, ovf, srcs, part};
     for (void *p : ps)
       if (p) (void)hipFree(p);
     *this = PsWorkspace();
@@ -827,6 +875,19 @@ struct gu_graph {
   Units ou;                   // work units over out-edges (Brandes backward)
   int tuned_variant = -1;     // autotuner cache
   int tuned_direction_alpha = 0;
+  // Output row tables of one launch (device, with a pinned host mirror):
+  // [0,256) distance rows, [256,512) path-count rows, [512,768) dependency
+  // rows, [768,1024) Brandes accumulation targets.
+  void **rowtab = nullptr;
+  void **rowtab_h = nullptr;
+  int *ecc = nullptr;         // device [256] per-source eccentricity
+  int *ecc_h = nullptr;       // pinned mirror
+  int *misc_h = nullptr;      // pinned: [0] overflow flag, [64, 320) depths
+  u64 *part_h = nullptr;      // pinned [3 * 256] per-source sums
+  // Handles made by gu_graph_share read the root's graph arrays and units
+  // and own only their workspaces; the root outlives them.
+  gu_graph *root = nullptr;
+  std::atomic<int> views{0};
 };
 
 namespace {
@@ -901,14 +962,19 @@ void transpose_csr(int n, long long m, const int32_t *R, const int32_t *C, std::
     for (int k = R[v]; k < R[v + 1]; k++) CT[(size_t)pos[C[k]]++] = v;
 }
 
+// Buffers are sized for the widest batch requested so far (w.NW words per
+// vertex); a narrower batch uses the front of each buffer with its own
+// [n][64*NW] layout, so mixed widths never reallocate.
 int ensure_ws(gu_graph *g, int NW, bool stack, size_t want_S, hipStream_t st) {
   Workspace &w = g->ws;
   const size_t n = (size_t)g->n;
-  const size_t WL = (size_t)kWL * NW;
-  if (w.NW != NW || (stack && !w.stack)) {
+  if (NW > w.NW || (stack && !w.stack)) {
+    NW = std::max(NW, w.NW);
+    stack = stack || w.stack;
     w.release();
     w.NW = NW;
     w.n = g->n;
+    const size_t WL = (size_t)kWL * NW;
     GU_HIP(dalloc(&w.visit, n * NW));
     GU_HIP(dalloc(&w.fr, 3 * n * NW));
     GU_HIP(dalloc(&w.tag, n));
@@ -940,13 +1006,13 @@ int ensure_ws(gu_graph *g, int NW, bool stack, size_t want_S, hipStream_t st) {
     int *q2;
     u64 *m2;
     GU_HIP(dalloc(&q2, cap));
-    GU_HIP(dalloc(&m2, cap * NW));
+    GU_HIP(dalloc(&m2, cap * w.NW));
     GU_HIP(hipMemcpyAsync(q2, w.q, w.S_cap * 4, hipMemcpyDeviceToDevice, st));
-    GU_HIP(hipMemcpyAsync(m2, w.smask, w.S_cap * NW * 8, hipMemcpyDeviceToDevice, st));
+    GU_HIP(hipMemcpyAsync(m2, w.smask, w.S_cap * w.NW * 8, hipMemcpyDeviceToDevice, st));
     GU_HIP(hipStreamSynchronize(st));
     (void)hipFree(w.q);
     (void)hipFree(w.smask);
-    w.bytes += (cap - w.S_cap) * (4 + NW * 8);
+    w.bytes += (cap - w.S_cap) * (4 + w.NW * 8);
     w.q = q2;
     w.smask = m2;
     w.S_cap = cap;
@@ -956,11 +1022,15 @@ int ensure_ws(gu_graph *g, int NW, bool stack, size_t want_S, hipStream_t st) {
 
 enum Job { JOB_BFS = 0, JOB_BC = 1, JOB_DIAM = 2 };
 
+// Outputs of one batch: device row tables (entry b = row of the batch's
+// b-th source, null = not wanted), per-lane Brandes targets and the
+// per-lane eccentricities.
 struct BatchOut {
-  int *dist = nullptr;      // caller rows for this batch (pitch n)
-  u64 *sigma = nullptr;
-  float *delta = nullptr;
-  double *bc = nullptr;
+  int *const *dist = nullptr;
+  u64 *const *sigma = nullptr;
+  float *const *delta = nullptr;
+  double *const *bc = nullptr;
+  int *ecc = nullptr;
 };
 
 struct RunInfo {
@@ -1106,6 +1176,10 @@ int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const g
     if (out.dist) k_transpose_out<NW, int><<<og, 256, 0, st>>>(e, w.dvm, out.dist, INT_MAX);
     if (out.sigma) k_transpose_out<NW, u64><<<og, 256, 0, st>>>(e, w.svm, out.sigma, 0ull);
   }
+  if (out.ecc) {
+    k_fill_i32<<<1, 256, 0, st>>>(out.ecc, (size_t)nb, 0);
+    k_lane_ecc<NW><<<pick_grid(g, 4), 256, 0, st>>>(e, out.ecc);
+  }
   if (job == JOB_BC) {
     k_fill_i32<<<fgrid, 256, 0, st>>>(w.btag, 2 * n, -1);
     k_fill_i32<<<1, 256, 0, st>>>((int *)w.bwork, (size_t)last + 2, 0);
@@ -1128,6 +1202,7 @@ int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const g
   }
   k_teps<NW><<<pick_grid(g, 4), 256, 0, st>>>(e);
   GU_HIP(hipGetLastError());
+  if (out.ecc) GU_HIP(hipMemcpyAsync(g->ecc_h, out.ecc, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
   GU_HIP(hipMemcpyAsync(g->ctrl_h, g->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
   GU_HIP(hipStreamSynchronize(st));
   info->last_level = last;
@@ -1146,12 +1221,13 @@ int ensure_pws(gu_graph *g, bool need_dist) {
   PsWorkspace &p = g->pws;
   const size_t n = (size_t)g->n;
   if (p.cap == 0) {
-    GU_HIP(dalloc(&p.queues, (size_t)kPsCap * 2 * n));
+    GU_HIP(dalloc(&p.queues, (size_t)kPsCap * (2 * n + 2)));
     GU_HIP(dalloc(&p.depth, kPsCap));
     GU_HIP(dalloc(&p.ovf, 1));
     GU_HIP(dalloc(&p.part, 3 * kPsCap));
+    GU_HIP(dalloc(&p.srcs, kPsCap));
     p.cap = kPsCap;
-    p.bytes = (size_t)kPsCap * 2 * n * 4 + kPsCap * 20 + 4;
+    p.bytes = (size_t)kPsCap * (2 * n + 2) * 4 + kPsCap * 32 + 4;
   }
   if (need_dist && !p.dist) {
     GU_HIP(dalloc(&p.dscratch, (size_t)kPsCap * n));
@@ -1161,34 +1237,125 @@ int ensure_pws(gu_graph *g, bool need_dist) {
   return GU_OK;
 }
 
-// Per-source engine over nsrc sources: rows go to dist / sigma (caller rows,
-// may be null: distances then use a scratch of kPsCap rows).
-int run_persource(gu_graph *g, const int *sources, int nsrc, int *dist, u64 *sigma, gu_stats &s,
-                  int &dmax, hipStream_t st) {
-  int rc = ensure_pws(g, dist == nullptr);
+// Host view of a job's outputs: arrays of nsrc device row pointers (null
+// array or entry = not wanted), per-source Brandes targets and per-source
+// eccentricities (host).
+struct HostRows {
+  int *const *dist = nullptr;
+  u64 *const *sigma = nullptr;
+  float *const *delta = nullptr;
+  double *const *bc = nullptr;
+  int32_t *ecc = nullptr;
+};
+
+// Per-source engine over nsrc sources, kPsCap workgroups per launch.
+int run_persource(gu_graph *g, const int *sources, int nsrc, const HostRows &hr, bool sigma,
+                  gu_stats &s, int &dmax, hipStream_t st) {
+  int rc = ensure_pws(g, hr.dist == nullptr);
   if (rc) return rc;
   PsWorkspace &p = g->pws;
   const size_t n = (size_t)g->n;
-  std::vector<u64> part(3 * kPsCap);
-  std::vector<int> depth(kPsCap);
-  int *srcs_d = nullptr;
-  GU_HIP(dalloc(&srcs_d, (size_t)std::max(nsrc, 1)));
-  GU_HIP(hipMemcpyAsync(srcs_d, sources, sizeof(int) * nsrc, hipMemcpyHostToDevice, st));
-  GU_HIP(hipMemsetAsync(p.ovf, 0, sizeof(int), st));
+  u64 *part = g->part_h;
+  int *depth = g->misc_h + 64;
+  int ovf_any = 0;
   for (int b0 = 0; b0 < nsrc; b0 += kPsCap) {
     const int nb = std::min(kPsCap, nsrc - b0);
-    PsArgs a{g->R, g->C, g->n, srcs_d + b0,
-             dist ? dist + (size_t)b0 * n : p.dscratch,
-             sigma ? sigma + (size_t)b0 * n : nullptr,
-             p.queues, p.depth, p.part, p.ovf};
+    std::memcpy(g->srcs_h, sources + b0, sizeof(int) * nb);
+    for (int i = 0; i < nb; i++) {
+      int *dr = hr.dist ? hr.dist[b0 + i] : nullptr;
+      // a source whose distances are not wanted works in the scratch row
+      g->rowtab_h[i] = dr ? (void *)dr : (void *)(p.dscratch ? p.dscratch + (size_t)i * n : nullptr);
+      g->rowtab_h[256 + i] = sigma && hr.sigma ? (void *)hr.sigma[b0 + i] : nullptr;
+    }
+    bool need_scratch = false;
+    for (int i = 0; i < nb; i++) need_scratch |= g->rowtab_h[i] == nullptr;
+    if (need_scratch) {  // some rows were null in a caller's table
+      rc = ensure_pws(g, true);
+      if (rc) return rc;
+      for (int i = 0; i < nb; i++)
+        if (!g->rowtab_h[i]) g->rowtab_h[i] = p.dscratch + (size_t)i * n;
+    }
+    bool sig_rows = sigma && hr.sigma;
+    for (int i = 0; sig_rows && i < nb; i++) sig_rows = g->rowtab_h[256 + i] != nullptr;
+    if (sigma && !sig_rows) return GU_ERR_INVALID;  // path counts need every row
+    GU_HIP(hipMemcpyAsync(p.srcs, g->srcs_h, sizeof(int) * nb, hipMemcpyHostToDevice, st));
+    GU_HIP(hipMemcpyAsync(g->rowtab, g->rowtab_h, sizeof(void *) * 512, hipMemcpyHostToDevice, st));
+    if (b0 == 0) k_fill_i32<<<1, 64, 0, st>>>(p.ovf, 1, 0);
+    PsArgs a{g->R, g->C, g->n, p.srcs, (int *const *)g->rowtab,
+             sigma ? (u64 *const *)(g->rowtab + 256) : nullptr, p.queues, p.depth, p.part, p.ovf};
     if (sigma)
       k_persource<true><<<nb, 1024, 0, st>>>(a);
     else
       k_persource<false><<<nb, 1024, 0, st>>>(a);
     GU_HIP(hipGetLastError());
-    GU_HIP(hipMemcpyAsync(part.data(), p.part, sizeof(u64) * 3 * nb, hipMemcpyDeviceToHost, st));
-    GU_HIP(hipMemcpyAsync(depth.data(), p.depth, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
+    GU_HIP(hipMemcpyAsync(part, p.part, sizeof(u64) * 3 * nb, hipMemcpyDeviceToHost, st));
+    GU_HIP(hipMemcpyAsync(depth, p.depth, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
+    GU_HIP(hipMemcpyAsync(g->misc_h, p.ovf, sizeof(int), hipMemcpyDeviceToHost, st));
     GU_HIP(hipStreamSynchronize(st));
+    ovf_any |= g->misc_h[0];
+    for (int i = 0; i < nb; i++) {
+      s.teps_units += (int64_t)part[3 * i];
+      s.reached += (int64_t)part[3 * i + 1];
+      s.dag_edges += (int64_t)part[3 * i + 2];
+      dmax = std::max(dmax, depth[i]);
+      if (hr.ecc) hr.ecc[b0 + i] = depth[i];
+    }
+    s.batches++;
+  }
+  s.sigma_overflow |= ovf_any;
+  return GU_OK;
+}
+
+// Brandes on the per-source engine: forward and backward of each source in
+// one workgroup (k_ps_bc), then the FP64 accumulation into the targets in
+// source order (k_ps_bc_reduce).
+int run_persource_bc(gu_graph *g, const int *sources, int nsrc, const HostRows &hr, gu_stats &s,
+                     int &dmax, hipStream_t st) {
+  int rc = ensure_pws(g, true);
+  if (rc) return rc;
+  PsWorkspace &p = g->pws;
+  const size_t n = (size_t)g->n;
+  if (!p.sscratch) {
+    GU_HIP(dalloc(&p.sscratch, (size_t)kPsCap * n));
+    p.bytes += (size_t)kPsCap * n * 8;
+  }
+  bool need_x = hr.delta == nullptr;
+  for (int i = 0; !need_x && i < nsrc; i++) need_x = hr.delta[i] == nullptr;
+  if (need_x && !p.xscratch) {
+    GU_HIP(dalloc(&p.xscratch, (size_t)kPsCap * n));
+    p.bytes += (size_t)kPsCap * n * 4;
+  }
+  u64 *part = g->part_h;
+  int *depth = g->misc_h + 64;
+  int ovf_any = 0;
+  for (int b0 = 0; b0 < nsrc; b0 += kPsCap) {
+    const int nb = std::min(kPsCap, nsrc - b0);
+    std::memcpy(g->srcs_h, sources + b0, sizeof(int) * nb);
+    void **t = g->rowtab_h;
+    for (int i = 0; i < nb; i++) {
+      t[i] = p.dscratch + (size_t)i * n;
+      t[256 + i] = p.sscratch + (size_t)i * n;
+      float *x = hr.delta ? hr.delta[b0 + i] : nullptr;
+      t[512 + i] = x ? (void *)x : (void *)(p.xscratch + (size_t)i * n);
+      t[768 + i] = hr.bc ? (void *)hr.bc[b0 + i] : nullptr;
+    }
+    GU_HIP(hipMemcpyAsync(p.srcs, g->srcs_h, sizeof(int) * nb, hipMemcpyHostToDevice, st));
+    GU_HIP(hipMemcpyAsync(g->rowtab, t, sizeof(void *) * 1024, hipMemcpyHostToDevice, st));
+    if (b0 == 0) k_fill_i32<<<1, 64, 0, st>>>(p.ovf, 1, 0);
+    PsBcArgs a{g->R, g->C, g->n, p.srcs, (int *const *)g->rowtab, (u64 *const *)(g->rowtab + 256),
+               (float *const *)(g->rowtab + 512), p.queues, p.depth, p.part, p.ovf};
+    k_ps_bc<<<nb, 1024, 0, st>>>(a);
+    GU_HIP(hipGetLastError());
+    if (hr.bc) {
+      k_ps_bc_reduce<<<pick_grid(g, 8), 256, 0, st>>>((float *const *)(g->rowtab + 512),
+                                                       (double *const *)(g->rowtab + 768), nb, g->n);
+      GU_HIP(hipGetLastError());
+    }
+    GU_HIP(hipMemcpyAsync(part, p.part, sizeof(u64) * 3 * nb, hipMemcpyDeviceToHost, st));
+    GU_HIP(hipMemcpyAsync(depth, p.depth, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
+    GU_HIP(hipMemcpyAsync(g->misc_h, p.ovf, sizeof(int), hipMemcpyDeviceToHost, st));
+    GU_HIP(hipStreamSynchronize(st));
+    ovf_any |= g->misc_h[0];
     for (int i = 0; i < nb; i++) {
       s.teps_units += (int64_t)part[3 * i];
       s.reached += (int64_t)part[3 * i + 1];
@@ -1197,17 +1364,14 @@ int run_persource(gu_graph *g, const int *sources, int nsrc, int *dist, u64 *sig
     }
     s.batches++;
   }
-  int ovf = 0;
-  GU_HIP(hipMemcpy(&ovf, p.ovf, sizeof(int), hipMemcpyDeviceToHost));
-  s.sigma_overflow |= ovf;
-  (void)hipFree(srcs_d);
+  s.sigma_overflow |= ovf_any;
   return GU_OK;
 }
 
 // Engine choice from the degree profile: bounded-degree, sparse graphs are
 // the high-diameter ones whose sources do not share levels.
 int resolve_engine(gu_graph *g, const gu_config &cfg, int job) {
-  if (job == 1) return GU_ENGINE_BATCHED;  // Brandes needs the batched stacks
+  (void)job;
   if (cfg.engine == GU_ENGINE_BATCHED || cfg.engine == GU_ENGINE_PER_SOURCE) return cfg.engine;
   return (g->max_deg <= 64 && g->avg_deg <= 8.0) ? GU_ENGINE_PER_SOURCE : GU_ENGINE_BATCHED;
 }
@@ -1229,14 +1393,14 @@ int resolve_width(const gu_config &cfg, int nsrc) {
   return nsrc <= 64 ? 64 : (nsrc <= 128 ? 128 : 256);
 }
 
-int run_job(gu_graph *g, const int32_t *sources, int32_t nsrc, Job job, int32_t *dist,
-            uint64_t *sigma, float *delta, double *bc, int32_t *diam, const gu_config *cfgp,
-            gu_stats *stats) {
+int run_job(gu_graph *g, const int32_t *sources, int32_t nsrc, Job job, const HostRows &hr,
+            int32_t *diam, const gu_config *cfgp, gu_stats *stats) {
   if (!g) return GU_ERR_INVALID;
   gu_config cfg{};
   if (cfgp) cfg = *cfgp;
   if (cfg.direction < 0 || cfg.direction > 2) return GU_ERR_INVALID;
   if (cfg.variant < 0 || cfg.variant > GU_VARIANT_AUTO) return GU_ERR_INVALID;
+  if (cfg.engine < 0 || cfg.engine > GU_ENGINE_PER_SOURCE) return GU_ERR_INVALID;
   if (cfg.wg_size != 0 && cfg.wg_size != 256 && cfg.wg_size != 512 && cfg.wg_size != 1024)
     return GU_ERR_INVALID;
   if (nsrc < 0 || (nsrc > 0 && !sources)) return GU_ERR_INVALID;
@@ -1246,7 +1410,7 @@ int run_job(gu_graph *g, const int32_t *sources, int32_t nsrc, Job job, int32_t 
   if (width < 0) return GU_ERR_INVALID;
   const int variant = resolve_variant(g, cfg);
   // Path counts are computed for Brandes and when the caller asks for them.
-  const bool want_sigma = job == JOB_BC || (job == JOB_BFS && sigma != nullptr);
+  const bool want_sigma = job == JOB_BC || (job == JOB_BFS && hr.sigma != nullptr);
   hipStream_t st = (hipStream_t)cfg.stream;
   GU_HIP(hipSetDevice(g->device));
   hipEvent_t e0, e1;
@@ -1260,15 +1424,24 @@ int run_job(gu_graph *g, const int32_t *sources, int32_t nsrc, Job job, int32_t 
   int rc = GU_OK;
   int dmax = 0;
   if (s.engine_used == GU_ENGINE_PER_SOURCE && nsrc > 0)
-    rc = run_persource(g, sources, nsrc, dist, want_sigma ? (u64 *)sigma : nullptr, s, dmax, st);
+    rc = job == JOB_BC ? run_persource_bc(g, sources, nsrc, hr, s, dmax, st)
+                       : run_persource(g, sources, nsrc, hr, want_sigma, s, dmax, st);
   for (int b0 = 0; s.engine_used == GU_ENGINE_BATCHED && b0 < nsrc && rc == GU_OK; b0 += width) {
     int nb = std::min(width, nsrc - b0);
+    void **t = g->rowtab_h;
+    for (int i = 0; i < nb; i++) {
+      t[i] = hr.dist ? (void *)hr.dist[b0 + i] : nullptr;
+      t[256 + i] = hr.sigma ? (void *)hr.sigma[b0 + i] : nullptr;
+      t[512 + i] = hr.delta ? (void *)hr.delta[b0 + i] : nullptr;
+      t[768 + i] = hr.bc ? (void *)hr.bc[b0 + i] : nullptr;
+    }
+    GU_HIP(hipMemcpyAsync(g->rowtab, t, sizeof(void *) * 1024, hipMemcpyHostToDevice, st));
     BatchOut o;
-    const size_t row = (size_t)b0 * (size_t)g->n;
-    if (dist) o.dist = dist + row;
-    if (sigma) o.sigma = (u64 *)sigma + row;
-    if (delta) o.delta = delta + row;
-    o.bc = bc;
+    if (hr.dist) o.dist = (int *const *)g->rowtab;
+    if (hr.sigma) o.sigma = (u64 *const *)(g->rowtab + 256);
+    if (hr.delta) o.delta = (float *const *)(g->rowtab + 512);
+    if (hr.bc) o.bc = (double *const *)(g->rowtab + 768);
+    if (hr.ecc) o.ecc = g->ecc;
     RunInfo info;
     if (width == 64)
       rc = run_batch<1>(g, sources + b0, nb, job, want_sigma, cfg, variant, o, &info, st);
@@ -1277,6 +1450,7 @@ int run_job(gu_graph *g, const int32_t *sources, int32_t nsrc, Job job, int32_t 
     else
       rc = run_batch<4>(g, sources + b0, nb, job, want_sigma, cfg, variant, o, &info, st);
     if (rc) break;
+    if (hr.ecc) std::memcpy(hr.ecc + b0, g->ecc_h, sizeof(int) * nb);
     s.batches++;
     s.topdown_levels += info.td;
     s.bottomup_levels += info.bu;
@@ -1318,6 +1492,7 @@ const char *gu_strerror(int s) {
     case GU_ERR_IO: return "cannot open file";
     case GU_ERR_NODEVICE: return "no HIP device";
     case GU_ERR_CAPACITY: return "output capacity exceeded";
+    case GU_ERR_TIMEOUT: return "timed out";
     default: return "unknown status";
   }
 }
@@ -1434,6 +1609,28 @@ int gu_synchronize(void) {
   return GU_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// Pinned staging and the launch tables every handle owns.
+int alloc_handle_bufs(gu_graph *g) {
+  if (hipHostMalloc((void **)&g->ctrl_h, sizeof(Ctrl), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void **)&g->srcs_h, sizeof(int) * 256, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void **)&g->rowtab_h, sizeof(void *) * 1024, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void **)&g->ecc_h, sizeof(int) * 256, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void **)&g->misc_h, sizeof(int) * 320, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void **)&g->part_h, sizeof(u64) * 768, hipHostMallocDefault) != hipSuccess)
+    return GU_ERR_NOMEM;
+  std::memset(g->ctrl_h, 0, sizeof(Ctrl));
+  std::memset(g->rowtab_h, 0, sizeof(void *) * 1024);
+  if (dalloc(&g->rowtab, 1024) != hipSuccess || dalloc(&g->ecc, 256) != hipSuccess)
+    return GU_ERR_NOMEM;
+  return GU_OK;
+}
+}  // namespace
+
+extern "C" {
+
 int gu_graph_create(const int32_t *R, const int32_t *C, const int32_t *F, int32_t n, int64_t m,
                     int32_t directed, gu_graph_t *out) {
   if (!out || !R || n < 0 || m < 0 || m >= INT32_MAX || (m > 0 && !C)) return GU_ERR_INVALID;
@@ -1469,12 +1666,7 @@ int gu_graph_create(const int32_t *R, const int32_t *C, const int32_t *F, int32_
       rc = GU_ERR_NOMEM;
       break;
     }
-    if (hipHostMalloc((void **)&g->ctrl_h, sizeof(Ctrl), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&g->srcs_h, sizeof(int) * 256, hipHostMallocDefault) != hipSuccess) {
-      rc = GU_ERR_NOMEM;
-      break;
-    }
-    std::memset(g->ctrl_h, 0, sizeof(Ctrl));
+    if ((rc = alloc_handle_bufs(g)) != GU_OK) break;
     if (hipMemcpy(g->R, R, sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice) != hipSuccess ||
         (m > 0 && hipMemcpy(g->C, C, sizeof(int) * (size_t)m, hipMemcpyHostToDevice) != hipSuccess)) {
       rc = GU_ERR_HIP;
@@ -1519,18 +1711,65 @@ int gu_graph_create(const int32_t *R, const int32_t *C, const int32_t *F, int32_
 
 int gu_graph_destroy(gu_graph_t g) {
   if (!g) return GU_ERR_INVALID;
+  if (g->views.load() != 0) return GU_ERR_INVALID;  // shared handles still use it
   g->ws.release();
-  if (g->R) (void)hipFree(g->R);
-  if (g->C) (void)hipFree(g->C);
-  if (g->F) (void)hipFree(g->F);
-  g->pu.release();
-  g->ou.release();
-  if (g->RT) (void)hipFree(g->RT);
-  if (g->CT) (void)hipFree(g->CT);
+  g->pws.release();
+  if (!g->root) {
+    if (g->R) (void)hipFree(g->R);
+    if (g->C) (void)hipFree(g->C);
+    if (g->F) (void)hipFree(g->F);
+    g->pu.release();
+    g->ou.release();
+    if (g->RT) (void)hipFree(g->RT);
+    if (g->CT) (void)hipFree(g->CT);
+  } else {
+    g->root->views.fetch_sub(1);
+  }
   if (g->ctrl) (void)hipFree(g->ctrl);
-  if (g->ctrl_h) (void)hipHostFree(g->ctrl_h);
-  if (g->srcs_h) (void)hipHostFree(g->srcs_h);
+  if (g->rowtab) (void)hipFree(g->rowtab);
+  if (g->ecc) (void)hipFree(g->ecc);
+  void *pinned[] = {g->ctrl_h, g->srcs_h, g->rowtab_h, g->ecc_h, g->misc_h, g->part_h};
+  for (void *p : pinned)
+    if (p) (void)hipHostFree(p);
   delete g;
+  return GU_OK;
+}
+
+int gu_graph_share(gu_graph_t g, gu_graph_t *out) {
+  if (!g || !out) return GU_ERR_INVALID;
+  *out = nullptr;
+  gu_graph *root = g->root ? g->root : g;
+  gu_graph *v = new (std::nothrow) gu_graph;
+  if (!v) return GU_ERR_NOMEM;
+  v->n = root->n;
+  v->m = root->m;
+  v->directed = root->directed;
+  v->R = root->R;
+  v->C = root->C;
+  v->F = root->F;
+  v->cus = root->cus;
+  v->device = root->device;
+  v->max_deg = root->max_deg;
+  v->avg_deg = root->avg_deg;
+  v->RT = root->RT;
+  v->CT = root->CT;
+  v->pu = root->pu;
+  v->pu.owned = false;
+  v->ou = root->ou;
+  v->ou.owned = false;
+  v->tuned_variant = root->tuned_variant;
+  v->tuned_direction_alpha = root->tuned_direction_alpha;
+  v->root = root;
+  root->views.fetch_add(1);
+  int rc = GU_OK;
+  if (hipSetDevice(root->device) != hipSuccess) rc = GU_ERR_HIP;
+  if (rc == GU_OK && dalloc(&v->ctrl, 1) != hipSuccess) rc = GU_ERR_NOMEM;
+  if (rc == GU_OK) rc = alloc_handle_bufs(v);
+  if (rc != GU_OK) {
+    gu_graph_destroy(v);
+    return rc;
+  }
+  *out = v;
   return GU_OK;
 }
 
@@ -1551,21 +1790,74 @@ int gu_graph_device_bytes(gu_graph_t g, int64_t *bytes) {
   return GU_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// Row i of a contiguous [nsrc x n] buffer.
+template <class T>
+std::vector<T *> contiguous_rows(T *base, int32_t nsrc, int32_t n) {
+  std::vector<T *> r;
+  if (base && nsrc > 0) {
+    r.resize((size_t)nsrc);
+    for (int32_t i = 0; i < nsrc; i++) r[(size_t)i] = base + (size_t)i * (size_t)n;
+  }
+  return r;
+}
+}  // namespace
+
+extern "C" {
+
 int gu_multi_search(gu_graph_t g, const int32_t *sources, int32_t nsrc, int32_t *dist,
                     uint64_t *sigma, const gu_config *cfg, gu_stats *stats) {
-  return run_job(g, sources, nsrc, JOB_BFS, dist, sigma, nullptr, nullptr, nullptr, cfg, stats);
+  if (!g || nsrc < 0) return GU_ERR_INVALID;
+  std::vector<int32_t *> dr = contiguous_rows(dist, nsrc, g->n);
+  std::vector<u64 *> sr = contiguous_rows((u64 *)sigma, nsrc, g->n);
+  HostRows hr;
+  if (dist) hr.dist = dr.data();
+  if (sigma) hr.sigma = sr.data();
+  return run_job(g, sources, nsrc, JOB_BFS, hr, nullptr, cfg, stats);
+}
+
+int gu_multi_search_rows(gu_graph_t g, const int32_t *sources, int32_t nsrc,
+                         int32_t *const *dist_rows, uint64_t *const *sigma_rows, int32_t *ecc,
+                         const gu_config *cfg, gu_stats *stats) {
+  if (!g || nsrc < 0) return GU_ERR_INVALID;
+  HostRows hr;
+  hr.dist = dist_rows;
+  hr.sigma = (u64 *const *)sigma_rows;
+  hr.ecc = ecc;
+  // eccentricities without rows: the diameter job (no transposes)
+  const Job job = (!dist_rows && !sigma_rows && ecc) ? JOB_DIAM : JOB_BFS;
+  return run_job(g, sources, nsrc, job, hr, nullptr, cfg, stats);
 }
 
 int gu_betweenness(gu_graph_t g, const int32_t *sources, int32_t nsrc, float *delta, double *bc,
                    const gu_config *cfg, gu_stats *stats) {
-  return run_job(g, sources, nsrc, JOB_BC, nullptr, nullptr, delta, bc, nullptr, cfg, stats);
+  if (!g || nsrc < 0) return GU_ERR_INVALID;
+  std::vector<float *> xr = contiguous_rows(delta, nsrc, g->n);
+  std::vector<double *> bt;
+  if (bc) bt.assign((size_t)std::max(nsrc, 0), bc);
+  HostRows hr;
+  if (delta) hr.delta = xr.data();
+  if (bc) hr.bc = bt.data();
+  return run_job(g, sources, nsrc, JOB_BC, hr, nullptr, cfg, stats);
+}
+
+int gu_betweenness_rows(gu_graph_t g, const int32_t *sources, int32_t nsrc,
+                        float *const *delta_rows, double *const *bc_targets,
+                        const gu_config *cfg, gu_stats *stats) {
+  if (!g || nsrc < 0) return GU_ERR_INVALID;
+  HostRows hr;
+  hr.delta = delta_rows;
+  hr.bc = bc_targets;
+  return run_job(g, sources, nsrc, JOB_BC, hr, nullptr, cfg, stats);
 }
 
 int gu_diameter(gu_graph_t g, const int32_t *sources, int32_t nsrc, int32_t *diameter,
                 const gu_config *cfg, gu_stats *stats) {
   if (!diameter) return GU_ERR_INVALID;
   *diameter = 0;
-  return run_job(g, sources, nsrc, JOB_DIAM, nullptr, nullptr, nullptr, nullptr, diameter, cfg, stats);
+  return run_job(g, sources, nsrc, JOB_DIAM, HostRows(), diameter, cfg, stats);
 }
 
 int gu_lbs(const int32_t *counts, int64_t S, int32_t *scan, int32_t *result, int64_t result_cap,

@@ -39,7 +39,8 @@ extern "C" {
 #define GU_ERR_WEIGHTED (-6)    /* weighted METIS header (parse.cpp:164-173) */
 #define GU_ERR_IO (-7)          /* file cannot be opened */
 #define GU_ERR_NODEVICE (-8)    /* no HIP device */
-#define GU_ERR_CAPACITY (-9)    /* output buffer too small */
+#define GU_ERR_CAPACITY (-9)    /* output buffer too small / over budget */
+#define GU_ERR_TIMEOUT (-10)    /* a wait ran out of time */
 
 /* Expansion strategies for top-down levels.  Each mirrors one reference
  * kernel; all produce identical distances and path counts. */
@@ -60,7 +61,8 @@ extern "C" {
  * them (best when sources share levels: low-diameter graphs).  PER_SOURCE
  * runs one workgroup per source with the whole level loop inside one launch
  * (best for high-diameter, bounded-degree graphs).  AUTO picks from the
- * degree profile.  Brandes betweenness always uses BATCHED. */
+ * degree profile (PER_SOURCE when max degree <= 64 and mean degree <= 8).
+ * Both engines run every job (search, path counts, Brandes, diameter). */
 #define GU_ENGINE_AUTO 0
 #define GU_ENGINE_BATCHED 1
 #define GU_ENGINE_PER_SOURCE 2
@@ -143,6 +145,12 @@ int gu_graph_arrays(gu_graph_t g, const int32_t **R_dev, const int32_t **C_dev,
                     const int32_t **F_dev, int32_t *n, int64_t *m);
 /* Bytes of HBM the handle holds (graph + workspace). */
 int gu_graph_device_bytes(gu_graph_t g, int64_t *bytes);
+/* A second handle on the same resident graph (device_graph.h:7-22 has one
+ * owner; this adds concurrent users): it reads g's device arrays and owns
+ * its own search workspace, so two handles can run searches on two streams
+ * at once.  Destroy every shared handle before g (gu_graph_destroy on g
+ * returns GU_ERR_INVALID while shared handles exist). */
+int gu_graph_share(gu_graph_t g, gu_graph_t *out);
 
 /* ---- multi_search ---------------------------------------------------- */
 /* Multi-source BFS.  Replaces multi_search_{linear_atomics,edge_parallel,
@@ -156,6 +164,18 @@ int gu_multi_search(gu_graph_t g, const int32_t *sources, int32_t nsrc,
                     int32_t *dist_dev, uint64_t *sigma_dev,
                     const gu_config *cfg, gu_stats *stats);
 
+/* gu_multi_search with a device row pointer per source (host arrays of nsrc
+ * pointers to n-entry device rows; a NULL array means not wanted; NULL
+ * entries skip distance rows but path counts need every row), plus an
+ * optional host array ecc[nsrc] receiving each source's eccentricity (its
+ * largest finite distance, diameter_sampling's per-source maximum).  With
+ * both row arrays NULL only eccentricities are computed.  Lets one launch
+ * serve sources of several queries whose outputs live apart. */
+int gu_multi_search_rows(gu_graph_t g, const int32_t *sources, int32_t nsrc,
+                         int32_t *const *dist_rows_dev,
+                         uint64_t *const *sigma_rows_dev, int32_t *ecc,
+                         const gu_config *cfg, gu_stats *stats);
+
 /* Brandes betweenness over the given sources.  Replaces
  * betweenness_centrality_setup (betweenness_centrality.cuh:14).
  * delta_dev: [nsrc x n] float dependency rows (the reference's delta_h),
@@ -164,6 +184,18 @@ int gu_multi_search(gu_graph_t g, const int32_t *sources, int32_t nsrc,
 int gu_betweenness(gu_graph_t g, const int32_t *sources, int32_t nsrc,
                    float *delta_dev, double *bc_dev, const gu_config *cfg,
                    gu_stats *stats);
+
+/* gu_betweenness with per-source outputs: delta_rows_dev[i] (or NULL) gets
+ * source i's dependency row; source i's dependencies are added into the
+ * [n] double vector bc_targets_dev[i] (or not accumulated when NULL).
+ * Within one launch, consecutive sources with the same target are summed
+ * in source order and added once, so a query's sources packed together
+ * accumulate as in gu_betweenness.  Two calls running at once must not
+ * share a target. */
+int gu_betweenness_rows(gu_graph_t g, const int32_t *sources, int32_t nsrc,
+                        float *const *delta_rows_dev,
+                        double *const *bc_targets_dev, const gu_config *cfg,
+                        gu_stats *stats);
 
 /* Largest finite distance from any of the sources.  Replaces
  * diameter_sampling_setup (diameter_sampling.cuh:13). */
@@ -179,6 +211,89 @@ int gu_diameter(gu_graph_t g, const int32_t *sources, int32_t nsrc,
 int gu_lbs(const int32_t *counts_dev, int64_t S, int32_t *scan_dev,
            int32_t *result_dev, int64_t result_cap, int64_t *W_out,
            const gu_config *cfg, gu_stats *stats);
+
+/* ---- query server (no counterpart in the reference) -------------------- */
+/* A continuous-batching server over resident graphs.  Clients submit BFS,
+ * Brandes BC and diameter-sampling queries of any number of sources from
+ * any thread; the server admits them in arrival order while their outputs
+ * fit an HBM budget, packs the sources of admitted queries of one graph and
+ * kind into shared multi-source launches, and keeps up to `slots` launches
+ * in flight (one stream and one shared graph handle per slot).  Outputs stay
+ * in HBM in a block the query holds until gu_server_release.  A query's
+ * distances, path counts and diameter are identical whatever it was packed
+ * with; its BC differs only in the grouping of FP64 sums. */
+typedef struct gu_server *gu_server_t;
+
+#define GU_QUERY_BFS 0       /* distance rows (+ path-count rows) */
+#define GU_QUERY_BC 1        /* BC vector over the query's sources */
+#define GU_QUERY_DIAMETER 2  /* max eccentricity over the sources */
+#define GU_QF_SIGMA 1        /* BFS: also path counts */
+#define GU_QF_DELTA 2        /* BC: also per-source dependency rows */
+
+typedef struct {
+  int64_t hbm_budget;  /* bytes for query outputs (0 = a quarter of free HBM) */
+  int32_t slots;       /* launches in flight (0 = 4) */
+  int32_t max_pack;    /* sources per launch, <= 256 (0 = 256) */
+  int32_t variant;     /* GU_VARIANT_* for every launch (0 = AUTO) */
+  int32_t engine;      /* GU_ENGINE_* for every launch (0 = AUTO) */
+} gu_server_config;
+
+typedef struct {
+  int32_t graph;            /* id from gu_server_add_graph */
+  int32_t kind;             /* GU_QUERY_* */
+  int32_t nsrc;             /* >= 1 */
+  int32_t flags;            /* GU_QF_* */
+  const int32_t *sources;   /* host, copied by gu_server_submit */
+} gu_query;
+
+typedef struct {
+  int32_t status;           /* GU_OK or the launch's error */
+  int32_t kind;
+  int32_t nsrc;
+  int32_t diameter;         /* GU_QUERY_DIAMETER */
+  int32_t launches;         /* launches that carried its sources */
+  int32_t reserved;
+  int32_t *dist_dev;        /* [nsrc x n] (BFS) */
+  uint64_t *sigma_dev;      /* [nsrc x n] (BFS with GU_QF_SIGMA) */
+  float *delta_dev;         /* [nsrc x n] (BC with GU_QF_DELTA) */
+  double *bc_dev;           /* [n] (BC) */
+  double submit_ms;         /* server clock (ms since creation) */
+  double admit_ms;
+  double first_launch_ms;
+  double done_ms;
+  int64_t hbm_bytes;        /* arena bytes held until release */
+} gu_query_result;
+
+typedef struct {
+  int64_t submitted, completed, rejected;
+  int64_t launches, sources;
+  int64_t hbm_in_use, hbm_peak, hbm_budget;
+  int32_t waiting;          /* submitted, not yet admitted */
+  int32_t active;           /* admitted, not finished */
+} gu_server_stats;
+
+/* Creates the server on the current device: the output arena, one stream
+ * per slot and the worker threads. */
+int gu_server_create(const gu_server_config *cfg, gu_server_t *out);
+/* Registers a resident graph (the caller keeps ownership and destroys it
+ * after the server). */
+int gu_server_add_graph(gu_server_t s, gu_graph_t g, int32_t *graph_id);
+/* Queues a query; returns at once.  A query whose outputs exceed the whole
+ * budget finishes immediately with status GU_ERR_CAPACITY. */
+int gu_server_submit(gu_server_t s, const gu_query *q, int64_t *ticket);
+/* Waits for one query (timeout_ms < 0: no limit; GU_ERR_TIMEOUT). */
+int gu_server_wait(gu_server_t s, int64_t ticket, int32_t timeout_ms,
+                   gu_query_result *res);
+/* Waits for the next finished query not yet reported. */
+int gu_server_next_done(gu_server_t s, int32_t timeout_ms, int64_t *ticket,
+                        gu_query_result *res);
+int gu_server_poll(gu_server_t s, int64_t ticket, int32_t *done);
+/* Frees a finished query's outputs (the device pointers become invalid). */
+int gu_server_release(gu_server_t s, int64_t ticket);
+int gu_server_now_ms(gu_server_t s, double *ms);
+int gu_server_get_stats(gu_server_t s, gu_server_stats *out);
+/* Stops the workers after their current launches and frees the server. */
+int gu_server_destroy(gu_server_t s);
 
 /* ---- host graph I/O and generators (parse.h:15-40) -------------------- */
 /* Host arrays returned by these calls are released with gu_host_free. */

@@ -34,7 +34,7 @@ def test_library_exports_every_declared_symbol(gu):
 
 
 def test_status_strings(gu):
-    codes = [0, -1, -2, -3, -4, -5, -6, -7, -8, -9]
+    codes = [0, -1, -2, -3, -4, -5, -6, -7, -8, -9, -10]
     msgs = [gu.lib().gu_strerror(c) for c in codes]
     assert all(isinstance(m, bytes) and m for m in msgs)
     assert len(set(msgs)) == len(msgs)
@@ -44,11 +44,10 @@ def test_status_strings(gu):
 def test_struct_layouts_match_header(gu, tmp_path):
     """ctypes mirrors of gu_config / gu_stats / gu_device_props have the C
     sizes and field offsets (compiled here with gcc against the header)."""
-    fields = {
-        "gu_config": [f[0] for f in gu.Config._fields_],
-        "gu_stats": [f[0] for f in gu.Stats._fields_],
-        "gu_device_props": [f[0] for f in gu.DeviceProps._fields_],
-    }
+    classes = {"gu_config": gu.Config, "gu_stats": gu.Stats, "gu_device_props": gu.DeviceProps,
+               "gu_server_config": gu.ServerConfig, "gu_query": gu.Query,
+               "gu_query_result": gu.QueryResult, "gu_server_stats": gu.ServerStats}
+    fields = {k: [f[0] for f in c._fields_] for k, c in classes.items()}
     src = ['#include <stdio.h>', '#include <stddef.h>', '#include "graph_utils_amd.h"',
            "int main(void) {"]
     for st, fl in fields.items():
@@ -64,8 +63,7 @@ def test_struct_layouts_match_header(gu, tmp_path):
     out = dict(line.split() for line in subprocess.run([str(exe)], check=True,
                                                          capture_output=True, text=True)
                .stdout.splitlines())
-    for st, cls in (("gu_config", gu.Config), ("gu_stats", gu.Stats),
-                    ("gu_device_props", gu.DeviceProps)):
+    for st, cls in classes.items():
         assert int(out[st]) == ctypes.sizeof(cls), st
         for f in fields[st]:
             assert int(out[st + "." + f]) == getattr(cls, f).offset, (st, f)
