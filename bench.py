@@ -112,6 +112,9 @@ def bench_grid(args, gu, props):
     cfg = gu.make_config(args.variant, direction=args.direction, batch_width=args.width or 256,
                          alpha=args.alpha, beta=args.beta, engine=args.engine)
     with gu.DeviceGraph(hg) as dg:
+        tuned = None
+        if not args.no_tune and args.variant == "auto" and args.engine == 0:
+            tuned = dg.autotune(hg.sample_sources(k, seed=3998), gu.QUERY_BFS)
         for i in range(args.warmup):
             dg.multi_search_device(batches[i], dist, sig, cfg)
         gu.synchronize()
@@ -140,6 +143,7 @@ def bench_grid(args, gu, props):
            "engine": stats[-1].engine_used,
            "generator_seconds": round(gen_s, 2), "device": props["name"], "arch": props["arch"]},
           {"levels_per_step": levels, "us_per_level": round(dev_ms / args.steps / levels * 1e3, 2),
+           "autotuned": tuned, "algorithmic_bytes_per_step": int(alg / args.steps),
            "roofline": {"bound": "hbm", "achieved": round(alg / (dev_ms / 1e3) / 1e9, 1),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(alg / (dev_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
@@ -160,6 +164,9 @@ def bench_bc(args, gu, props):
     bc = gu.DeviceBuffer(8 * n)
     res = {}
     with gu.DeviceGraph(hg) as dg:
+        tuned = None
+        if not args.no_tune and args.variant == "auto":
+            tuned = dg.autotune(src[:256], gu.QUERY_BC)
         for width in (64, 128, 256):
             cfg = gu.make_config(args.variant, batch_width=width)
             bc.zero()
@@ -197,7 +204,7 @@ def bench_bc(args, gu, props):
           {"workload": "R-MAT s22 ef16, 1024 sampled non-isolated sources per step", "n": n,
            "m_directed": hg.m, "generator_seconds": round(gen_s, 2), "device": props["name"],
            "arch": props["arch"]},
-          {"batch_width": best,
+          {"batch_width": best, "autotuned": tuned,
            "ms_by_width": {str(w): round(res[w][0] * 1e3, 3) for w in res},
            "levels_last_batch": st.max_level + 1, "bc_check": check})
 
@@ -357,8 +364,13 @@ def bench_serve(args, gu, props):
     qs = _serve_queries(gu, hgs, nq, seed=4242)
     budget = args.budget_gb << 30
     dgs = [gu.DeviceGraph(h) for h in hgs]
-    unit = {}
+    unit, tuned = {}, {}
     try:
+        if not args.no_tune:
+            for name, h, dg in zip(("rmat18", "rmat20", "grid1024"), hgs, dgs):
+                smp = h.sample_sources(256, seed=17)
+                tuned[name] = {"bfs": dg.autotune(smp, gu.QUERY_BFS), "bc": dg.autotune(smp, gu.QUERY_BC)}
+            print("serve: tuned %s" % json.dumps(tuned), file=sys.stderr, flush=True)
         # device time of one launch per graph and kind, 1 and 256 sources
         for name, h, dg in zip(("rmat18", "rmat20", "grid1024"), hgs, dgs):
             src = h.sample_sources(256, seed=7)
@@ -392,6 +404,7 @@ def bench_serve(args, gu, props):
             dg.close()
     served_qps = nq / (mk_l / 1e3)
     res = {
+        "autotuned": tuned,
         "launch_ms": unit,
         "saturated": {"served_qps": round(sat_qps, 2), "p50_ms": _pct(lat_s, 50), "p99_ms": _pct(lat_s, 99),
                       "server": st_s},
@@ -437,6 +450,7 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-variants", action="store_true", help="skip the per-variant sweep")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (degree-profile defaults)")
     ap.add_argument("--bc-check", type=int, default=0,
                     help="bc workload: compare BC over this many sources with the FP64 CPU oracle")
     ap.add_argument("--calib-bytes", type=int, default=0,
@@ -473,6 +487,9 @@ def main():
     sig = gu.DeviceBuffer(8 * k * n)
     cfg = gu.make_config(args.variant, alpha=args.alpha, beta=args.beta)
     with gu.DeviceGraph(hg) as dg:
+        tuned = None
+        if not args.no_tune and args.variant == "auto":
+            tuned = dg.autotune(hg.sample_sources(k, seed=999), gu.QUERY_BFS)
         for i in range(args.warmup):
             dg.multi_search_device(batches[i], dist, sig, cfg)
         gu.synchronize()
@@ -538,6 +555,7 @@ def main():
         "topdown_levels_last": stats[-1].topdown_levels,
         "bottomup_levels_last": stats[-1].bottomup_levels,
         "variant_used": stats[-1].variant_used,
+        "autotuned": tuned,
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 1),

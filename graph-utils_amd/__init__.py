@@ -61,6 +61,7 @@ EXPORTS = (
     "gu_multi_search_rows", "gu_betweenness_rows", "gu_server_create", "gu_server_add_graph",
     "gu_server_submit", "gu_server_wait", "gu_server_next_done", "gu_server_poll",
     "gu_server_release", "gu_server_now_ms", "gu_server_get_stats", "gu_server_destroy",
+    "gu_autotune",
 )
 
 QUERY_BFS, QUERY_BC, QUERY_DIAMETER = 0, 1, 2
@@ -158,6 +159,20 @@ class ServerStats(ctypes.Structure):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}
 
 
+class TuneResult(ctypes.Structure):
+    _fields_ = [("engine", ctypes.c_int32), ("variant", ctypes.c_int32),
+                ("batch_width", ctypes.c_int32), ("wg_size", ctypes.c_int32),
+                ("alpha", ctypes.c_int32), ("trials", ctypes.c_int32), ("sources", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("ms_default", ctypes.c_double),
+                ("ms_tuned", ctypes.c_double)]
+
+    def as_dict(self):
+        d = {f[0]: getattr(self, f[0]) for f in self._fields_ if f[0] != "reserved"}
+        d["ms_default"] = round(d["ms_default"], 4)
+        d["ms_tuned"] = round(d["ms_tuned"], 4)
+        return d
+
+
 _I32P = ctypes.POINTER(ctypes.c_int32)
 _I32PP = ctypes.POINTER(_I32P)
 _I64P = ctypes.POINTER(ctypes.c_int64)
@@ -229,6 +244,8 @@ def lib():
         "gu_server_now_ms": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double)]),
         "gu_server_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(ServerStats)]),
         "gu_server_destroy": (ctypes.c_int, [_P]),
+        "gu_autotune": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.POINTER(TuneResult)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -572,6 +589,14 @@ class DeviceGraph:
                                          ctypes.byref(cfg), ctypes.byref(st)),
                "gu_betweenness_rows")
         return st
+
+    def autotune(self, sources, kind=0):
+        """gu_autotune for kind QUERY_BFS (searches) or QUERY_BC (Brandes);
+        returns the chosen shape as a dict."""
+        src = np.ascontiguousarray(sources, dtype=np.int32)
+        r = TuneResult()
+        _check(lib().gu_autotune(self.h, _ptr(src), len(src), kind, ctypes.byref(r)), "gu_autotune")
+        return r.as_dict()
 
     def share(self):
         """A second handle on this resident graph with its own workspace

@@ -873,8 +873,13 @@ struct gu_graph {
   int *RT = nullptr, *CT = nullptr;  // in-edge CSR of a directed graph
   Units pu;                   // work units over in-edges (pulls)
   Units ou;                   // work units over out-edges (Brandes backward)
-  int tuned_variant = -1;     // autotuner cache
+  int tuned_variant = -1;     // autotuner cache (searches)
   int tuned_direction_alpha = 0;
+  // Autotuned launch shape per job family: [0] searches / diameter,
+  // [1] Brandes.  Zero fields mean "not tuned".
+  struct Tuned {
+    int engine = 0, variant = -1, width = 0, wg = 0, alpha = 0;
+  } tuned[2];
   // Output row tables of one launch (device, with a pinned host mirror):
   // [0,256) distance rows, [256,512) path-count rows, [512,768) dependency
   // rows, [768,1024) Brandes accumulation targets.
@@ -1398,6 +1403,18 @@ int run_job(gu_graph *g, const int32_t *sources, int32_t nsrc, Job job, const Ho
   if (!g) return GU_ERR_INVALID;
   gu_config cfg{};
   if (cfgp) cfg = *cfgp;
+  {
+    // fields left at "auto" take the autotuned shape of this job family
+    const gu_graph::Tuned &t = g->tuned[job == JOB_BC ? 1 : 0];
+    if (cfg.engine == GU_ENGINE_AUTO) cfg.engine = t.engine;
+    if (cfg.variant == GU_VARIANT_AUTO && t.variant >= 0) cfg.variant = t.variant;
+    if (cfg.wg_size == 0) cfg.wg_size = t.wg;
+    if (cfg.alpha == 0) cfg.alpha = t.alpha;
+    if (cfg.batch_width == 0 && t.width > 0) {
+      const int w = nsrc <= 64 ? 64 : (nsrc <= 128 ? 128 : 256);
+      cfg.batch_width = std::min(w, t.width);
+    }
+  }
   if (cfg.direction < 0 || cfg.direction > 2) return GU_ERR_INVALID;
   if (cfg.variant < 0 || cfg.variant > GU_VARIANT_AUTO) return GU_ERR_INVALID;
   if (cfg.engine < 0 || cfg.engine > GU_ENGINE_PER_SOURCE) return GU_ERR_INVALID;
@@ -1759,6 +1776,8 @@ int gu_graph_share(gu_graph_t g, gu_graph_t *out) {
   v->ou.owned = false;
   v->tuned_variant = root->tuned_variant;
   v->tuned_direction_alpha = root->tuned_direction_alpha;
+  v->tuned[0] = root->tuned[0];
+  v->tuned[1] = root->tuned[1];
   v->root = root;
   root->views.fetch_add(1);
   int rc = GU_OK;
@@ -1858,6 +1877,128 @@ int gu_diameter(gu_graph_t g, const int32_t *sources, int32_t nsrc, int32_t *dia
   if (!diameter) return GU_ERR_INVALID;
   *diameter = 0;
   return run_job(g, sources, nsrc, JOB_DIAM, HostRows(), diameter, cfg, stats);
+}
+
+int gu_autotune(gu_graph_t g, const int32_t *sources, int32_t nsrc, int32_t kind,
+                gu_tune_result *out) {
+  if (!g || !sources || nsrc <= 0 || (kind != GU_QUERY_BFS && kind != GU_QUERY_BC)) return GU_ERR_INVALID;
+  for (int i = 0; i < nsrc; i++)
+    if (sources[i] < 0 || sources[i] >= g->n) return GU_ERR_RANGE;
+  const int fam = kind == GU_QUERY_BC ? 1 : 0;
+  const int k = std::min(nsrc, 256);
+  const size_t n = (size_t)g->n;
+  GU_HIP(hipSetDevice(g->device));
+  // outputs of the trials: distance and path-count rows, or a BC vector
+  int32_t *dist = nullptr;
+  u64 *sig = nullptr;
+  double *bc = nullptr;
+  int rc = GU_OK;
+  if (fam == 0) {
+    if (dalloc(&dist, (size_t)k * n) != hipSuccess || dalloc(&sig, (size_t)k * n) != hipSuccess)
+      rc = GU_ERR_NOMEM;
+  } else if (dalloc(&bc, n) != hipSuccess) {
+    rc = GU_ERR_NOMEM;
+  }
+  gu_graph::Tuned saved = g->tuned[fam];
+  g->tuned[fam] = gu_graph::Tuned();
+  int trials = 0;
+  auto trial = [&](const gu_config &c, double *ms_out) -> int {
+    double best = 1e300;
+    for (int rep = 0; rep < 2; rep++) {
+      gu_stats st{};
+      int r = fam == 0 ? gu_multi_search(g, sources, k, dist, (uint64_t *)sig, &c, &st)
+                       : gu_betweenness(g, sources, k, nullptr, bc, &c, &st);
+      if (r) return r;
+      best = std::min(best, st.device_ms);
+    }
+    trials++;
+    *ms_out = best;
+    return GU_OK;
+  };
+  gu_config base{};
+  base.variant = GU_VARIANT_AUTO;
+  double t_default = 0, t_best = 1e300, t = 0;
+  gu_config best = base;
+  if (rc == GU_OK) rc = trial(base, &t_default);  // warms the workspace too
+  t_best = t_default;
+  best.engine = resolve_engine(g, base, fam == 1 ? JOB_BC : JOB_BFS);
+  // engine: per-source is a candidate on graphs without large hubs
+  if (rc == GU_OK && g->max_deg <= 1024) {
+    gu_config c = base;
+    c.engine = best.engine == GU_ENGINE_PER_SOURCE ? GU_ENGINE_BATCHED : GU_ENGINE_PER_SOURCE;
+    if ((rc = trial(c, &t)) == GU_OK && t < t_best) {
+      t_best = t;
+      best = c;
+    }
+  }
+  if (rc == GU_OK && best.engine == GU_ENGINE_BATCHED) {
+    best.variant = resolve_variant(g, base);
+    // top-down expander
+    const int vars[] = {V_SHUFFLE, V_WARP, V_CTA_WARP, V_SCAN, V_EDGE_PARALLEL};
+    for (int v : vars) {
+      gu_config c = best;
+      c.variant = v;
+      if ((rc = trial(c, &t)) != GU_OK) break;
+      if (t < t_best) {
+        t_best = t;
+        best = c;
+      }
+    }
+    // sources per launch, workgroup size, direction threshold
+    const int widths[] = {64, 128, 256}, wgs[] = {256, 512, 1024}, alphas[] = {4, 16, 64};
+    for (int w : widths) {
+      if (rc || w > ((k + 63) / 64) * 64) break;
+      gu_config c = best;
+      c.batch_width = w;
+      if ((rc = trial(c, &t)) == GU_OK && t < t_best) {
+        t_best = t;
+        best = c;
+      }
+    }
+    for (int w : wgs) {
+      if (rc) break;
+      gu_config c = best;
+      c.wg_size = w;
+      if ((rc = trial(c, &t)) == GU_OK && t < t_best) {
+        t_best = t;
+        best = c;
+      }
+    }
+    for (int a : alphas) {
+      if (rc) break;
+      gu_config c = best;
+      c.alpha = a;
+      if ((rc = trial(c, &t)) == GU_OK && t < t_best) {
+        t_best = t;
+        best = c;
+      }
+    }
+  }
+  if (dist) (void)hipFree(dist);
+  if (sig) (void)hipFree(sig);
+  if (bc) (void)hipFree(bc);
+  if (rc != GU_OK) {
+    g->tuned[fam] = saved;
+    return rc;
+  }
+  gu_graph::Tuned &tn = g->tuned[fam];
+  tn.engine = best.engine;
+  tn.variant = best.engine == GU_ENGINE_BATCHED ? best.variant : -1;
+  tn.width = best.batch_width;
+  tn.wg = best.wg_size;
+  tn.alpha = best.alpha;
+  if (out) {
+    out->engine = tn.engine;
+    out->variant = tn.variant >= 0 ? tn.variant : GU_VARIANT_AUTO;
+    out->batch_width = tn.width;
+    out->wg_size = tn.wg;
+    out->alpha = tn.alpha;
+    out->trials = trials;
+    out->ms_default = t_default;
+    out->ms_tuned = t_best;
+    out->sources = k;
+  }
+  return GU_OK;
 }
 
 int gu_lbs(const int32_t *counts, int64_t S, int32_t *scan, int32_t *result, int64_t result_cap,

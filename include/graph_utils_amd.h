@@ -202,6 +202,31 @@ int gu_betweenness_rows(gu_graph_t g, const int32_t *sources, int32_t nsrc,
 int gu_diameter(gu_graph_t g, const int32_t *sources, int32_t nsrc,
                 int32_t *diameter, const gu_config *cfg, gu_stats *stats);
 
+/* ---- autotuner (no counterpart in the reference) ---------------------- */
+typedef struct {
+  int32_t engine;       /* GU_ENGINE_* chosen */
+  int32_t variant;      /* GU_VARIANT_* (AUTO when the per-source engine won) */
+  int32_t batch_width;  /* 0 = by source count */
+  int32_t wg_size;      /* 0 = 256 */
+  int32_t alpha;        /* 0 = default direction threshold */
+  int32_t trials;       /* configurations timed */
+  int32_t sources;      /* sources per trial */
+  int32_t reserved;
+  double ms_default;    /* device ms of the untuned configuration */
+  double ms_tuned;      /* device ms of the chosen one */
+} gu_tune_result;
+
+/* Times launches of the given sources (up to 256) for the job family `kind`
+ * (GU_QUERY_BFS: searches with path counts, also used by diameter;
+ * GU_QUERY_BC: Brandes) over engines, top-down expanders (shuffle, warp,
+ * CTA+warp, scan, edge-parallel), sources per launch, workgroup sizes and
+ * the top-down -> bottom-up threshold, starting from the degree-profile
+ * defaults, and keeps the fastest in the handle: later calls whose
+ * gu_config fields are left at auto / 0 use it (handles made afterwards by
+ * gu_graph_share inherit it).  Allocates trial outputs for the duration. */
+int gu_autotune(gu_graph_t g, const int32_t *sources, int32_t nsrc,
+                int32_t kind, gu_tune_result *out);
+
 /* ---- load_balanced_search (load_balanced_search.cuh:10-13) ----------- */
 /* Device-wide form of extract_edges_block: scan_dev[i] = exclusive prefix of
  * counts_dev; result_dev[k] = upper_bound(scan, k) - 1 for k < W, the last

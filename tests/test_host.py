@@ -46,7 +46,8 @@ def test_struct_layouts_match_header(gu, tmp_path):
     sizes and field offsets (compiled here with gcc against the header)."""
     classes = {"gu_config": gu.Config, "gu_stats": gu.Stats, "gu_device_props": gu.DeviceProps,
                "gu_server_config": gu.ServerConfig, "gu_query": gu.Query,
-               "gu_query_result": gu.QueryResult, "gu_server_stats": gu.ServerStats}
+               "gu_query_result": gu.QueryResult, "gu_server_stats": gu.ServerStats,
+               "gu_tune_result": gu.TuneResult}
     fields = {k: [f[0] for f in c._fields_] for k, c in classes.items()}
     src = ['#include <stdio.h>', '#include <stddef.h>', '#include "graph_utils_amd.h"',
            "int main(void) {"]

@@ -221,3 +221,32 @@ def test_server_rejects_oversized_query(gu, gpu):
         srv.release(t)
         srv.release(t2)
         assert srv.stats()["rejected"] == 1
+
+
+def test_autotune_keeps_results_exact(gu, orc, gpu):
+    announce("autotune rmat12 / grid32")
+    hg = gu.HostGraph.rmat(12, 16, seed=41)
+    src = hg.sample_sources(128, seed=42)
+    with gu.DeviceGraph(hg) as dg:
+        ref_d, ref_s, _ = dg.multi_search(src, True, True)
+        t = dg.autotune(src, gu.QUERY_BFS)
+        assert t["trials"] >= 10 and t["ms_tuned"] <= t["ms_default"]
+        d, s, st = dg.multi_search(src, True, True)
+        assert st.engine_used == t["engine"]
+        if t["engine"] == 1:
+            assert st.variant_used == t["variant"]
+        assert np.array_equal(d, ref_d) and np.array_equal(s, ref_s)
+        tb = dg.autotune(src[:64], gu.QUERY_BC)
+        _, bc, _ = dg.betweenness(src[:64], delta=False)
+        view = dg.share()
+        _, bc2, st2 = view.betweenness(src[:64], delta=False)
+        assert st2.engine_used == tb["engine"]
+        view.close()
+    ref = orc.bc_f64(hg.R, hg.C, src[:64])
+    nz = ref > 0
+    assert np.max(np.abs(bc[nz] - ref[nz]) / ref[nz]) <= 1e-5
+    assert np.array_equal(bc, bc2)
+    hg = gu.HostGraph.grid(32, 0.1, seed=43)
+    with gu.DeviceGraph(hg) as dg:
+        t = dg.autotune(hg.sample_sources(64, seed=44), gu.QUERY_BFS)
+        assert t["engine"] in (1, 2)
