@@ -871,6 +871,10 @@ struct gu_graph {
   int max_deg = 0;
   double avg_deg = 0;         // over vertices with out-degree > 0
   int *RT = nullptr, *CT = nullptr;  // in-edge CSR of a directed graph
+  // Cluster-ordered copy of the graph for the per-source engine (bounded-
+  // degree graphs): LR/LC in new ids, nof[old] = new id (device and host).
+  int *LR = nullptr, *LC = nullptr, *nof = nullptr;
+  std::vector<int32_t> nof_h;
   Units pu;                   // work units over in-edges (pulls)
   Units ou;                   // work units over out-edges (Brandes backward)
   int tuned_variant = -1;     // autotuner cache (searches)
@@ -880,9 +884,11 @@ struct gu_graph {
   struct Tuned {
     int engine = 0, variant = -1, width = 0, wg = 0, alpha = 0;
   } tuned[2];
-  // Output row tables of one launch (device, with a pinned host mirror):
-  // [0,256) distance rows, [256,512) path-count rows, [512,768) dependency
-  // rows, [768,1024) Brandes accumulation targets.
+  // Output row tables of one launch (device, with a pinned host mirror),
+  // eight tables of 256 entries: [0,256) distance rows, [256,512) path-count
+  // rows, [512,768) dependency rows, [768,1024) Brandes accumulation
+  // targets, [1024,2048) the caller's rows when the per-source engine works
+  // in cluster order.
   void **rowtab = nullptr;
   void **rowtab_h = nullptr;
   int *ecc = nullptr;         // device [256] per-source eccentricity
@@ -898,6 +904,11 @@ struct gu_graph {
 namespace {
 
 int pick_grid(const gu_graph *g, int per_cu) { return std::max(1, g->cus * per_cu); }
+
+constexpr int kRowTab = 2048;
+
+// Host copy of the cluster order (a shared handle reads its root's).
+const std::vector<int32_t> &nof_host(const gu_graph *g) { return g->root ? g->root->nof_h : g->nof_h; }
 
 // Work-unit shape: a vertex of degree above kHubDeg is a hub whose adjacency
 // is pulled in segments of kSeg entries by separate waves; every other
@@ -1253,46 +1264,67 @@ struct HostRows {
   int32_t *ecc = nullptr;
 };
 
-// Per-source engine over nsrc sources, kPsCap workgroups per launch.
+// Per-source engine over nsrc sources, kPsCap workgroups per launch.  On a
+// cluster-ordered graph the traversal runs in new ids on scratch rows and
+// k_ps_unpermute writes the caller's rows.
 int run_persource(gu_graph *g, const int *sources, int nsrc, const HostRows &hr, bool sigma,
                   gu_stats &s, int &dmax, hipStream_t st) {
-  int rc = ensure_pws(g, hr.dist == nullptr);
+  const bool rl = g->LR != nullptr;
+  int rc = ensure_pws(g, rl || hr.dist == nullptr);
   if (rc) return rc;
   PsWorkspace &p = g->pws;
   const size_t n = (size_t)g->n;
+  if (rl && sigma && !p.sscratch) {
+    GU_HIP(dalloc(&p.sscratch, (size_t)kPsCap * n));
+    p.bytes += (size_t)kPsCap * n * 8;
+  }
+  const int32_t *nof = rl ? nof_host(g).data() : nullptr;
   u64 *part = g->part_h;
   int *depth = g->misc_h + 64;
   int ovf_any = 0;
   for (int b0 = 0; b0 < nsrc; b0 += kPsCap) {
     const int nb = std::min(kPsCap, nsrc - b0);
-    std::memcpy(g->srcs_h, sources + b0, sizeof(int) * nb);
+    void **t = g->rowtab_h;
     for (int i = 0; i < nb; i++) {
+      g->srcs_h[i] = rl ? nof[sources[b0 + i]] : sources[b0 + i];
       int *dr = hr.dist ? hr.dist[b0 + i] : nullptr;
-      // a source whose distances are not wanted works in the scratch row
-      g->rowtab_h[i] = dr ? (void *)dr : (void *)(p.dscratch ? p.dscratch + (size_t)i * n : nullptr);
-      g->rowtab_h[256 + i] = sigma && hr.sigma ? (void *)hr.sigma[b0 + i] : nullptr;
+      u64 *sr = sigma && hr.sigma ? hr.sigma[b0 + i] : nullptr;
+      if (sigma && !sr) return GU_ERR_INVALID;  // path counts need every row
+      if (rl) {
+        t[i] = p.dscratch + (size_t)i * n;
+        t[256 + i] = sigma ? (void *)(p.sscratch + (size_t)i * n) : nullptr;
+        t[1024 + i] = dr;
+        t[1280 + i] = sr;
+      } else {
+        // a source whose distances are not wanted works in the scratch row
+        if (!dr) {
+          rc = ensure_pws(g, true);
+          if (rc) return rc;
+        }
+        t[i] = dr ? (void *)dr : (void *)(p.dscratch + (size_t)i * n);
+        t[256 + i] = sr;
+      }
     }
-    bool need_scratch = false;
-    for (int i = 0; i < nb; i++) need_scratch |= g->rowtab_h[i] == nullptr;
-    if (need_scratch) {  // some rows were null in a caller's table
-      rc = ensure_pws(g, true);
-      if (rc) return rc;
-      for (int i = 0; i < nb; i++)
-        if (!g->rowtab_h[i]) g->rowtab_h[i] = p.dscratch + (size_t)i * n;
-    }
-    bool sig_rows = sigma && hr.sigma;
-    for (int i = 0; sig_rows && i < nb; i++) sig_rows = g->rowtab_h[256 + i] != nullptr;
-    if (sigma && !sig_rows) return GU_ERR_INVALID;  // path counts need every row
     GU_HIP(hipMemcpyAsync(p.srcs, g->srcs_h, sizeof(int) * nb, hipMemcpyHostToDevice, st));
-    GU_HIP(hipMemcpyAsync(g->rowtab, g->rowtab_h, sizeof(void *) * 512, hipMemcpyHostToDevice, st));
+    GU_HIP(hipMemcpyAsync(g->rowtab, t, sizeof(void *) * kRowTab, hipMemcpyHostToDevice, st));
     if (b0 == 0) k_fill_i32<<<1, 64, 0, st>>>(p.ovf, 1, 0);
-    PsArgs a{g->R, g->C, g->n, p.srcs, (int *const *)g->rowtab,
+    PsArgs a{rl ? g->LR : g->R, rl ? g->LC : g->C, g->n, p.srcs, (int *const *)g->rowtab,
              sigma ? (u64 *const *)(g->rowtab + 256) : nullptr, p.queues, p.depth, p.part, p.ovf};
     if (sigma)
       k_persource<true><<<nb, 1024, 0, st>>>(a);
     else
       k_persource<false><<<nb, 1024, 0, st>>>(a);
     GU_HIP(hipGetLastError());
+    if (rl) {
+      const dim3 ug((unsigned)std::max(1, std::min(pick_grid(g, 1), (g->n + 255) / 256)), (unsigned)nb);
+      if (hr.dist)
+        k_ps_unpermute<int><<<ug, 256, 0, st>>>((int *const *)g->rowtab, (int *const *)(g->rowtab + 1024),
+                                                g->nof, g->n);
+      if (sigma)
+        k_ps_unpermute<u64><<<ug, 256, 0, st>>>((u64 *const *)(g->rowtab + 256),
+                                                (u64 *const *)(g->rowtab + 1280), g->nof, g->n);
+      GU_HIP(hipGetLastError());
+    }
     GU_HIP(hipMemcpyAsync(part, p.part, sizeof(u64) * 3 * nb, hipMemcpyDeviceToHost, st));
     GU_HIP(hipMemcpyAsync(depth, p.depth, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
     GU_HIP(hipMemcpyAsync(g->misc_h, p.ovf, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1324,7 +1356,8 @@ int run_persource_bc(gu_graph *g, const int *sources, int nsrc, const HostRows &
     GU_HIP(dalloc(&p.sscratch, (size_t)kPsCap * n));
     p.bytes += (size_t)kPsCap * n * 8;
   }
-  bool need_x = hr.delta == nullptr;
+  const bool rl = g->LR != nullptr;
+  bool need_x = rl || hr.delta == nullptr;
   for (int i = 0; !need_x && i < nsrc; i++) need_x = hr.delta[i] == nullptr;
   if (need_x && !p.xscratch) {
     GU_HIP(dalloc(&p.xscratch, (size_t)kPsCap * n));
@@ -1335,27 +1368,36 @@ int run_persource_bc(gu_graph *g, const int *sources, int nsrc, const HostRows &
   int ovf_any = 0;
   for (int b0 = 0; b0 < nsrc; b0 += kPsCap) {
     const int nb = std::min(kPsCap, nsrc - b0);
-    std::memcpy(g->srcs_h, sources + b0, sizeof(int) * nb);
+    const int32_t *nof = rl ? nof_host(g).data() : nullptr;
     void **t = g->rowtab_h;
     for (int i = 0; i < nb; i++) {
+      g->srcs_h[i] = rl ? nof[sources[b0 + i]] : sources[b0 + i];
       t[i] = p.dscratch + (size_t)i * n;
       t[256 + i] = p.sscratch + (size_t)i * n;
       float *x = hr.delta ? hr.delta[b0 + i] : nullptr;
-      t[512 + i] = x ? (void *)x : (void *)(p.xscratch + (size_t)i * n);
+      // in cluster order the dependencies live in scratch rows (new ids)
+      t[512 + i] = (x && !rl) ? (void *)x : (void *)(p.xscratch + (size_t)i * n);
       t[768 + i] = hr.bc ? (void *)hr.bc[b0 + i] : nullptr;
+      t[1024 + i] = x;
     }
     GU_HIP(hipMemcpyAsync(p.srcs, g->srcs_h, sizeof(int) * nb, hipMemcpyHostToDevice, st));
-    GU_HIP(hipMemcpyAsync(g->rowtab, t, sizeof(void *) * 1024, hipMemcpyHostToDevice, st));
+    GU_HIP(hipMemcpyAsync(g->rowtab, t, sizeof(void *) * kRowTab, hipMemcpyHostToDevice, st));
     if (b0 == 0) k_fill_i32<<<1, 64, 0, st>>>(p.ovf, 1, 0);
-    PsBcArgs a{g->R, g->C, g->n, p.srcs, (int *const *)g->rowtab, (u64 *const *)(g->rowtab + 256),
-               (float *const *)(g->rowtab + 512), p.queues, p.depth, p.part, p.ovf};
+    PsBcArgs a{rl ? g->LR : g->R, rl ? g->LC : g->C, g->n, p.srcs, (int *const *)g->rowtab,
+               (u64 *const *)(g->rowtab + 256), (float *const *)(g->rowtab + 512), p.queues, p.depth,
+               p.part, p.ovf};
     k_ps_bc<<<nb, 1024, 0, st>>>(a);
     GU_HIP(hipGetLastError());
-    if (hr.bc) {
-      k_ps_bc_reduce<<<pick_grid(g, 8), 256, 0, st>>>((float *const *)(g->rowtab + 512),
-                                                       (double *const *)(g->rowtab + 768), nb, g->n);
-      GU_HIP(hipGetLastError());
+    if (rl && hr.delta) {
+      const dim3 ug((unsigned)std::max(1, std::min(pick_grid(g, 1), (g->n + 255) / 256)), (unsigned)nb);
+      k_ps_unpermute<float><<<ug, 256, 0, st>>>((float *const *)(g->rowtab + 512),
+                                                (float *const *)(g->rowtab + 1024), g->nof, g->n);
     }
+    if (hr.bc)
+      k_ps_bc_reduce<<<pick_grid(g, 8), 256, 0, st>>>((float *const *)(g->rowtab + 512),
+                                                       (double *const *)(g->rowtab + 768), nb, g->n,
+                                                       rl ? g->nof : nullptr);
+    GU_HIP(hipGetLastError());
     GU_HIP(hipMemcpyAsync(part, p.part, sizeof(u64) * 3 * nb, hipMemcpyDeviceToHost, st));
     GU_HIP(hipMemcpyAsync(depth, p.depth, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
     GU_HIP(hipMemcpyAsync(g->misc_h, p.ovf, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1452,7 +1494,7 @@ int run_job(gu_graph *g, const int32_t *sources, int32_t nsrc, Job job, const Ho
       t[512 + i] = hr.delta ? (void *)hr.delta[b0 + i] : nullptr;
       t[768 + i] = hr.bc ? (void *)hr.bc[b0 + i] : nullptr;
     }
-    GU_HIP(hipMemcpyAsync(g->rowtab, t, sizeof(void *) * 1024, hipMemcpyHostToDevice, st));
+    GU_HIP(hipMemcpyAsync(g->rowtab, t, sizeof(void *) * kRowTab, hipMemcpyHostToDevice, st));
     BatchOut o;
     if (hr.dist) o.dist = (int *const *)g->rowtab;
     if (hr.sigma) o.sigma = (u64 *const *)(g->rowtab + 256);
@@ -1629,18 +1671,71 @@ int gu_synchronize(void) {
 }  // extern "C"
 
 namespace {
+// Locality order for the per-source engine.  On a bounded-degree, high-
+// diameter graph each source's frontier is a thin ring; with the input
+// numbering a visited vertex touches separate cache lines for itself and
+// for each neighbour, and the ring returns to them only after they were
+// evicted.  Grouping vertices into clusters of 32 grown by BFS (from the
+// lowest-numbered unassigned vertex, over unassigned vertices) and giving
+// each cluster consecutive ids puts most neighbours in the same 128-byte
+// line of every per-vertex array.  Adjacency order within a row is kept.
+void cluster_order(int n, const int32_t *R, const int32_t *C, std::vector<int32_t> &nof,
+                   std::vector<int32_t> &oon) {
+  nof.assign((size_t)n, -1);
+  oon.assign((size_t)n, 0);
+  std::vector<int32_t> q;
+  int next = 0;
+  for (int s = 0; s < n; s++) {
+    if (nof[(size_t)s] >= 0) continue;
+    q.clear();
+    q.push_back(s);
+    int got = 0;
+    for (size_t h = 0; h < q.size() && got < 32; h++) {
+      const int v = q[h];
+      if (nof[(size_t)v] >= 0) continue;
+      nof[(size_t)v] = next;
+      oon[(size_t)next] = v;
+      next++;
+      got++;
+      for (int k = R[v]; k < R[v + 1]; k++)
+        if (nof[(size_t)C[k]] < 0) q.push_back(C[k]);
+    }
+  }
+}
+
+int build_cluster_copy(gu_graph *g, const int32_t *R, const int32_t *C) {
+  const int n = g->n;
+  std::vector<int32_t> oon, LR((size_t)n + 1), LC((size_t)std::max<long long>(g->m, 1));
+  cluster_order(n, R, C, g->nof_h, oon);
+  LR[0] = 0;
+  for (int u = 0; u < n; u++) {
+    const int v = oon[(size_t)u];
+    int o = LR[(size_t)u];
+    for (int k = R[v]; k < R[v + 1]; k++) LC[(size_t)o++] = g->nof_h[(size_t)C[k]];
+    LR[(size_t)u + 1] = o;
+  }
+  if (dalloc(&g->LR, (size_t)n + 1) != hipSuccess || dalloc(&g->LC, LC.size()) != hipSuccess ||
+      dalloc(&g->nof, (size_t)std::max(n, 1)) != hipSuccess)
+    return GU_ERR_NOMEM;
+  if (hipMemcpy(g->LR, LR.data(), sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(g->LC, LC.data(), sizeof(int) * LC.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      (n > 0 && hipMemcpy(g->nof, g->nof_h.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice) != hipSuccess))
+    return GU_ERR_HIP;
+  return GU_OK;
+}
+
 // Pinned staging and the launch tables every handle owns.
 int alloc_handle_bufs(gu_graph *g) {
   if (hipHostMalloc((void **)&g->ctrl_h, sizeof(Ctrl), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void **)&g->srcs_h, sizeof(int) * 256, hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void **)&g->rowtab_h, sizeof(void *) * 1024, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void **)&g->rowtab_h, sizeof(void *) * kRowTab, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void **)&g->ecc_h, sizeof(int) * 256, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void **)&g->misc_h, sizeof(int) * 320, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void **)&g->part_h, sizeof(u64) * 768, hipHostMallocDefault) != hipSuccess)
     return GU_ERR_NOMEM;
   std::memset(g->ctrl_h, 0, sizeof(Ctrl));
-  std::memset(g->rowtab_h, 0, sizeof(void *) * 1024);
-  if (dalloc(&g->rowtab, 1024) != hipSuccess || dalloc(&g->ecc, 256) != hipSuccess)
+  std::memset(g->rowtab_h, 0, sizeof(void *) * kRowTab);
+  if (dalloc(&g->rowtab, kRowTab) != hipSuccess || dalloc(&g->ecc, 256) != hipSuccess)
     return GU_ERR_NOMEM;
   return GU_OK;
 }
@@ -1708,6 +1803,8 @@ int gu_graph_create(const int32_t *R, const int32_t *C, const int32_t *F, int32_
       g->ou = g->pu;
       g->ou.owned = false;
     }
+    // the per-source engine's locality copy, on graphs it is chosen for
+    if (n > 0 && maxd <= 64 && g->avg_deg <= 8.0 && (rc = build_cluster_copy(g, R, C)) != GU_OK) break;
     if (F) {
       if (m > 0 && hipMemcpy(g->F, F, sizeof(int) * (size_t)m, hipMemcpyHostToDevice) != hipSuccess) {
         rc = GU_ERR_HIP;
@@ -1739,6 +1836,9 @@ int gu_graph_destroy(gu_graph_t g) {
     g->ou.release();
     if (g->RT) (void)hipFree(g->RT);
     if (g->CT) (void)hipFree(g->CT);
+    if (g->LR) (void)hipFree(g->LR);
+    if (g->LC) (void)hipFree(g->LC);
+    if (g->nof) (void)hipFree(g->nof);
   } else {
     g->root->views.fetch_sub(1);
   }
@@ -1770,6 +1870,9 @@ int gu_graph_share(gu_graph_t g, gu_graph_t *out) {
   v->avg_deg = root->avg_deg;
   v->RT = root->RT;
   v->CT = root->CT;
+  v->LR = root->LR;
+  v->LC = root->LC;
+  v->nof = root->nof;
   v->pu = root->pu;
   v->pu.owned = false;
   v->ou = root->ou;
@@ -1806,6 +1909,7 @@ int gu_graph_arrays(gu_graph_t g, const int32_t **R, const int32_t **C, const in
 int gu_graph_device_bytes(gu_graph_t g, int64_t *bytes) {
   if (!g || !bytes) return GU_ERR_INVALID;
   *bytes = (int64_t)(((size_t)g->n + 1) * 4 + (size_t)g->m * 8 + g->ws.bytes + g->pws.bytes);
+  if (g->LR) *bytes += (int64_t)(((size_t)g->n + 1) * 4 + (size_t)g->m * 4 + (size_t)g->n * 4);
   return GU_OK;
 }
 
