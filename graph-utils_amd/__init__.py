@@ -61,7 +61,7 @@ EXPORTS = (
     "gu_multi_search_rows", "gu_betweenness_rows", "gu_server_create", "gu_server_add_graph",
     "gu_server_submit", "gu_server_wait", "gu_server_next_done", "gu_server_poll",
     "gu_server_release", "gu_server_now_ms", "gu_server_get_stats", "gu_server_destroy",
-    "gu_autotune",
+    "gu_autotune", "gu_parse_market", "gu_parse_snap", "gu_write_edgelist",
 )
 
 QUERY_BFS, QUERY_BC, QUERY_DIAMETER = 0, 1, 2
@@ -215,6 +215,11 @@ def lib():
                                   ctypes.POINTER(Config), ctypes.POINTER(Stats)]),
         "gu_parse_metis": (ctypes.c_int, [ctypes.c_char_p, _I32PP, _I32PP, _I32PP, _I32P, _I64P]),
         "gu_write_metis": (ctypes.c_int, [ctypes.c_char_p, _P, _P, ctypes.c_int32, ctypes.c_int64]),
+        "gu_parse_market": (ctypes.c_int, [ctypes.c_char_p, _I32PP, _I32PP, _I32PP, _I32P, _I64P]),
+        "gu_parse_snap": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, _I32PP, _I32PP, _I32PP,
+                                         _I32P, _I64P]),
+        "gu_write_edgelist": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, _P, _P, ctypes.c_int32,
+                                             ctypes.c_int64]),
         "gu_gen_er": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64,
                                      _I32PP, _I32PP, _I32PP, _I32P, _I64P]),
         "gu_gen_rmat": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
@@ -406,6 +411,33 @@ class HostGraph:
                                     ctypes.byref(F), ctypes.byref(n), ctypes.byref(m)),
                "parse_metis(%s)" % path)
         return cls._from_out(R, C, F, n, m)
+
+    @classmethod
+    def parse_market(cls, path):
+        """parse_market (parse.cpp:210-332): symmetrised pattern Matrix Market."""
+        R, C, F = _I32P(), _I32P(), _I32P()
+        n, m = ctypes.c_int32(), ctypes.c_int64()
+        _check(lib().gu_parse_market(os.fsencode(path), ctypes.byref(R), ctypes.byref(C),
+                                     ctypes.byref(F), ctypes.byref(n), ctypes.byref(m)),
+               "parse_market(%s)" % path)
+        return cls._from_out(R, C, F, n, m)
+
+    @classmethod
+    def parse_snap(cls, path, header=False):
+        """parse_snap (parse.cpp:335-434): directed edge list."""
+        R, C, F = _I32P(), _I32P(), _I32P()
+        n, m = ctypes.c_int32(), ctypes.c_int64()
+        _check(lib().gu_parse_snap(os.fsencode(path), 1 if header else 0, ctypes.byref(R),
+                                   ctypes.byref(C), ctypes.byref(F), ctypes.byref(n), ctypes.byref(m)),
+               "parse_snap(%s)" % path)
+        g = cls._from_out(R, C, F, n, m)
+        g.directed = True
+        return g
+
+    def write_edgelist(self, path, header=False):
+        """host_graph::write_edgelist_to_file (parse.cpp:436-456)."""
+        _check(lib().gu_write_edgelist(os.fsencode(path), 1 if header else 0, _ptr(self.F), _ptr(self.C),
+                                       self.n, self.m), "write_edgelist")
 
     @classmethod
     def er(cls, n, m_undirected, seed=0):

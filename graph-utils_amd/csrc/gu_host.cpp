@@ -169,11 +169,152 @@ bool to_int(const std::string &s, long long *out) {
   return true;
 }
 
+// The reference readers' line splitter (parse.cpp:138-157 and its copies in
+// parse_market / parse_snap): a blank or tab ends a non-empty token; one
+// arriving while the token is empty is kept in it.
+void split_line(const std::string &line, std::vector<std::string> &tok) {
+  tok.clear();
+  std::string t;
+  for (char ch : line) {
+    if ((ch == '\t' || ch == ' ') && !t.empty()) {
+      tok.push_back(t);
+      t.clear();
+    } else {
+      t.push_back(ch);
+    }
+  }
+  if (!t.empty()) tok.push_back(t);
+}
+
+// R from the row of every entry, as the reference readers induce it
+// (parse.cpp:319-331, 416-429): R[v] is the index of the first entry whose
+// row exceeds v-1 while rows ascend; rows that never appear take the
+// position reached.  Rows must lie in [0, n).
+void induce_R(int64_t n, const std::vector<int32_t> &Fv, int64_t m, std::vector<int32_t> &Rv) {
+  Rv.assign((size_t)n + 1, 0);
+  int64_t last = 0;
+  for (int64_t i = 0; i < m; i++)
+    while (Fv[(size_t)i] > last && last < n + 1) Rv[(size_t)++last] = (int32_t)i;
+  while (last < n) Rv[(size_t)++last] = (int32_t)m;
+}
+
 }  // namespace
 
 extern "C" {
 
 void gu_host_free(void *p) { std::free(p); }
+
+// parse_market (parse.cpp:210-332): pattern Matrix Market; '%' lines
+// skipped; the first other line holds rows, columns and nonzeros (m = 2 x
+// nonzeros); every further line "u v" (1-based) adds (u,v) and (v,u); the
+// entries are sorted by (row, column) and R induced from the rows.  Where
+// the reference prints and exits (wrong field counts) or reads past its
+// arrays (fewer entries than the header, ids outside [1,n]) this reader
+// returns an error; more entries than the header keep the first m after
+// sorting, as the reference's R does.
+int gu_parse_market(const char *path, int32_t **R, int32_t **C, int32_t **F, int32_t *n_out,
+                    int64_t *m_out) {
+  if (!path || !R || !C || !n_out || !m_out) return GU_ERR_INVALID;
+  std::ifstream in(path, std::ifstream::in);
+  if (!in.good()) return GU_ERR_IO;
+  std::string line;
+  std::vector<std::string> tok;
+  bool first = true;
+  long long n = 0, m = 0;
+  std::vector<std::pair<int32_t, int32_t>> e;
+  while (std::getline(in, line)) {
+    if (!line.empty() && line[0] == '%') continue;
+    split_line(line, tok);
+    if (first) {
+      long long a, b, c;
+      if (tok.size() != 3) return GU_ERR_PARSE;
+      if (!to_int(tok[0], &a) || !to_int(tok[1], &b) || !to_int(tok[2], &c)) return GU_ERR_PARSE;
+      if (a < 0 || c < 0 || 2 * c >= INT32_MAX) return GU_ERR_PARSE;
+      n = a;
+      m = 2 * c;  // a non-square header is only warned about by the reference
+      e.reserve((size_t)m);
+      first = false;
+      continue;
+    }
+    if (tok.size() != 2) return GU_ERR_PARSE;  // pattern matrices only
+    long long u, v;
+    if (!to_int(tok[0], &u) || !to_int(tok[1], &v)) return GU_ERR_PARSE;
+    if (u < 1 || u > n || v < 1 || v > n) return GU_ERR_RANGE;
+    e.emplace_back((int32_t)(u - 1), (int32_t)(v - 1));
+    e.emplace_back((int32_t)(v - 1), (int32_t)(u - 1));
+  }
+  if (first) return GU_ERR_PARSE;
+  if ((long long)e.size() < m) return GU_ERR_PARSE;
+  std::sort(e.begin(), e.end());
+  std::vector<int32_t> Fv((size_t)m), Cv((size_t)m), Rv;
+  for (long long i = 0; i < m; i++) {
+    Fv[(size_t)i] = e[(size_t)i].first;
+    Cv[(size_t)i] = e[(size_t)i].second;
+  }
+  induce_R(n, Fv, m, Rv);
+  *R = dup_out(Rv);
+  *C = dup_out(Cv);
+  if (F) *F = dup_out(Fv);
+  *n_out = (int32_t)n;
+  *m_out = m;
+  return (*R && *C) ? GU_OK : GU_ERR_NOMEM;
+}
+
+// parse_snap (parse.cpp:335-434): '#' lines skipped; each other line "u v"
+// (0-based, further fields ignored) is one directed entry in file order;
+// n = number of distinct ids, m = number of lines, R induced from the rows
+// in file order (a file not sorted by source yields the reference's R for
+// it).  The reference ignores its header flag, so a header line of an
+// "edgelist_h" file is read as an entry; `header` is accepted and ignored
+// the same way.  Where the reference reads out of bounds (fewer than two
+// fields, ids outside [0,n)) this reader returns an error.  The graph is
+// directed.
+int gu_parse_snap(const char *path, int32_t header, int32_t **R, int32_t **C, int32_t **F,
+                  int32_t *n_out, int64_t *m_out) {
+  (void)header;
+  if (!path || !R || !C || !n_out || !m_out) return GU_ERR_INVALID;
+  std::ifstream in(path, std::ifstream::in);
+  if (!in.good()) return GU_ERR_IO;
+  std::string line;
+  std::vector<std::string> tok;
+  std::vector<int32_t> Fv, Cv, Rv;
+  std::unordered_set<int32_t> ids;
+  while (std::getline(in, line)) {
+    if (!line.empty() && line[0] == '#') continue;
+    split_line(line, tok);
+    if (tok.size() < 2) return GU_ERR_PARSE;
+    long long u, v;
+    if (!to_int(tok[0], &u) || !to_int(tok[1], &v)) return GU_ERR_PARSE;
+    if (u < 0 || v < 0) return GU_ERR_RANGE;
+    if (Fv.size() >= (size_t)INT32_MAX - 1) return GU_ERR_CAPACITY;
+    Fv.push_back((int32_t)u);
+    Cv.push_back((int32_t)v);
+    ids.insert((int32_t)u);
+    ids.insert((int32_t)v);
+  }
+  const int64_t n = (int64_t)ids.size(), m = (int64_t)Fv.size();
+  for (int64_t i = 0; i < m; i++)
+    if (Fv[(size_t)i] >= n || Cv[(size_t)i] >= n) return GU_ERR_RANGE;  // ids not 0..n-1
+  induce_R(n, Fv, m, Rv);
+  *R = dup_out(Rv);
+  *C = dup_out(Cv);
+  if (F) *F = dup_out(Fv);
+  *n_out = (int32_t)n;
+  *m_out = m;
+  return (*R && *C) ? GU_OK : GU_ERR_NOMEM;
+}
+
+// host_graph::write_edgelist_to_file (parse.cpp:436-456): optional "n m"
+// line, then "F[k] C[k]" per entry.
+int gu_write_edgelist(const char *path, int32_t header, const int32_t *F, const int32_t *C, int32_t n,
+                      int64_t m) {
+  if (!path || n < 0 || m < 0 || (m > 0 && (!F || !C))) return GU_ERR_INVALID;
+  FILE *f = std::fopen(path, "w");
+  if (!f) return GU_ERR_IO;
+  if (header) std::fprintf(f, "%d %lld\n", n, (long long)m);
+  for (int64_t k = 0; k < m; k++) std::fprintf(f, "%d %d\n", F[k], C[k]);
+  return std::fclose(f) == 0 ? GU_OK : GU_ERR_IO;
+}
 
 int gu_parse_metis(const char *path, int32_t **R, int32_t **C, int32_t **F,
                    int32_t *n_out, int64_t *m_out) {
@@ -187,17 +328,7 @@ int gu_parse_metis(const char *path, int32_t **R, int32_t **C, int32_t **F,
   std::vector<std::string> tok;
   while (std::getline(in, line)) {
     if (!line.empty() && line[0] == '%') continue;
-    tok.clear();
-    std::string t;
-    for (char ch : line) {
-      if ((ch == '\t' || ch == ' ') && !t.empty()) {
-        tok.push_back(t);
-        t.clear();
-      } else {
-        t.push_back(ch);
-      }
-    }
-    if (!t.empty()) tok.push_back(t);
+    split_line(line, tok);
     if (first) {
       long long a, b, c;
       if (tok.size() < 2 || !to_int(tok[0], &a) || !to_int(tok[1], &b)) return GU_ERR_PARSE;

@@ -141,27 +141,55 @@ unsigned long long host_graph::count_degree_zero_vertices() const {
   return z;
 }
 
-host_graph parse_metis(char *file) {
-  int32_t *R = nullptr, *C = nullptr, *F = nullptr, n = 0;
-  int64_t m = 0;
-  check(gu_parse_metis(file, &R, &C, &F, &n, &m), "parse_metis");
+namespace {
+host_graph take_graph(int32_t *R, int32_t *C, int32_t *F, int32_t n, int64_t m, bool directed) {
   host_graph g;
   g.n = n;
   g.m = (int)m;
   g.R.assign(R, R + n + 1);
   g.C.assign(C, C + m);
   g.F.assign(F, F + m);
-  g.directed = false;
+  g.directed = directed;
   gu_host_free(R);
   gu_host_free(C);
   gu_host_free(F);
   return g;
 }
+}  // namespace
 
+host_graph parse_metis(char *file) {
+  int32_t *R = nullptr, *C = nullptr, *F = nullptr, n = 0;
+  int64_t m = 0;
+  check(gu_parse_metis(file, &R, &C, &F, &n, &m), "parse_metis");
+  return take_graph(R, C, F, n, m, false);
+}
+
+host_graph parse_market(char *file) {
+  int32_t *R = nullptr, *C = nullptr, *F = nullptr, n = 0;
+  int64_t m = 0;
+  check(gu_parse_market(file, &R, &C, &F, &n, &m), "parse_market");
+  return take_graph(R, C, F, n, m, false);
+}
+
+host_graph parse_snap(char *file, bool header) {
+  int32_t *R = nullptr, *C = nullptr, *F = nullptr, n = 0;
+  int64_t m = 0;
+  check(gu_parse_snap(file, header ? 1 : 0, &R, &C, &F, &n, &m), "parse_snap");
+  return take_graph(R, C, F, n, m, true);
+}
+
+bool host_graph::write_edgelist_to_file(const std::string &file, bool header) const {
+  return gu_write_edgelist(file.c_str(), header ? 1 : 0, F.data(), C.data(), n, m) == GU_OK;
+}
+
+// parse (parse.cpp:88-113): dimacs (METIS), edgelist, edgelist_h, market.
 host_graph parse(const program_options &op) {
-  if (!op.format || std::string(op.format) != "dimacs")
-    throw gu_error(GU_ERR_INVALID, "unsupported graph format (only dimacs/METIS is provided)");
-  return parse_metis(op.infile);
+  const std::string f = op.format ? op.format : "";
+  if (f == "dimacs") return parse_metis(op.infile);
+  if (f == "edgelist") return parse_snap(op.infile, false);
+  if (f == "edgelist_h") return parse_snap(op.infile, true);
+  if (f == "market") return parse_market(op.infile);
+  throw gu_error(GU_ERR_INVALID, "unsupported graph file type");
 }
 
 device_graph::device_graph(const host_graph &g) {

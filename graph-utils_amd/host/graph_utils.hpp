@@ -3,7 +3,9 @@
 // compile unchanged, implemented over the C ABI (include/graph_utils_amd.h).
 //
 //   program_options, parse_arguments      util.h:10-25, util.cpp:3-77
-//   host_graph, parse, parse_metis        parse.h:15-40, parse.cpp:88-208
+//   host_graph, parse, parse_metis,       parse.h:15-40, parse.cpp:88-456
+//   parse_snap, parse_market,
+//   write_edgelist_to_file
 //   device_graph                          device_graph.h:7-22
 //   start_clock, end_clock, choose_device util_device.cuh:38-41
 //   configure_grid, transfer_result       common.cuh:9-14
@@ -65,6 +67,7 @@ class host_graph {
   void print_from_array() const;
   void print_adjacency_list() const;
   unsigned long long count_degree_zero_vertices() const;
+  bool write_edgelist_to_file(const std::string &file, bool header) const;  // true on success
 
   std::vector<int> C;  // column ids
   std::vector<int> R;  // CSR offsets
@@ -74,8 +77,10 @@ class host_graph {
   bool directed = false;
 };
 
-host_graph parse(const program_options &op);  // "dimacs" (METIS) only
+host_graph parse(const program_options &op);  // dimacs, edgelist, edgelist_h, market
 host_graph parse_metis(char *file);
+host_graph parse_snap(char *file, bool header);
+host_graph parse_market(char *file);
 
 class device_graph {
  public:

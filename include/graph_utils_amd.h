@@ -326,6 +326,18 @@ int gu_server_destroy(gu_server_t s);
  * the graph is undirected and m = 2 x header edge count. */
 int gu_parse_metis(const char *path, int32_t **R, int32_t **C, int32_t **F,
                    int32_t *n, int64_t *m);
+/* parse_market (parse.cpp:210-332): pattern Matrix Market, symmetrised
+ * (every "u v" line adds both directions), entries sorted by (row, col). */
+int gu_parse_market(const char *path, int32_t **R, int32_t **C, int32_t **F,
+                    int32_t *n, int64_t *m);
+/* parse_snap (parse.cpp:335-434): "u v" lines (0-based), '#' comments, a
+ * directed graph with n = number of distinct ids; `header` is ignored as in
+ * the reference (a header line is read as an entry). */
+int gu_parse_snap(const char *path, int32_t header, int32_t **R, int32_t **C,
+                  int32_t **F, int32_t *n, int64_t *m);
+/* host_graph::write_edgelist_to_file (parse.cpp:436-456). */
+int gu_write_edgelist(const char *path, int32_t header, const int32_t *F,
+                      const int32_t *C, int32_t n, int64_t m);
 /* Writes a METIS file with single spaces and '\n' (1-based ids). */
 int gu_write_metis(const char *path, const int32_t *R, const int32_t *C,
                    int32_t n, int64_t m);

@@ -182,3 +182,50 @@ def ref_parse_metis(path):
         return a
 
     return take(R, n.value + 1), take(C, m.value), take(F, m.value), n.value, m.value, bool(dr.value)
+
+
+def _ref_lib():
+    L = ctypes.CDLL(REF_PARSE)
+    I32P = ctypes.POINTER(ctypes.c_int32)
+    L.ref_parse_market.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32),
+                                   ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(I32P),
+                                   ctypes.POINTER(I32P), ctypes.POINTER(I32P),
+                                   ctypes.POINTER(ctypes.c_int64)]
+    L.ref_parse_snap.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),
+                                 ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(I32P),
+                                 ctypes.POINTER(I32P), ctypes.POINTER(I32P),
+                                 ctypes.POINTER(ctypes.c_int32)]
+    L.ref_free.argtypes = [ctypes.c_void_p]
+    return L, I32P
+
+
+def _take_ref(L, p, k):
+    a = np.ctypeslib.as_array(p, shape=(k,)).copy() if k > 0 else np.zeros(0, np.int32)
+    L.ref_free(ctypes.cast(p, ctypes.c_void_p))
+    return a
+
+
+def ref_parse_market(path):
+    """The reference's parse_market (parse.cpp:210-332).  Returns
+    (R, C, F, n, m) with C and F cut to m entries."""
+    L, I32P = _ref_lib()
+    n, m, cl = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int64()
+    R, C, F = I32P(), I32P(), I32P()
+    L.ref_parse_market(os.fsencode(path), ctypes.byref(n), ctypes.byref(m), ctypes.byref(R),
+                       ctypes.byref(C), ctypes.byref(F), ctypes.byref(cl))
+    Ra = _take_ref(L, R, n.value + 1)
+    Ca = _take_ref(L, C, cl.value)[:m.value]
+    Fa = _take_ref(L, F, cl.value)[:m.value]
+    return Ra, Ca, Fa, n.value, m.value
+
+
+def ref_parse_snap(path, header=False):
+    """The reference's parse_snap (parse.cpp:335-434).  Returns
+    (R, C, F, n, m, directed)."""
+    L, I32P = _ref_lib()
+    n, m, dr = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int32()
+    R, C, F = I32P(), I32P(), I32P()
+    L.ref_parse_snap(os.fsencode(path), 1 if header else 0, ctypes.byref(n), ctypes.byref(m),
+                     ctypes.byref(R), ctypes.byref(C), ctypes.byref(F), ctypes.byref(dr))
+    return (_take_ref(L, R, n.value + 1), _take_ref(L, C, m.value), _take_ref(L, F, m.value),
+            n.value, m.value, bool(dr.value))

@@ -1,9 +1,10 @@
-// ref_parse_shim.cpp -- C entry points over the reference's own METIS reader.
+// ref_parse_shim.cpp -- C entry points over the reference's own graph readers.
 //
 // TEST INFRASTRUCTURE ONLY.  oracle/Makefile compiles this file together with
 // the reference's unmodified parse.cpp and util.cpp (read in place from the
 // reference checkout) into oracle/_ref/libref_parse.so, so tests can compare
-// the product's METIS reader with parse_metis (parse.cpp:115-208) itself.
+// the product's readers with parse_metis, parse_market and parse_snap
+// (parse.cpp:115-434) themselves.
 // Nothing here restates the reader; it only copies host_graph's vectors out.
 #include <cstdint>
 #include <cstdlib>
@@ -31,6 +32,38 @@ int ref_parse_metis(const char *path, int32_t *n, int64_t *m, int32_t **R,
   if (!g.R.empty()) std::memcpy(*R, g.R.data(), sizeof(int32_t) * g.R.size());
   if (!g.C.empty()) std::memcpy(*C, g.C.data(), sizeof(int32_t) * g.C.size());
   if (!g.F.empty()) std::memcpy(*F, g.F.data(), sizeof(int32_t) * g.F.size());
+  return 0;
+}
+
+static void copy_out(const host_graph &g, int32_t *n, int64_t *m, int32_t **R, int32_t **C, int32_t **F) {
+  *n = g.n;
+  *m = g.m;
+  *R = (int32_t *)std::malloc(sizeof(int32_t) * (g.R.size() ? g.R.size() : 1));
+  *C = (int32_t *)std::malloc(sizeof(int32_t) * (g.C.size() ? g.C.size() : 1));
+  *F = (int32_t *)std::malloc(sizeof(int32_t) * (g.F.size() ? g.F.size() : 1));
+  if (!g.R.empty()) std::memcpy(*R, g.R.data(), sizeof(int32_t) * g.R.size());
+  if (!g.C.empty()) std::memcpy(*C, g.C.data(), sizeof(int32_t) * g.C.size());
+  if (!g.F.empty()) std::memcpy(*F, g.F.data(), sizeof(int32_t) * g.F.size());
+}
+
+// The reference's parse_market / parse_snap (parse.cpp:210-434).  Arrays
+// are returned with the lengths the reference's vectors have (C and F may
+// be longer than m when the file holds more entries than its header).
+int ref_parse_market(const char *path, int32_t *n, int64_t *m, int32_t **R, int32_t **C, int32_t **F,
+                     int64_t *c_len) {
+  std::vector<char> p(path, path + std::strlen(path) + 1);
+  host_graph g = parse_market(p.data());
+  copy_out(g, n, m, R, C, F);
+  *c_len = (int64_t)g.C.size();
+  return 0;
+}
+
+int ref_parse_snap(const char *path, int32_t header, int32_t *n, int64_t *m, int32_t **R, int32_t **C,
+                   int32_t **F, int32_t *directed) {
+  std::vector<char> p(path, path + std::strlen(path) + 1);
+  host_graph g = parse_snap(p.data(), header != 0);
+  copy_out(g, n, m, R, C, F);
+  *directed = g.directed ? 1 : 0;
   return 0;
 }
 

@@ -271,3 +271,132 @@ def test_sample_sources(gu):
                                          (5, 5, 256)])
 def test_row_sources_match_oracle(gu, orc, start, end, G):
     assert np.array_equal(gu.row_sources(start, end, G), orc.rows_for_range(start, end, G))
+
+
+# --------------------------------------------------------------------------
+# Matrix Market and SNAP readers, edge-list writer (parse.cpp:210-456)
+# --------------------------------------------------------------------------
+def _mm_text(hg, banner=True, tabs=False):
+    """Lower-triangle pattern Matrix Market text of an undirected graph."""
+    rows = [(int(v) + 1, int(u) + 1) for u in range(hg.n) for v in hg.C[hg.R[u]:hg.R[u + 1]] if v > u]
+    sep = "\t" if tabs else " "
+    head = "%%MatrixMarket matrix coordinate pattern symmetric\n% comment\n" if banner else ""
+    return head + "%d%s%d%s%d\n" % (hg.n, sep, hg.n, sep, len(rows)) + "".join(
+        "%d%s%d\n" % (a, sep, b) for a, b in rows)
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_market_matches_reference_reader(gu, orc, name, tmp_path):
+    _ref_or_skip(orc)
+    hg = gu.HostGraph.parse_metis(graph_path(name))
+    p = tmp_path / (name + ".mtx")
+    p.write_text(_mm_text(hg, tabs=(name == "test3")))
+    g = gu.HostGraph.parse_market(str(p))
+    R, C, F, n, m = orc.ref_parse_market(str(p))
+    assert (g.n, g.m) == (n, m)
+    assert np.array_equal(g.R, R) and np.array_equal(g.C, C) and np.array_equal(g.F, F)
+    # symmetrised and sorted: the same CSR as the METIS file's (sorted rows)
+    assert np.array_equal(g.R, hg.R)
+    assert np.array_equal(g.C, np.concatenate([np.sort(hg.C[hg.R[u]:hg.R[u + 1]]) for u in range(hg.n)]))
+
+
+def test_market_quirks_match_reference(gu, orc, tmp_path):
+    _ref_or_skip(orc)
+    cases = {
+        "selfloop_and_dups": "4 4 4\n1 1\n2 1\n2 1\n4 3\n",        # kept twice each
+        "non_square": "3 5 2\n1 2\n3 2\n",                         # the reference only warns
+        "isolated_tail": "6 6 2\n2 1\n3 2\n",                      # rows 3..5 empty
+        "more_than_header": "3 3 1\n2 1\n3 2\n",                   # first m entries after sorting
+    }
+    for name, text in cases.items():
+        p = tmp_path / (name + ".mtx")
+        p.write_text(text)
+        g = gu.HostGraph.parse_market(str(p))
+        R, C, F, n, m = orc.ref_parse_market(str(p))
+        assert (g.n, g.m) == (n, m), name
+        assert np.array_equal(g.R, R) and np.array_equal(g.C, C) and np.array_equal(g.F, F), name
+
+
+@pytest.mark.parametrize("text,code", [
+    ("3 3\n1 2\n", -5),              # header needs three fields (parse.cpp exits -2)
+    ("3 3 1\n1 2 1\n", -5),          # values: only pattern matrices
+    ("3 3 2\n1 2\n", -5),            # fewer entries than the header
+    ("3 3 1\n1 4\n", -4),            # id beyond n
+])
+def test_market_errors(gu, tmp_path, text, code):
+    p = tmp_path / "bad.mtx"
+    p.write_text(text)
+    with pytest.raises(gu.GuError) as e:
+        gu.HostGraph.parse_market(str(p))
+    assert e.value.status == code
+
+
+SNAP_FILES = {
+    "sorted_tabs": "# c\n0\t2\n0\t3\n1\t3\n2\t0\n3\t1\n",
+    "extra_fields": "0 1 7\n1 2 8\n2 0 9\n",
+    "unsorted": "2 0\n0 1\n1 2\n0 2\n",       # the reference's R for an unsorted file
+    "selfloop_crlf": "0 0\r\n0 1\r\n1 0\r\n",
+    "isolated_target": "0 1\n0 2\n",
+}
+
+
+@pytest.mark.parametrize("name", sorted(SNAP_FILES))
+@pytest.mark.parametrize("header", [False, True])
+def test_snap_matches_reference_reader(gu, orc, name, header, tmp_path):
+    _ref_or_skip(orc)
+    p = tmp_path / (name + ".txt")
+    p.write_bytes(SNAP_FILES[name].encode())
+    g = gu.HostGraph.parse_snap(str(p), header)
+    R, C, F, n, m, directed = orc.ref_parse_snap(str(p), header)
+    assert directed and g.directed
+    assert (g.n, g.m) == (n, m)
+    assert np.array_equal(g.R, R) and np.array_equal(g.C, C) and np.array_equal(g.F, F)
+
+
+def test_snap_reference_fixture(gu, orc):
+    """graphs/test7.txt of the reference: 11 vertices, 13 directed entries."""
+    path = os.path.join(ROOT, "tests", "golden", "graphs", "test7.txt")
+    g = gu.HostGraph.parse_snap(path)
+    assert (g.n, g.m) == (11, 13)
+    assert g.R.tolist() == [0, 3, 5, 8, 9, 10, 11, 11, 13, 13, 13, 13]
+    assert g.C[:3].tolist() == [2, 3, 4]
+    if orc.ref_parse_available():
+        R, C, F, n, m, _ = orc.ref_parse_snap(path)
+        assert np.array_equal(g.R, R) and np.array_equal(g.C, C)
+
+
+@pytest.mark.parametrize("text,code", [
+    ("0\n", -5),                 # one field
+    ("0 1\n5 0\n", -4),          # id 5 with only 3 distinct ids
+    ("a b\n", -5),
+    ("0 -1\n", -4),
+])
+def test_snap_errors(gu, tmp_path, text, code):
+    p = tmp_path / "bad.txt"
+    p.write_text(text)
+    with pytest.raises(gu.GuError) as e:
+        gu.HostGraph.parse_snap(str(p))
+    assert e.value.status == code
+
+
+@pytest.mark.parametrize("header", [False, True])
+def test_edgelist_round_trip(gu, orc, tmp_path, header):
+    hg = gu.HostGraph.parse_metis(graph_path("test3"))
+    p = tmp_path / "el.txt"
+    hg.write_edgelist(str(p), header)
+    lines = p.read_text().splitlines()
+    assert len(lines) == hg.m + (1 if header else 0)
+    if header:
+        assert lines[0] == "%d %d" % (hg.n, hg.m)
+        # the reference's reader ignores its header flag: "n m" becomes an
+        # entry whose ids lie outside the graph (the reference then indexes
+        # past R); this reader refuses it
+        with pytest.raises(gu.GuError) as e:
+            gu.HostGraph.parse_snap(str(p), True)
+        assert e.value.status == -4
+        return
+    g = gu.HostGraph.parse_snap(str(p))   # rows ascend: the same CSR back
+    assert np.array_equal(g.R, hg.R) and np.array_equal(g.C, hg.C)
+    if orc.ref_parse_available():
+        R, C, F, n, m, _ = orc.ref_parse_snap(str(p))
+        assert np.array_equal(g.R, R) and np.array_equal(g.C, C)
