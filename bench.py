@@ -451,6 +451,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-variants", action="store_true", help="skip the per-variant sweep")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (degree-profile defaults)")
+    ap.add_argument("--level-graphs", type=int, default=0,
+                    help="batched engine: 0/1 replay captured level chunks, -1 launch kernels directly")
     ap.add_argument("--bc-check", type=int, default=0,
                     help="bc workload: compare BC over this many sources with the FP64 CPU oracle")
     ap.add_argument("--calib-bytes", type=int, default=0,
@@ -485,7 +487,7 @@ def main():
     batches = [hg.sample_sources(k, seed=1000 + i) for i in range(nb)]
     dist = gu.DeviceBuffer(4 * k * n)
     sig = gu.DeviceBuffer(8 * k * n)
-    cfg = gu.make_config(args.variant, alpha=args.alpha, beta=args.beta)
+    cfg = gu.make_config(args.variant, alpha=args.alpha, beta=args.beta, level_graphs=args.level_graphs)
     with gu.DeviceGraph(hg) as dg:
         tuned = None
         if not args.no_tune and args.variant == "auto":

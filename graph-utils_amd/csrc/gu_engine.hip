@@ -895,6 +895,15 @@ struct gu_graph {
   int *ecc_h = nullptr;       // pinned mirror
   int *misc_h = nullptr;      // pinned: [0] overflow flag, [64, 320) depths
   u64 *part_h = nullptr;      // pinned [3 * 256] per-source sums
+  // Captured chunks of levels of the batched engine, keyed by the launch
+  // parameters and the engine arguments (which hold every buffer address);
+  // captured on a private stream, replayed on the caller's.
+  struct LevelGraph {
+    std::vector<char> key;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<LevelGraph> level_graphs;
+  hipStream_t cap_stream = nullptr;
   // Handles made by gu_graph_share read the root's graph arrays and units
   // and own only their workspaces; the root outlives them.
   gu_graph *root = nullptr;
@@ -906,6 +915,12 @@ namespace {
 int pick_grid(const gu_graph *g, int per_cu) { return std::max(1, g->cus * per_cu); }
 
 constexpr int kRowTab = 2048;
+
+void clear_level_graphs(gu_graph *g) {
+  for (auto &x : g->level_graphs)
+    if (x.exec) (void)hipGraphExecDestroy(x.exec);
+  g->level_graphs.clear();
+}
 
 // Host copy of the cluster order (a shared handle reads its root's).
 const std::vector<int32_t> &nof_host(const gu_graph *g) { return g->root ? g->root->nof_h : g->nof_h; }
@@ -987,6 +1002,7 @@ int ensure_ws(gu_graph *g, int NW, bool stack, size_t want_S, hipStream_t st) {
   if (NW > w.NW || (stack && !w.stack)) {
     NW = std::max(NW, w.NW);
     stack = stack || w.stack;
+    clear_level_graphs(g);
     w.release();
     w.NW = NW;
     w.n = g->n;
@@ -1028,6 +1044,7 @@ int ensure_ws(gu_graph *g, int NW, bool stack, size_t want_S, hipStream_t st) {
     GU_HIP(hipStreamSynchronize(st));
     (void)hipFree(w.q);
     (void)hipFree(w.smask);
+    clear_level_graphs(g);
     w.bytes += (cap - w.S_cap) * (4 + w.NW * 8);
     w.q = q2;
     w.smask = m2;
@@ -1117,6 +1134,47 @@ void launch_level(gu_graph *g, const Eng<NW> &e, int variant, int block, int gri
   k_control<NW><<<1, 256, 0, st>>>(e);
 }
 
+// The executable graph of K consecutive levels for these launch parameters,
+// captured on first use.  Level kernels read the level, direction and
+// counters from the control block, so one capture serves every chunk of K
+// levels of every batch with the same arguments.
+template <int NW>
+int level_graph(gu_graph *g, const Eng<NW> &e, int variant, int block, int grid, int K, bool sigma,
+                hipGraphExec_t *out) {
+  std::vector<char> key(sizeof(Eng<NW>) + 6 * sizeof(int));
+  int hdr[6] = {NW, variant, block, grid, K, sigma ? 1 : 0};
+  std::memcpy(key.data(), hdr, sizeof(hdr));
+  std::memcpy(key.data() + sizeof(hdr), &e, sizeof(Eng<NW>));
+  for (auto &x : g->level_graphs)
+    if (x.key == key) {
+      *out = x.exec;
+      return GU_OK;
+    }
+  if (!g->cap_stream) GU_HIP(hipStreamCreateWithFlags(&g->cap_stream, hipStreamNonBlocking));
+  hipStream_t cs = g->cap_stream;
+  GU_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+  for (int i = 0; i < K; i++) {
+    if (sigma)
+      launch_level<NW, true>(g, e, variant, block, grid, cs);
+    else
+      launch_level<NW, false>(g, e, variant, block, grid, cs);
+  }
+  hipGraph_t gr = nullptr;
+  hipError_t err = hipStreamEndCapture(cs, &gr);
+  if (err != hipSuccess) return fail(err);
+  hipGraphExec_t ex = nullptr;
+  err = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(gr);
+  if (err != hipSuccess) return fail(err);
+  if (g->level_graphs.size() >= 24) {  // bounded cache: drop the oldest
+    (void)hipGraphExecDestroy(g->level_graphs.front().exec);
+    g->level_graphs.erase(g->level_graphs.begin());
+  }
+  g->level_graphs.push_back(gu_graph::LevelGraph{std::move(key), ex});
+  *out = ex;
+  return GU_OK;
+}
+
 int default_alpha(const gu_config &c) { return c.alpha > 0 ? c.alpha : 16; }
 int default_beta(const gu_config &c) { return c.beta > 0 ? c.beta : 64; }
 
@@ -1162,6 +1220,7 @@ int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const g
   // stack mode reserves room for every level of a chunk, so its chunks stay short
   const int Kmax = cfg.sync_every > 0 ? cfg.sync_every : (stack ? 8 : 64);
   int levels_launched = 0;
+  unsigned peak = 0;
   while (true) {
     if (stack) {
       // every level adds at most n stack entries
@@ -1170,11 +1229,18 @@ int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const g
       e = make_eng<NW>(g, nb, stack, sigma);
       set_grids(e);
     }
-    for (int i = 0; i < K; i++) {
-      if (sigma)
-        launch_level<NW, true>(g, e, variant, block, grid, st);
-      else
-        launch_level<NW, false>(g, e, variant, block, grid, st);
+    if (cfg.level_graphs >= 0) {
+      hipGraphExec_t ex = nullptr;
+      rc = level_graph<NW>(g, e, variant, block, grid, K, sigma, &ex);
+      if (rc) return rc;
+      GU_HIP(hipGraphLaunch(ex, st));
+    } else {
+      for (int i = 0; i < K; i++) {
+        if (sigma)
+          launch_level<NW, true>(g, e, variant, block, grid, st);
+        else
+          launch_level<NW, false>(g, e, variant, block, grid, st);
+      }
     }
     levels_launched += K;
     GU_HIP(hipGetLastError());
@@ -1182,7 +1248,11 @@ int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const g
     GU_HIP(hipStreamSynchronize(st));
     if (g->ctrl_h->done) break;
     if (levels_launched > g->n + 2) return GU_ERR_INVALID;  // cannot happen for a valid graph
-    K = std::min(K * 2, Kmax);
+    // Past the frontier's peak the search is in its tail: short chunks, so
+    // few levels are launched after the last non-empty one.
+    const unsigned cur = g->ctrl_h->qlen[g->ctrl_h->level % 3];
+    peak = std::max(peak, cur);
+    K = (cfg.sync_every <= 0 && (u64)cur * 8 < (u64)peak) ? 2 : std::min(K * 2, Kmax);
   }
   const int last = g->ctrl_h->last_level;
   if (job == JOB_BFS && (out.dist || out.sigma))
@@ -1826,6 +1896,8 @@ int gu_graph_create(const int32_t *R, const int32_t *C, const int32_t *F, int32_
 int gu_graph_destroy(gu_graph_t g) {
   if (!g) return GU_ERR_INVALID;
   if (g->views.load() != 0) return GU_ERR_INVALID;  // shared handles still use it
+  clear_level_graphs(g);
+  if (g->cap_stream) (void)hipStreamDestroy(g->cap_stream);
   g->ws.release();
   g->pws.release();
   if (!g->root) {

@@ -301,6 +301,12 @@ std::vector<std::vector<int>> multi_search_shuffle_based_setup(const device_grap
   return run_distances(g, start, end, GU_VARIANT_SHUFFLE, "shuffle-based neighbor gathering");
 }
 
+// transitive_closure_setup (transitive_closure.cu:4-37): the reachability
+// rows are the BFS distance rows (INT_MAX = not reachable).
+void transitive_closure_setup(const device_graph &g, int start, int end, std::vector<std::vector<int>> &d) {
+  d = run_distances(g, start, end, GU_VARIANT_AUTO, "shuffle-based transitive closure");
+}
+
 void count_shortest_paths_setup(const device_graph &g, int start, int end,
                                 std::vector<std::vector<unsigned long long>> &sigma_h) {
   sigma_h.clear();

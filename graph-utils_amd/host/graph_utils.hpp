@@ -14,6 +14,7 @@
 //                                         scan_based.cuh:15, CTA_warp.cuh:15,
 //                                         shuffle_based.cuh:16
 //   count_shortest_paths_setup            count_shortest_paths.cuh:13
+//   transitive_closure_setup              transitive_closure.cuh:13
 //   betweenness_centrality_setup          betweenness_centrality.cuh:14
 //   diameter_sampling_setup               diameter_sampling.cuh:13
 //
@@ -125,6 +126,8 @@ std::vector<std::vector<int>> multi_search_shuffle_based_setup(const device_grap
                                                                int start, int end);
 void count_shortest_paths_setup(const device_graph &g, int start, int end,
                                 std::vector<std::vector<unsigned long long>> &sigma_h);
+// transitive_closure.cuh:13 -- distance rows; INT_MAX marks unreachable.
+void transitive_closure_setup(const device_graph &g, int start, int end, std::vector<std::vector<int>> &d);
 void betweenness_centrality_setup(const device_graph &g, int start, int end,
                                   std::vector<std::vector<float>> &delta_h);
 int diameter_sampling_setup(const device_graph &g, int start, int end);

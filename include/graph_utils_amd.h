@@ -80,6 +80,10 @@ typedef struct {
                            than n / beta vertices (0 = default) */
   int32_t sync_every;   /* levels launched between host checks (0 = adaptive) */
   int32_t engine;       /* GU_ENGINE_* (default AUTO) */
+  int32_t level_graphs; /* batched engine: 0 / 1 = replay each chunk of levels
+                           as a captured HIP graph, -1 = launch its kernels
+                           one by one */
+  int32_t reserved;
   void *stream;         /* hipStream_t; NULL = the default stream */
 } gu_config;
 

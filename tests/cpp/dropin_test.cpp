@@ -59,6 +59,11 @@ int main(int argc, char **argv) {
   ok &= check_dist(g, multi_search_scan_based_setup(g_d, start, end), start, end, "Scan based");
   ok &= check_dist(g, multi_search_CTA_warp_based_setup(g_d, start, end), start, end, "CTA+warp");
   ok &= check_dist(g, multi_search_shuffle_based_setup(g_d, op, start, end), start, end, "Shuffle based");
+  {
+    std::vector<std::vector<int>> tc;
+    transitive_closure_setup(g_d, start, end, tc);  // examples/transitive_closure/reachability.cu
+    ok &= check_dist(g, tc, start, end, "Transitive closure");
+  }
 
   dim3 grid, block;
   size_t rows = configure_grid(grid, block, start, end);

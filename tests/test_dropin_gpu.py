@@ -1,0 +1,27 @@
+"""The C++ drop-in surface on the GPU: tests/cpp/dropin_test.cpp is a
+graph-utils example driver written against the reference's signatures
+(parse -> device_graph -> every *_setup -> the reference's verification),
+checked with the oracle.  Built by __graft_entry__.build()."""
+import os
+import subprocess
+
+import pytest
+
+from conftest import ROOT, announce, graph_path
+
+pytestmark = pytest.mark.gpu
+
+EXE = os.path.join(ROOT, "tests", "cpp", "_build", "dropin_test")
+
+
+@pytest.mark.parametrize("name", ["jazz", "test3", "test6"])
+def test_dropin_driver(gpu, name):
+    if not os.path.exists(EXE):
+        pytest.fail("tests/cpp/_build/dropin_test is not built (run __graft_entry__.build())")
+    announce("dropin_test %s" % name)
+    r = subprocess.run([EXE, "-i", graph_path(name), "-f", "dimacs"], capture_output=True, text=True,
+                       timeout=300)
+    print(r.stdout[-3000:])
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
+    assert "Test failed" not in r.stdout
+    assert r.stdout.count("Test passed") >= 9
