@@ -165,11 +165,11 @@ class TuneResult(ctypes.Structure):
     _fields_ = [("engine", ctypes.c_int32), ("variant", ctypes.c_int32),
                 ("batch_width", ctypes.c_int32), ("wg_size", ctypes.c_int32),
                 ("alpha", ctypes.c_int32), ("trials", ctypes.c_int32), ("sources", ctypes.c_int32),
-                ("reserved", ctypes.c_int32), ("ms_default", ctypes.c_double),
+                ("beta", ctypes.c_int32), ("ms_default", ctypes.c_double),
                 ("ms_tuned", ctypes.c_double)]
 
     def as_dict(self):
-        d = {f[0]: getattr(self, f[0]) for f in self._fields_ if f[0] != "reserved"}
+        d = {f[0]: getattr(self, f[0]) for f in self._fields_}
         d["ms_default"] = round(d["ms_default"], 4)
         d["ms_tuned"] = round(d["ms_tuned"], 4)
         return d

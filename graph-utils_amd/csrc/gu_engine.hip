@@ -882,7 +882,7 @@ struct gu_graph {
   // Autotuned launch shape per job family: [0] searches / diameter,
   // [1] Brandes.  Zero fields mean "not tuned".
   struct Tuned {
-    int engine = 0, variant = -1, width = 0, wg = 0, alpha = 0;
+    int engine = 0, variant = -1, width = 0, wg = 0, alpha = 0, beta = 0;
   } tuned[2];
   // Output row tables of one launch (device, with a pinned host mirror),
   // eight tables of 256 entries: [0,256) distance rows, [256,512) path-count
@@ -1522,6 +1522,7 @@ int run_job(gu_graph *g, const int32_t *sources, int32_t nsrc, Job job, const Ho
     if (cfg.variant == GU_VARIANT_AUTO && t.variant >= 0) cfg.variant = t.variant;
     if (cfg.wg_size == 0) cfg.wg_size = t.wg;
     if (cfg.alpha == 0) cfg.alpha = t.alpha;
+    if (cfg.beta == 0) cfg.beta = t.beta;
     if (cfg.batch_width == 0 && t.width > 0) {
       const int w = nsrc <= 64 ? 64 : (nsrc <= 128 ? 128 : 256);
       cfg.batch_width = std::min(w, t.width);
@@ -2121,7 +2122,8 @@ int gu_autotune(gu_graph_t g, const int32_t *sources, int32_t nsrc, int32_t kind
       }
     }
     // sources per launch, workgroup size, direction threshold
-    const int widths[] = {64, 128, 256}, wgs[] = {256, 512, 1024}, alphas[] = {4, 16, 64};
+    const int widths[] = {64, 128, 256}, wgs[] = {256, 512, 1024}, alphas[] = {4, 16, 64, 256, 1024};
+    const int betas[] = {16, 64, 256};
     for (int w : widths) {
       if (rc || w > ((k + 63) / 64) * 64) break;
       gu_config c = best;
@@ -2149,6 +2151,15 @@ int gu_autotune(gu_graph_t g, const int32_t *sources, int32_t nsrc, int32_t kind
         best = c;
       }
     }
+    for (int b : betas) {
+      if (rc) break;
+      gu_config c = best;
+      c.beta = b;
+      if ((rc = trial(c, &t)) == GU_OK && t < t_best) {
+        t_best = t;
+        best = c;
+      }
+    }
   }
   if (dist) (void)hipFree(dist);
   if (sig) (void)hipFree(sig);
@@ -2163,12 +2174,14 @@ int gu_autotune(gu_graph_t g, const int32_t *sources, int32_t nsrc, int32_t kind
   tn.width = best.batch_width;
   tn.wg = best.wg_size;
   tn.alpha = best.alpha;
+  tn.beta = best.beta;
   if (out) {
     out->engine = tn.engine;
     out->variant = tn.variant >= 0 ? tn.variant : GU_VARIANT_AUTO;
     out->batch_width = tn.width;
     out->wg_size = tn.wg;
     out->alpha = tn.alpha;
+    out->beta = tn.beta;
     out->trials = trials;
     out->ms_default = t_default;
     out->ms_tuned = t_best;

@@ -212,10 +212,10 @@ typedef struct {
   int32_t variant;      /* GU_VARIANT_* (AUTO when the per-source engine won) */
   int32_t batch_width;  /* 0 = by source count */
   int32_t wg_size;      /* 0 = 256 */
-  int32_t alpha;        /* 0 = default direction threshold */
+  int32_t alpha;        /* 0 = default top-down -> bottom-up threshold */
   int32_t trials;       /* configurations timed */
   int32_t sources;      /* sources per trial */
-  int32_t reserved;
+  int32_t beta;         /* 0 = default bottom-up -> top-down threshold */
   double ms_default;    /* device ms of the untuned configuration */
   double ms_tuned;      /* device ms of the chosen one */
 } gu_tune_result;
@@ -224,7 +224,7 @@ typedef struct {
  * (GU_QUERY_BFS: searches with path counts, also used by diameter;
  * GU_QUERY_BC: Brandes) over engines, top-down expanders (shuffle, warp,
  * CTA+warp, scan, edge-parallel), sources per launch, workgroup sizes and
- * the top-down -> bottom-up threshold, starting from the degree-profile
+ * both direction-switch thresholds, starting from the degree-profile
  * defaults, and keeps the fastest in the handle: later calls whose
  * gu_config fields are left at auto / 0 use it (handles made afterwards by
  * gu_graph_share inherit it).  Allocates trial outputs for the duration. */
