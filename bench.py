@@ -452,7 +452,8 @@ def main():
     ap.add_argument("--no-variants", action="store_true", help="skip the per-variant sweep")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (degree-profile defaults)")
     ap.add_argument("--level-graphs", type=int, default=0,
-                    help="batched engine: 0/1 replay captured level chunks, -1 launch kernels directly")
+                    help="batched engine: 1 replays captured chunks of levels as HIP graphs, 0 launches "
+                         "the kernels directly")
     ap.add_argument("--bc-check", type=int, default=0,
                     help="bc workload: compare BC over this many sources with the FP64 CPU oracle")
     ap.add_argument("--calib-bytes", type=int, default=0,

@@ -1229,7 +1229,7 @@ int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const g
       e = make_eng<NW>(g, nb, stack, sigma);
       set_grids(e);
     }
-    if (cfg.level_graphs >= 0) {
+    if (cfg.level_graphs > 0) {
       hipGraphExec_t ex = nullptr;
       rc = level_graph<NW>(g, e, variant, block, grid, K, sigma, &ex);
       if (rc) return rc;
@@ -1380,10 +1380,13 @@ int run_persource(gu_graph *g, const int *sources, int nsrc, const HostRows &hr,
     if (b0 == 0) k_fill_i32<<<1, 64, 0, st>>>(p.ovf, 1, 0);
     PsArgs a{rl ? g->LR : g->R, rl ? g->LC : g->C, g->n, p.srcs, (int *const *)g->rowtab,
              sigma ? (u64 *const *)(g->rowtab + 256) : nullptr, p.queues, p.depth, p.part, p.ovf};
-    if (sigma)
-      k_persource<true><<<nb, 1024, 0, st>>>(a);
+    // undirected graphs pull path counts from the previous level
+    if (sigma && !g->directed)
+      k_persource<true, true><<<nb, 1024, 0, st>>>(a);
+    else if (sigma)
+      k_persource<true, false><<<nb, 1024, 0, st>>>(a);
     else
-      k_persource<false><<<nb, 1024, 0, st>>>(a);
+      k_persource<false, false><<<nb, 1024, 0, st>>>(a);
     GU_HIP(hipGetLastError());
     if (rl) {
       const dim3 ug((unsigned)std::max(1, std::min(pick_grid(g, 1), (g->n + 255) / 256)), (unsigned)nb);
@@ -1456,7 +1459,10 @@ int run_persource_bc(gu_graph *g, const int *sources, int nsrc, const HostRows &
     PsBcArgs a{rl ? g->LR : g->R, rl ? g->LC : g->C, g->n, p.srcs, (int *const *)g->rowtab,
                (u64 *const *)(g->rowtab + 256), (float *const *)(g->rowtab + 512), p.queues, p.depth,
                p.part, p.ovf};
-    k_ps_bc<<<nb, 1024, 0, st>>>(a);
+    if (g->directed)
+      k_ps_bc<false><<<nb, 1024, 0, st>>>(a);
+    else
+      k_ps_bc<true><<<nb, 1024, 0, st>>>(a);
     GU_HIP(hipGetLastError());
     if (rl && hr.delta) {
       const dim3 ug((unsigned)std::max(1, std::min(pick_grid(g, 1), (g->n + 255) / 256)), (unsigned)nb);

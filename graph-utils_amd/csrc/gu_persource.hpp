@@ -6,9 +6,10 @@
 // sharing a level across sources buys nothing while thousands of levels each
 // pay global passes.  Here a workgroup owns one source, as the reference's
 // kernels do (one block per source, shuffle_based.cuh:104), and separates
-// levels with workgroup barriers instead of kernel launches.  Distances and
-// path counts go straight into the caller's rows; the frontier
-// queues live in HBM per workgroup.
+// levels with workgroup barriers instead of kernel launches.  The frontier
+// of a level lives in LDS (vertex and adjacency bounds, so the next level
+// needs no row-offset loads), with entries beyond its capacity in an HBM
+// queue per workgroup.
 //
 // Visit (count_shortest_paths' visit, shuffle_based.cu:130-141): a compare-
 // and-swap claims an unvisited vertex for level L+1; every edge v->w with
@@ -25,7 +26,7 @@ struct PsArgs {
   const int *srcs;   // [nsrc]
   int *const *dist;  // [nsrc] distance rows of n entries
   u64 *const *sigma; // [nsrc] path-count rows, or null
-  int *queues;       // [nsrc][2][n]
+  int *queues;       // [nsrc][2n+2]: frontier entries beyond the LDS capacity
   int *depth;        // [nsrc] deepest level reached
   u64 *part;         // [nsrc][3] (TEPS units, reached vertices, DAG edges)
   int *ovf;          // set when a path count wrapped
@@ -38,225 +39,246 @@ __device__ __forceinline__ int ld_l2(const int *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-#ifndef GU_PS_PREREAD
-#define GU_PS_PREREAD 1
-#endif
-#ifndef GU_PS_SPEC
-#define GU_PS_SPEC 0
-#endif
-#ifndef GU_PS_NORET
-#define GU_PS_NORET 0
+#ifndef GU_PS_CAP
+#define GU_PS_CAP 3072
 #endif
 
-// Visits for NV frontier vertices of one thread, B adjacency entries of
-// each per step: all column loads, then all claims, then all path-count
-// adds, so a step costs three memory round trips for up to NV*B edges.  A
-// claim is a compare-and-swap INT_MAX -> L+1 whose return is the vertex's
-// distance either way, so no separate read of d precedes it.  Every edge
-// v->w with d[w] = L+1 adds sigma[v] to sigma[w] (count_shortest_paths'
-// visit, shuffle_based.cu:130-141); claimed vertices are appended to Q2 at
-// positions taken from the level's LDS counter, one atomic per thread and
-// step.
-template <bool SIGMA, int NV, int B>
-__device__ __forceinline__ void ps_visit_multi(const int *C, int *d, u64 *sg, int (&r)[NV],
-                                               const int (&re)[NV], const u64 (&sv)[NV], int L, int &ovf,
-                                               u64 &dag, unsigned *cnt, int *Q2) {
-  while (true) {
-    bool any = false;
-#pragma unroll
-    for (int u = 0; u < NV; u++) any |= r[u] < re[u];
-    if (!any) break;
-    int w[NV][B], o[NV][B];
-#pragma unroll
-    for (int u = 0; u < NV; u++)
-#pragma unroll
-      for (int j = 0; j < B; j++) w[u][j] = r[u] + j < re[u] ? C[r[u] + j] : -1;
-#if GU_PS_PREREAD
-    // a plain read first: only unvisited vertices take the atomic
-#pragma unroll
-    for (int u = 0; u < NV; u++)
-#pragma unroll
-      for (int j = 0; j < B; j++) o[u][j] = w[u][j] >= 0 ? d[w[u][j]] : -1;
-#pragma unroll
-    for (int u = 0; u < NV; u++)
-#pragma unroll
-      for (int j = 0; j < B; j++)
-        if (o[u][j] == INT_MAX) o[u][j] = atomicCAS(&d[w[u][j]], INT_MAX, L + 1);
-#else
-#pragma unroll
-    for (int u = 0; u < NV; u++)
-#pragma unroll
-      for (int j = 0; j < B; j++) o[u][j] = w[u][j] >= 0 ? atomicCAS(&d[w[u][j]], INT_MAX, L + 1) : -1;
-#endif
-    unsigned claims = 0;
-#pragma unroll
-    for (int u = 0; u < NV; u++)
-#pragma unroll
-      for (int j = 0; j < B; j++) claims += o[u][j] == INT_MAX;
-    if (claims) {
-      unsigned pos = atomicAdd(cnt, claims);
-#pragma unroll
-      for (int u = 0; u < NV; u++)
-#pragma unroll
-        for (int j = 0; j < B; j++)
-          if (o[u][j] == INT_MAX) Q2[pos++] = w[u][j];
-    }
-    if (SIGMA) {
-      u64 old[NV][B];
-#if GU_PS_NORET
-#pragma unroll
-      for (int u = 0; u < NV; u++)
-#pragma unroll
-        for (int j = 0; j < B; j++) {
-          old[u][j] = 0ull;
-          if (o[u][j] == INT_MAX || o[u][j] == L + 1) (void)atomicAdd(&sg[w[u][j]], sv[u]);
-        }
-#else
-#pragma unroll
-      for (int u = 0; u < NV; u++)
-#pragma unroll
-        for (int j = 0; j < B; j++)
-          old[u][j] = (o[u][j] == INT_MAX || o[u][j] == L + 1) ? atomicAdd(&sg[w[u][j]], sv[u]) : 0ull;
-#endif
-#pragma unroll
-      for (int u = 0; u < NV; u++)
-#pragma unroll
-        for (int j = 0; j < B; j++)
-          if (o[u][j] == INT_MAX || o[u][j] == L + 1) {
-            ovf |= old[u][j] + sv[u] < sv[u];
-            dag++;
-          }
-    }
-#pragma unroll
-    for (int u = 0; u < NV; u++) r[u] += B;
-  }
-}
+// LDS frontier of the per-source kernels: two buffers of kPsCapLds entries
+// (vertex, first and one-past-last adjacency index); a level's entries
+// beyond that go to `spill` (vertices only) in HBM.
+constexpr int kPsCapLds = GU_PS_CAP;
+struct PsFront {
+  int v[2][kPsCapLds];
+  int r[2][kPsCapLds];
+  int e[2][kPsCapLds];
+};
 
 // Vertices of degree above this are strip-mined by a whole wave.
 constexpr int kPsSerialDeg = 32;
-#ifndef GU_PS_NV
-#define GU_PS_NV 1
-#endif
-#ifndef GU_PS_B
-#define GU_PS_B 8
-#endif
-constexpr int kPsNV = GU_PS_NV, kPsB = GU_PS_B;
 
-// Expansion inside the workgroup: each thread takes kPsNV frontier vertices
-// at a time (ps_visit_multi; one vertex and 8-entry steps measured fastest
-// with 256 sources in flight: more requests per thread only add memory
-// contention); adjacency lists longer than kPsSerialDeg are
-// strip-mined by the wave, the lanes electing the highest lane that still
-// has one (race_and_resolve, warp_based.cu:109-145).  Levels are separated
-// by one workgroup barrier: the level counters rotate over three LDS words,
-// the one two levels back being cleared for the next level.
-template <bool SIGMA>
-__global__ __launch_bounds__(1024) void k_persource(PsArgs a) {
-  __shared__ unsigned s_cnt[3];
-  __shared__ u64 s_red[16][2];
-  __shared__ int s_ovf;
-  u64 dag = 0;
-  const int tid = threadIdx.x, T = blockDim.x;
-  const int lane = lane_id();
-  const int s = a.srcs[blockIdx.x];
-  const size_t row = (size_t)blockIdx.x * a.n;
-  int *d = a.dist[blockIdx.x];
-  u64 *sg = SIGMA ? a.sigma[blockIdx.x] : nullptr;
-  int *Q = a.queues + 2 * row;
-  int *Q2 = Q + a.n;
-  for (int i = tid; i < a.n; i += T) {
-    d[i] = INT_MAX;
-    if (SIGMA) sg[i] = 0;
+// Reads entry k of buffer b: (vertex, adjacency bounds).
+__device__ __forceinline__ void ps_entry(const PsFront &f, const int *spill, const int *R, int b,
+                                         unsigned k, int &v, int &r, int &re) {
+  if (k < (unsigned)kPsCapLds) {
+    v = f.v[b][k];
+    r = f.r[b][k];
+    re = f.e[b][k];
+  } else {
+    v = spill[k - kPsCapLds];
+    r = R[v];
+    re = R[v + 1];
   }
+}
+
+// Appends claimed vertex w at position p of the next buffer.  The bounds
+// are loaded here, while the level's path-count adds are in flight.
+__device__ __forceinline__ void ps_push(PsFront &f, int *spill, int *S, const int *R, int b, unsigned p,
+                                        int w) {
+  if (S) S[p] = w;
+  if (p < (unsigned)kPsCapLds) {
+    f.v[b][p] = w;
+    f.r[b][p] = R[w];
+    f.e[b][p] = R[w + 1];
+  } else {
+    spill[p - kPsCapLds] = w;
+  }
+}
+
+// Visits of frontier vertex v (level L) over its adjacency [r, re) in
+// steps of 8: the column loads, then reads of d, then compare-and-swap
+// claims INT_MAX -> L+1 of the unvisited neighbours, whose positions one LDS
+// atomic per step reserves (S, for Brandes, also records them in HBM).
+// Path counts (count_shortest_paths' visit, shuffle_based.cu:130-141: every
+// edge u->w with d[w] = d[u]+1 adds sigma[u] to sigma[w]) are formed in one
+// of two ways:
+//  - PULL (undirected graphs, where the in-neighbours are the adjacency):
+//    sigma[v] is the sum of sigma[u] over the neighbours u with d[u] = L-1,
+//    read in the same pass, so no atomics; d and sigma are read through L2
+//    because other workgroup waves wrote them through L2;
+//  - push (directed graphs): each edge v->w with d[w] = L+1 adds sigma[v]
+//    to sigma[w] atomically.
+// Both sums are exact mod 2^64; `ovf` flags a wrap.
+// Lanes past the adjacency carry w = -1 and o = -1; every use of o that a
+// level can equal (o == L-1 is -1 at level 0) is guarded by w >= 0.
+template <bool SIGMA, bool PULL>
+__device__ __forceinline__ void ps_visit(const int *R, const int *C, int *d, u64 *sg, int v, int r, int re,
+                                         int L, u64 sv, int &ovf, u64 &dag, unsigned *cnt, PsFront &f,
+                                         int *spill, int *S, int nbuf) {
+  constexpr int B = 8;
+  u64 acc = 0;
+  for (; r < re; r += B) {
+    int w[B], o[B];
+#pragma unroll
+    for (int j = 0; j < B; j++) w[j] = r + j < re ? C[r + j] : -1;
+#pragma unroll
+    for (int j = 0; j < B; j++) o[j] = w[j] >= 0 ? ((SIGMA && PULL) ? ld_l2(&d[w[j]]) : d[w[j]]) : -1;
+#pragma unroll
+    for (int j = 0; j < B; j++)
+      if (o[j] == INT_MAX) o[j] = atomicCAS(&d[w[j]], INT_MAX, L + 1);  // INT_MAX back: claimed
+    unsigned claims = 0;
+#pragma unroll
+    for (int j = 0; j < B; j++) claims += o[j] == INT_MAX;
+    unsigned pos = claims ? atomicAdd(cnt, claims) : 0u;
+    u64 x[B];
+    if (SIGMA) {
+#pragma unroll
+      for (int j = 0; j < B; j++) {
+        if (PULL)
+          x[j] = (w[j] >= 0 && L >= 1 && o[j] == L - 1) ? ld_l2(&sg[w[j]]) : 0ull;
+        else
+          x[j] = (o[j] == INT_MAX || o[j] == L + 1) ? atomicAdd(&sg[w[j]], sv) : 0ull;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < B; j++)
+      if (o[j] == INT_MAX) ps_push(f, spill, S, R, nbuf, pos++, w[j]);
+    if (SIGMA) {
+#pragma unroll
+      for (int j = 0; j < B; j++) {
+        if (PULL && w[j] >= 0 && L >= 1 && o[j] == L - 1) {
+          acc += x[j];
+          ovf |= acc < x[j];
+          dag++;
+        }
+        if (!PULL && (o[j] == INT_MAX || o[j] == L + 1)) {
+          ovf |= x[j] + sv < sv;
+          dag++;
+        }
+      }
+    }
+  }
+  if (SIGMA && PULL && v >= 0 && L >= 1) sg[v] = acc;
+}
+
+// A wave strip-mines the adjacency lists longer than kPsSerialDeg of its
+// lanes, electing the highest lane that still has one (race_and_resolve,
+// warp_based.cu:109-145); in PULL mode the lanes' partial path counts are
+// summed across the wave and the elected lane stores them.
+template <bool SIGMA, bool PULL>
+__device__ __forceinline__ void ps_visit_wave(const int *R, const int *C, int *d, u64 *sg, int v, int rr,
+                                              int rre, int L, u64 sv, int &ovf, u64 &dag, unsigned *cnt,
+                                              PsFront &f, int *spill, int *S, int nbuf) {
+  const int lane = lane_id();
+  while (true) {
+    const bool pend = rr < rre;
+    if (!ballot(pend)) break;
+    const int win = race_and_resolve_wave(pend);
+    const int rw = readlane(rr, win), rew = readlane(rre, win), vw = readlane(v, win);
+    const u64 svw = (SIGMA && !PULL) ? readlane64(sv, win) : 0ull;
+    if (lane == win) rr = rre;
+    u64 acc = 0;
+    int wrap = 0;
+    for (int e0 = rw; e0 < rew; e0 += kWave) {
+      const int ed = e0 + lane;
+      int w = -1, o = -1;
+      if (ed < rew) {
+        w = C[ed];
+        o = (SIGMA && PULL) ? ld_l2(&d[w]) : d[w];
+        if (o == INT_MAX) o = atomicCAS(&d[w], INT_MAX, L + 1);
+      }
+      const bool claimed = o == INT_MAX;
+      uint64_t m = ballot(claimed);
+      unsigned b = 0;
+      if (m) {
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        if (lane == leader) b = atomicAdd(cnt, (unsigned)__popcll(m));
+        b = (unsigned)readlane((int)b, leader);
+      }
+      if (SIGMA && PULL && w >= 0 && L >= 1 && o == L - 1) {
+        const u64 x = ld_l2(&sg[w]);
+        acc += x;
+        wrap |= acc < x;
+        dag++;
+      }
+      if (SIGMA && !PULL && (claimed || o == L + 1)) {
+        const u64 old = atomicAdd(&sg[w], svw);
+        ovf |= old + svw < svw;
+        dag++;
+      }
+      if (claimed) ps_push(f, spill, S, R, nbuf, b + rank_in(m), w);
+    }
+    if (SIGMA && PULL) {
+      for (int o = 32; o >= 1; o >>= 1) {
+        const u64 y = __shfl_xor(acc, o, kWave);
+        acc += y;
+        wrap |= acc < y;
+      }
+      ovf |= wrap;
+      if (lane == win && L >= 1) sg[vw] = acc;
+    }
+  }
+}
+
+// The level loop of the search and Brandes kernels.  Levels are separated
+// by one workgroup barrier: the level counters rotate over three LDS words,
+// the one two levels back being cleared for the next level.  With BC every
+// level's vertices are also appended to S in HBM (E[l] = start of level l),
+// and a frontier's entries beyond the LDS capacity are read back from S;
+// otherwise they go to the two `spill` queues.  Returns the deepest level.
+template <bool SIGMA, bool PULL, bool BC>
+__device__ int ps_levels(const int *R, const int *C, int n, int s, int *d, u64 *sg, int *spill0, int *S,
+                         int *E, PsFront &f, unsigned *s_cnt, int &ovf, u64 &teps, u64 &reached, u64 &dag) {
+  const int tid = threadIdx.x, T = blockDim.x;
   if (tid < 3) s_cnt[tid] = 0;
-  if (tid == 0) s_ovf = 0;
-  __syncthreads();
   if (tid == 0) {
     d[s] = 0;
     if (SIGMA) sg[s] = 1;
-    Q[0] = s;
+    f.v[0][0] = s;
+    f.r[0][0] = R[s];
+    f.e[0][0] = R[s + 1];
+    if (BC) {
+      S[0] = s;
+      E[0] = 0;
+      E[1] = 1;
+    }
   }
   __syncthreads();
-  unsigned qn = 1;
-  int L = 0;
-  int ovf = 0;
-  u64 teps = 0, reached = 1;
+  unsigned qn = 1, qb = 0, qs = 1;  // frontier size; its start and end in S
+  int L = 0, b = 0;
+  reached = 1;
   while (true) {
     unsigned *cnt = &s_cnt[L % 3];
     if (tid == 0) s_cnt[(L + 1) % 3] = 0;
-    for (unsigned base = 0; base < qn; base += kPsNV * T) {
-      int r[kPsNV], re[kPsNV], rb[kPsNV], rbe[kPsNV];
-      u64 sv[kPsNV];
-#pragma unroll
-      for (int u = 0; u < kPsNV; u++) {
-        const unsigned k = base + u * T + tid;
-        const int v = k < qn ? Q[k] : -1;
-        r[u] = v >= 0 ? a.R[v] : 0;
-        re[u] = v >= 0 ? a.R[v + 1] : 0;
-        sv[u] = (SIGMA && v >= 0) ? ld_l2(&sg[v]) : 0ull;
-      }
-#pragma unroll
-      for (int u = 0; u < kPsNV; u++) {
-        teps += (u64)(re[u] - r[u]);
-        const bool big = re[u] - r[u] > kPsSerialDeg;
-        rb[u] = big ? r[u] : 0;
-        rbe[u] = big ? re[u] : 0;
-        if (big) re[u] = r[u];
-      }
-      ps_visit_multi<SIGMA, kPsNV, kPsB>(a.C, d, sg, r, re, sv, L, ovf, dag, cnt, Q2);
-      // long lists: the wave strip-mines the elected lane's list
-      for (int u = 0; u < kPsNV; u++) {
-        int rr = rb[u];
-        const int rre = rbe[u];
-        while (true) {
-          const bool pend = rr < rre;
-          if (!ballot(pend)) break;
-          const int win = race_and_resolve_wave(pend);
-          const int rw = readlane(rr, win), rew = readlane(rre, win);
-          const u64 svw = SIGMA ? readlane64(sv[u], win) : 0ull;
-          if (lane == win) rr = rre;
-          for (int e0 = rw; e0 < rew; e0 += kWave) {
-            const int ed = e0 + lane;
-            int w = -1, o = -1;
-            if (ed < rew) {
-              w = a.C[ed];
-              o = atomicCAS(&d[w], INT_MAX, L + 1);
-            }
-            const bool claimed = o == INT_MAX;
-            uint64_t m = ballot(claimed);
-            if (m) {
-              const int leader = __ffsll((unsigned long long)m) - 1;
-              unsigned b = 0;
-              if (lane == leader) b = atomicAdd(cnt, (unsigned)__popcll(m));
-              b = (unsigned)readlane((int)b, leader);
-              if (claimed) Q2[b + rank_in(m)] = w;
-            }
-            if (SIGMA && (claimed || o == L + 1)) {
-              const u64 old = atomicAdd(&sg[w], svw);
-              ovf |= old + svw < svw;
-              dag++;
-            }
-          }
-        }
-      }
+    int *Snext = BC ? S + qs : nullptr;
+    int *sp_cur = BC ? S + qb + kPsCapLds : spill0 + (size_t)b * n;
+    int *sp_next = BC ? S + qs + kPsCapLds : spill0 + (size_t)(b ^ 1) * n;
+    for (unsigned base = 0; base < qn; base += T) {
+      const unsigned k = base + tid;
+      int v = -1, r = 0, re = 0;
+      if (k < qn) ps_entry(f, sp_cur, R, b, k, v, r, re);
+      const u64 sv = (SIGMA && !PULL && v >= 0) ? ld_l2(&sg[v]) : 0ull;
+      teps += (u64)(re - r);
+      const bool big = re - r > kPsSerialDeg;
+      ps_visit<SIGMA, PULL>(R, C, d, sg, v, big ? 0 : r, big ? 0 : re, L, sv, ovf, dag, cnt, f, sp_next,
+                            Snext, b ^ 1);
+      ps_visit_wave<SIGMA, PULL>(R, C, d, sg, v, big ? r : 0, big ? re : 0, L, sv, ovf, dag, cnt, f, sp_next,
+                                 Snext, b ^ 1);
     }
     __syncthreads();
     const unsigned q2n = *cnt;
     if (q2n == 0) break;
     reached += q2n;
-    int *t = Q;
-    Q = Q2;
-    Q2 = t;
     qn = q2n;
+    qb = qs;
+    qs += q2n;
+    b ^= 1;
     L++;
+    if (BC && tid == 0) E[L + 1] = (int)qs;
   }
-  // workgroup sums (lane-parallel per wave, then per workgroup)
+  return L;
+}
+
+// Workgroup sums of the per-thread counters; thread 0 writes the source's
+// part / depth entries.
+__device__ __forceinline__ void ps_finish(u64 teps, u64 dag, u64 reached, int L, int ovf, int *s_ovf,
+                                          u64 (*s_red)[2], u64 *part, int *depth, int *ovf_out) {
+  const int tid = threadIdx.x, T = blockDim.x;
   for (int o = 32; o >= 1; o >>= 1) {
     teps += __shfl_xor(teps, o, kWave);
     dag += __shfl_xor(dag, o, kWave);
   }
-  if (ovf) atomicOr(&s_ovf, 1);
-  if (lane == 0) {
+  if (ovf) atomicOr(s_ovf, 1);
+  if (lane_id() == 0) {
     s_red[wave_id()][0] = teps;
     s_red[wave_id()][1] = dag;
   }
@@ -267,12 +289,36 @@ __global__ __launch_bounds__(1024) void k_persource(PsArgs a) {
       t += s_red[w][0];
       g += s_red[w][1];
     }
-    a.part[3 * blockIdx.x] = t;
-    a.part[3 * blockIdx.x + 1] = reached;
-    a.part[3 * blockIdx.x + 2] = g;
-    a.depth[blockIdx.x] = L;
-    if (s_ovf) atomicOr(a.ovf, 1);
+    part[3 * blockIdx.x] = t;
+    part[3 * blockIdx.x + 1] = reached;
+    part[3 * blockIdx.x + 2] = g;
+    depth[blockIdx.x] = L;
+    if (*s_ovf) atomicOr(ovf_out, 1);
   }
+}
+
+template <bool SIGMA, bool PULL>
+__global__ __launch_bounds__(1024) void k_persource(PsArgs a) {
+  __shared__ PsFront f;
+  __shared__ unsigned s_cnt[3];
+  __shared__ u64 s_red[16][2];
+  __shared__ int s_ovf;
+  const int tid = threadIdx.x, T = blockDim.x;
+  const int s = a.srcs[blockIdx.x];
+  int *d = a.dist[blockIdx.x];
+  u64 *sg = SIGMA ? a.sigma[blockIdx.x] : nullptr;
+  int *spill = a.queues + (size_t)blockIdx.x * (2 * (size_t)a.n + 2);
+  for (int i = tid; i < a.n; i += T) {
+    d[i] = INT_MAX;
+    if (SIGMA) sg[i] = 0;
+  }
+  if (tid == 0) s_ovf = 0;
+  __syncthreads();
+  int ovf = 0;
+  u64 teps = 0, reached = 1, dag = 0;
+  const int L = ps_levels<SIGMA, PULL, false>(a.R, a.C, a.n, s, d, sg, spill, nullptr, nullptr, f, s_cnt,
+                                              ovf, teps, reached, dag);
+  ps_finish(teps, dag, reached, L, ovf, &s_ovf, s_red, a.part, a.depth, a.ovf);
 }
 
 // Brandes on one workgroup per source (the dependency accumulation of
@@ -292,7 +338,7 @@ struct PsBcArgs {
   int *const *dist;     // [nsrc] scratch rows
   u64 *const *sigma;    // [nsrc] scratch rows
   float *const *delta;  // [nsrc] dependency rows
-  int *stack;           // [nsrc][2n+2]: S, then level starts E
+  int *stack;           // [nsrc][2n+2]: S, then level starts E (HBM)
   int *depth;
   u64 *part;
   int *ovf;
@@ -307,152 +353,68 @@ __device__ __forceinline__ void ps_neumaier(float &sum, float &comp, float x) {
   sum = t;
 }
 
+template <bool PULL>
 __global__ __launch_bounds__(1024) void k_ps_bc(PsBcArgs a) {
+  __shared__ PsFront f;
   __shared__ unsigned s_cnt[3];
   __shared__ u64 s_red[16][2];
   __shared__ int s_ovf;
   const int tid = threadIdx.x, T = blockDim.x;
-  const int lane = lane_id();
   const int s = a.srcs[blockIdx.x];
   int *d = a.dist[blockIdx.x];
   u64 *sg = a.sigma[blockIdx.x];
   float *dl = a.delta[blockIdx.x];
   int *S = a.stack + (size_t)blockIdx.x * (2 * (size_t)a.n + 2);
   int *E = S + a.n;
+  // frontier entries beyond the LDS capacity are read back from S itself:
+  // a level's vertices occupy S[E[l], E[l+1]) in frontier order
   for (int i = tid; i < a.n; i += T) {
     d[i] = INT_MAX;
     sg[i] = 0;
     dl[i] = 0.0f;
   }
-  if (tid < 3) s_cnt[tid] = 0;
   if (tid == 0) s_ovf = 0;
   __syncthreads();
-  if (tid == 0) {
-    d[s] = 0;
-    sg[s] = 1;
-    S[0] = s;
-    E[0] = 0;
-    E[1] = 1;
-  }
-  __syncthreads();
-  unsigned qb = 0, qe = 1;
-  int L = 0, ovf = 0;
+  int ovf = 0;
   u64 teps = 0, reached = 1, dag = 0;
-  while (true) {
-    unsigned *cnt = &s_cnt[L % 3];
-    if (tid == 0) s_cnt[(L + 1) % 3] = 0;
-    for (unsigned base = qb; base < qe; base += kPsNV * T) {
-      int r[kPsNV], re[kPsNV];
-      u64 sv[kPsNV];
-#pragma unroll
-      for (int u = 0; u < kPsNV; u++) {
-        const unsigned k = base + u * T + tid;
-        const int v = k < qe ? S[k] : -1;
-        r[u] = v >= 0 ? a.R[v] : 0;
-        re[u] = v >= 0 ? a.R[v + 1] : 0;
-        sv[u] = v >= 0 ? ld_l2(&sg[v]) : 0ull;
-      }
-#pragma unroll
-      for (int u = 0; u < kPsNV; u++) teps += (u64)(re[u] - r[u]);
-      ps_visit_multi<true, kPsNV, kPsB>(a.C, d, sg, r, re, sv, L, ovf, dag, cnt, S + qe);
-    }
-    __syncthreads();
-    const unsigned q2n = *cnt;
-    if (q2n == 0) break;
-    reached += q2n;
-    qb = qe;
-    qe += q2n;
-    L++;
-    if (tid == 0) E[L + 1] = (int)qe;
-  }
+  const int L = ps_levels<true, PULL, true>(a.R, a.C, a.n, s, d, sg, nullptr, S, E, f, s_cnt, ovf, teps,
+                                            reached, dag);
   __syncthreads();
-  // backward, deepest level first (its vertices have no successors); each
-  // thread takes kPsNV vertices, loads d, sigma and delta of a chunk's
-  // neighbours together and sums the successors in adjacency order
+  // backward, deepest level first (its vertices have no successors): each
+  // thread sums the successors of one vertex in adjacency order, loading a
+  // chunk's distances together, then its successors' path counts and
+  // dependencies together
   for (int l = L - 1; l >= 1; l--) {
     const int b0 = E[l], b1 = E[l + 1];
-    for (int i0 = b0 + tid; i0 < b1; i0 += kPsNV * T) {
-      int v[kPsNV], r[kPsNV], re[kPsNV];
-      float sv[kPsNV], sum[kPsNV], comp[kPsNV];
+    for (int i = b0 + tid; i < b1; i += T) {
+      const int v = S[i];
+      int r = a.R[v];
+      const int re = a.R[v + 1];
+      const float sv = (float)ld_l2(&sg[v]);
+      float sum = 0.0f, comp = 0.0f;
+      constexpr int B = 8;
+      for (; r < re; r += B) {
+        int w[B];
+        bool on[B];
+        float sw[B], xw[B];
 #pragma unroll
-      for (int u = 0; u < kPsNV; u++) {
-        const int i = i0 + u * T;
-        v[u] = i < b1 ? S[i] : -1;
-        r[u] = v[u] >= 0 ? a.R[v[u]] : 0;
-        re[u] = v[u] >= 0 ? a.R[v[u] + 1] : 0;
-        sv[u] = v[u] >= 0 ? (float)ld_l2(&sg[v[u]]) : 0.0f;
-        sum[u] = 0.0f;
-        comp[u] = 0.0f;
+        for (int j = 0; j < B; j++) w[j] = r + j < re ? a.C[r + j] : -1;
+#pragma unroll
+        for (int j = 0; j < B; j++) on[j] = w[j] >= 0 && ld_l2(&d[w[j]]) == l + 1;
+#pragma unroll
+        for (int j = 0; j < B; j++) {
+          sw[j] = on[j] ? (float)ld_l2(&sg[w[j]]) : 1.0f;
+          xw[j] = on[j] ? dl[w[j]] : 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < B; j++)
+          if (on[j]) ps_neumaier(sum, comp, (sv / sw[j]) * (1.0f + xw[j]));
       }
-      while (true) {
-        bool any = false;
-#pragma unroll
-        for (int u = 0; u < kPsNV; u++) any |= r[u] < re[u];
-        if (!any) break;
-        int w[kPsNV][kPsB];
-        int dw[kPsNV][kPsB];
-        float sw[kPsNV][kPsB], xw[kPsNV][kPsB];
-#pragma unroll
-        for (int u = 0; u < kPsNV; u++)
-#pragma unroll
-          for (int j = 0; j < kPsB; j++) w[u][j] = r[u] + j < re[u] ? a.C[r[u] + j] : -1;
-#pragma unroll
-        for (int u = 0; u < kPsNV; u++)
-#pragma unroll
-          for (int j = 0; j < kPsB; j++) {
-            const int x = w[u][j] >= 0 ? w[u][j] : 0;
-            dw[u][j] = w[u][j] >= 0 ? ld_l2(&d[x]) : -1;
-#if GU_PS_SPEC
-            sw[u][j] = (float)ld_l2(&sg[x]);
-            xw[u][j] = dl[x];
-#endif
-          }
-#if !GU_PS_SPEC
-#pragma unroll
-        for (int u = 0; u < kPsNV; u++)
-#pragma unroll
-          for (int j = 0; j < kPsB; j++) {
-            const bool on = dw[u][j] == l + 1;
-            sw[u][j] = on ? (float)ld_l2(&sg[w[u][j]]) : 1.0f;
-            xw[u][j] = on ? dl[w[u][j]] : 0.0f;
-          }
-#endif
-#pragma unroll
-        for (int u = 0; u < kPsNV; u++)
-#pragma unroll
-          for (int j = 0; j < kPsB; j++)
-            if (dw[u][j] == l + 1) ps_neumaier(sum[u], comp[u], (sv[u] / sw[u][j]) * (1.0f + xw[u][j]));
-#pragma unroll
-        for (int u = 0; u < kPsNV; u++) r[u] += kPsB;
-      }
-#pragma unroll
-      for (int u = 0; u < kPsNV; u++)
-        if (v[u] >= 0) dl[v[u]] = sum[u] + comp[u];
+      dl[v] = sum + comp;
     }
     __syncthreads();
   }
-  for (int o = 32; o >= 1; o >>= 1) {
-    teps += __shfl_xor(teps, o, kWave);
-    dag += __shfl_xor(dag, o, kWave);
-  }
-  if (ovf) atomicOr(&s_ovf, 1);
-  if (lane == 0) {
-    s_red[wave_id()][0] = teps;
-    s_red[wave_id()][1] = dag;
-  }
-  __syncthreads();
-  if (tid == 0) {
-    u64 t = 0, g = 0;
-    for (int w = 0; w < T / kWave; w++) {
-      t += s_red[w][0];
-      g += s_red[w][1];
-    }
-    a.part[3 * blockIdx.x] = t;
-    a.part[3 * blockIdx.x + 1] = reached;
-    a.part[3 * blockIdx.x + 2] = g;
-    a.depth[blockIdx.x] = L;
-    if (s_ovf) atomicOr(a.ovf, 1);
-  }
+  ps_finish(teps, dag, reached, L, ovf, &s_ovf, s_red, a.part, a.depth, a.ovf);
 }
 
 // BC accumulation of one per-source launch: for every vertex, the rows'

@@ -80,9 +80,12 @@ typedef struct {
                            than n / beta vertices (0 = default) */
   int32_t sync_every;   /* levels launched between host checks (0 = adaptive) */
   int32_t engine;       /* GU_ENGINE_* (default AUTO) */
-  int32_t level_graphs; /* batched engine: 0 / 1 = replay each chunk of levels
-                           as a captured HIP graph, -1 = launch its kernels
-                           one by one */
+  int32_t level_graphs; /* batched engine: 1 = replay each chunk of levels as
+                           a HIP graph captured on the handle's private
+                           stream; 0 = launch the kernels one by one.  A
+                           capture in progress makes HIP refuse legacy
+                           (NULL) stream work from other threads, so graphs
+                           are opt-in. */
   int32_t reserved;
   void *stream;         /* hipStream_t; NULL = the default stream */
 } gu_config;
@@ -140,7 +143,11 @@ const char *gu_strerror(int status);
 
 /* ---- device_graph (device_graph.h:7-22, device_graph.cpp:3-14) ------- */
 /* Copies R[n+1], C[m] and (optional) F[m] to HBM.  F may be NULL: it is then
- * rebuilt on the device from R.  Validates R monotone and C in range. */
+ * rebuilt on the device from R.  Validates R monotone and C in range.
+ * directed = 0 declares the CSR symmetric (every entry v->w has its w->v,
+ * as parse_metis and parse_market produce); the engines then pull path
+ * counts and frontier membership over the same adjacency.  directed = 1
+ * builds the in-edge CSR for the pulls. */
 int gu_graph_create(const int32_t *R, const int32_t *C, const int32_t *F,
                     int32_t n, int64_t m, int32_t directed, gu_graph_t *out);
 int gu_graph_destroy(gu_graph_t g);

@@ -493,7 +493,7 @@ def test_engine_auto_choice(gu, gpu, rmat14):
     assert st.engine_used == PS
 
 
-@pytest.mark.parametrize("graphs", [1, -1])
+@pytest.mark.parametrize("graphs", [1, 0])
 def test_level_graph_replay_matches_direct(gu, orc, gpu, graphs):
     """Captured chunks of levels replayed many times (several batches, widths
     and both job kinds, with eager work in between) equal direct launches and
@@ -507,8 +507,8 @@ def test_level_graph_replay_matches_direct(gu, orc, gpu, graphs):
                 d, s, _ = dg.multi_search(src, True, True, gu.make_config(level_graphs=graphs))
                 od, os_ = _oracle_rows(orc, hg, src[:6])
                 assert np.array_equal(d[:6], od) and np.array_equal(s[:6], os_)
-                d2, s2, _ = dg.multi_search(src, True, True, gu.make_config(level_graphs=-1))
+                d2, s2, _ = dg.multi_search(src, True, True, gu.make_config(level_graphs=0))
                 assert np.array_equal(d, d2) and np.array_equal(s, s2)
             x1, b1, _ = dg.betweenness(src[:40], config=gu.make_config(level_graphs=graphs))
-            x2, b2, _ = dg.betweenness(src[:40], config=gu.make_config(level_graphs=-1))
+            x2, b2, _ = dg.betweenness(src[:40], config=gu.make_config(level_graphs=0))
             assert np.array_equal(x1, x2) and np.array_equal(b1, b2)
