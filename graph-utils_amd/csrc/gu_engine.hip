@@ -883,6 +883,7 @@ struct gu_graph {
   // [1] Brandes.  Zero fields mean "not tuned".
   struct Tuned {
     int engine = 0, variant = -1, width = 0, wg = 0, alpha = 0, beta = 0;
+    int sources = 0;  // launch size it was tuned on
   } tuned[2];
   // Output row tables of one launch (device, with a pinned host mirror),
   // eight tables of 256 entries: [0,256) distance rows, [256,512) path-count
@@ -1521,8 +1522,10 @@ int run_job(gu_graph *g, const int32_t *sources, int32_t nsrc, Job job, const Ho
   if (!g) return GU_ERR_INVALID;
   gu_config cfg{};
   if (cfgp) cfg = *cfgp;
-  {
-    // fields left at "auto" take the autotuned shape of this job family
+  if (2 * nsrc >= g->tuned[job == JOB_BC ? 1 : 0].sources) {
+    // fields left at "auto" take the autotuned shape of this job family,
+    // for launches of about the size it was tuned on or larger (small
+    // launches keep the degree-profile defaults)
     const gu_graph::Tuned &t = g->tuned[job == JOB_BC ? 1 : 0];
     if (cfg.engine == GU_ENGINE_AUTO) cfg.engine = t.engine;
     if (cfg.variant == GU_VARIANT_AUTO && t.variant >= 0) cfg.variant = t.variant;
@@ -2181,6 +2184,7 @@ int gu_autotune(gu_graph_t g, const int32_t *sources, int32_t nsrc, int32_t kind
   tn.wg = best.wg_size;
   tn.alpha = best.alpha;
   tn.beta = best.beta;
+  tn.sources = k;
   if (out) {
     out->engine = tn.engine;
     out->variant = tn.variant >= 0 ? tn.variant : GU_VARIANT_AUTO;

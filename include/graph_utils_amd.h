@@ -232,9 +232,10 @@ typedef struct {
  * GU_QUERY_BC: Brandes) over engines, top-down expanders (shuffle, warp,
  * CTA+warp, scan, edge-parallel), sources per launch, workgroup sizes and
  * both direction-switch thresholds, starting from the degree-profile
- * defaults, and keeps the fastest in the handle: later calls whose
- * gu_config fields are left at auto / 0 use it (handles made afterwards by
- * gu_graph_share inherit it).  Allocates trial outputs for the duration. */
+ * defaults, and keeps the fastest in the handle: later calls of at least
+ * half that many sources whose gu_config fields are left at auto / 0 use it
+ * (handles made afterwards by gu_graph_share inherit it).  Allocates trial
+ * outputs for the duration. */
 int gu_autotune(gu_graph_t g, const int32_t *sources, int32_t nsrc,
                 int32_t kind, gu_tune_result *out);
 
