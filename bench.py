@@ -352,9 +352,9 @@ def bench_serve(args, gu, props):
     largest component) resident in HBM; a seeded stream of BFS / BC /
     diameter queries of 1-512 sources.  Measures the served form's saturated
     throughput (every query queued at t=0), then runs the stream with Poisson
-    arrivals at 70% of that rate through the server and through the plain
-    path (one query per call, serial), reporting queries/s and p50/p99
-    latency (scheduled arrival -> result ready) for each."""
+    arrivals at 70% of that rate through the server, and at 90% through the
+    server and through the plain path (one query per call, serial), reporting
+    queries/s and p50/p99 latency (scheduled arrival -> result ready)."""
     t0 = time.time()
     hgs = [gu.HostGraph.rmat(18, 16, 0.57, 0.19, 0.19, seed=18),
            gu.HostGraph.rmat(20, 16, 0.57, 0.19, 0.19, seed=20),
@@ -389,12 +389,17 @@ def bench_serve(args, gu, props):
         zero = np.zeros(nq)
         lat_s, mk_s, st_s = _served(gu, dgs, qs, zero, budget, args.slots, 256)
         sat_qps = nq / (mk_s / 1e3)
-        qps = args.qps if args.qps > 0 else 0.7 * sat_qps
-        gaps = np.random.default_rng(99).exponential(1e3 / qps, nq)
-        arr = np.cumsum(gaps) - gaps[0]
+        # the same seeded stream at two fixed offered loads: 70% of the
+        # saturated rate (latency) and 90% (the comparison with the plain
+        # path, which runs the identical arrivals)
+        qps = args.qps if args.qps > 0 else 0.9 * sat_qps
+        unit_gaps = np.random.default_rng(99).exponential(1.0, nq)
+        arr_of = lambda rate: (np.cumsum(unit_gaps) - unit_gaps[0]) * (1e3 / rate)
+        arr, arr70 = arr_of(qps), arr_of(0.7 * sat_qps)
         print("serve: saturated %.1f q/s; offered %.1f q/s" % (sat_qps, qps), file=sys.stderr, flush=True)
+        lat_70, mk_70, st_70 = _served(gu, dgs, qs, arr70, budget, args.slots, 256)
         lat_l, mk_l, st_l = _served(gu, dgs, qs, arr, budget, args.slots, 256)
-        print("serve: served run done", file=sys.stderr, flush=True)
+        print("serve: served runs done", file=sys.stderr, flush=True)
         lat_pd, mk_pd = _plain(gu, dgs, qs, arr, False)
         print("serve: plain (device results) done", file=sys.stderr, flush=True)
         lat_ph, mk_ph = _plain(gu, dgs, qs, arr, True)
@@ -408,6 +413,8 @@ def bench_serve(args, gu, props):
         "launch_ms": unit,
         "saturated": {"served_qps": round(sat_qps, 2), "p50_ms": _pct(lat_s, 50), "p99_ms": _pct(lat_s, 99),
                       "server": st_s},
+        "served_at_70pct": {"offered_qps": round(0.7 * sat_qps, 2), "qps": round(nq / (mk_70 / 1e3), 2),
+                            "p50_ms": _pct(lat_70, 50), "p99_ms": _pct(lat_70, 99), "server": st_70},
         "offered_qps": round(qps, 2),
         "served": {"qps": round(served_qps, 2), "p50_ms": _pct(lat_l, 50), "p99_ms": _pct(lat_l, 99),
                    "server": st_l},
@@ -433,8 +440,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="bfs", choices=["bfs", "grid", "bc", "lbs", "serve"])
     ap.add_argument("--queries", type=int, default=200, help="serve workload: queries in the stream")
-    ap.add_argument("--qps", type=float, default=0.0, help="serve workload: offered load (0 = 70%% of "
-                    "the measured saturated rate)")
+    ap.add_argument("--qps", type=float, default=0.0, help="serve workload: offered load of the "
+                    "comparison run (0 = 90%% of the measured saturated rate)")
     ap.add_argument("--slots", type=int, default=4, help="serve workload: launches in flight")
     ap.add_argument("--budget-gb", type=int, default=64, help="serve workload: HBM budget for outputs")
     ap.add_argument("--gpus", type=int, default=1)
