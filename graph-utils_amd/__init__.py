@@ -61,7 +61,7 @@ EXPORTS = (
     "gu_multi_search_rows", "gu_betweenness_rows", "gu_server_create", "gu_server_add_graph",
     "gu_server_submit", "gu_server_wait", "gu_server_next_done", "gu_server_poll",
     "gu_server_release", "gu_server_now_ms", "gu_server_get_stats", "gu_server_destroy",
-    "gu_autotune", "gu_parse_market", "gu_parse_snap", "gu_write_edgelist",
+    "gu_autotune", "gu_parse_market", "gu_parse_snap", "gu_write_edgelist", "gu_reverse",
 )
 
 QUERY_BFS, QUERY_BC, QUERY_DIAMETER = 0, 1, 2
@@ -222,6 +222,7 @@ def lib():
                                          _I32P, _I64P]),
         "gu_write_edgelist": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, _P, _P, ctypes.c_int32,
                                              ctypes.c_int64]),
+        "gu_reverse": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int64, _I32PP, _I32PP, _I32PP]),
         "gu_gen_er": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64,
                                      _I32PP, _I32PP, _I32PP, _I32P, _I64P]),
         "gu_gen_rmat": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
@@ -434,6 +435,15 @@ class HostGraph:
                "parse_snap(%s)" % path)
         g = cls._from_out(R, C, F, n, m)
         g.directed = True
+        return g
+
+    def reverse(self):
+        """reverse (graph-utils/reverse.cpp:3-44): the transposed graph."""
+        R, C, F = _I32P(), _I32P(), _I32P()
+        _check(lib().gu_reverse(_ptr(self.R), _ptr(self.C), self.n, self.m, ctypes.byref(R),
+                                ctypes.byref(C), ctypes.byref(F)), "reverse")
+        g = HostGraph(_take(R, self.n + 1), _take(C, self.m), _take(F, self.m), directed=self.directed,
+                      m=self.m)
         return g
 
     def write_edgelist(self, path, header=False):

@@ -304,6 +304,34 @@ int gu_parse_snap(const char *path, int32_t header, int32_t **R, int32_t **C, in
   return (*R && *C) ? GU_OK : GU_ERR_NOMEM;
 }
 
+// reverse (graph-utils/reverse.cpp:3-44): the transpose with its entries
+// ordered by (target, source), R induced from the targets.  A counting sort
+// by target over the entries in row order gives that order.
+int gu_reverse(const int32_t *R, const int32_t *C, int32_t n, int64_t m, int32_t **R2, int32_t **C2,
+               int32_t **F2) {
+  if (!R || (!C && m > 0) || n < 0 || m < 0 || !R2 || !C2) return GU_ERR_INVALID;
+  if (R[0] != 0 || (int64_t)R[n] != m) return GU_ERR_INVALID;
+  std::vector<int64_t> cnt((size_t)n + 1, 0);
+  for (int64_t k = 0; k < m; k++) {
+    if (C[k] < 0 || C[k] >= n) return GU_ERR_RANGE;
+    cnt[(size_t)C[k] + 1]++;
+  }
+  for (int32_t v = 0; v < n; v++) cnt[(size_t)v + 1] += cnt[(size_t)v];
+  std::vector<int32_t> Rv((size_t)n + 1), Cv((size_t)m), Fv((size_t)m);
+  for (int32_t v = 0; v <= n; v++) Rv[(size_t)v] = (int32_t)cnt[(size_t)v];
+  std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+  for (int32_t u = 0; u < n; u++)
+    for (int32_t k = R[u]; k < R[u + 1]; k++) {
+      const int64_t p = pos[(size_t)C[k]]++;
+      Cv[(size_t)p] = u;
+      Fv[(size_t)p] = C[k];
+    }
+  *R2 = dup_out(Rv);
+  *C2 = dup_out(Cv);
+  if (F2) *F2 = dup_out(Fv);
+  return (*R2 && *C2) ? GU_OK : GU_ERR_NOMEM;
+}
+
 // host_graph::write_edgelist_to_file (parse.cpp:436-456): optional "n m"
 // line, then "F[k] C[k]" per entry.
 int gu_write_edgelist(const char *path, int32_t header, const int32_t *F, const int32_t *C, int32_t n,

@@ -3,6 +3,8 @@
 
 #include <getopt.h>
 
+#include <algorithm>
+#include <climits>
 #include <cstdlib>
 #include <cstring>
 #include <iomanip>
@@ -299,6 +301,40 @@ std::vector<std::vector<int>> multi_search_CTA_warp_based_setup(const device_gra
 std::vector<std::vector<int>> multi_search_shuffle_based_setup(const device_graph &g, const program_options &,
                                                                int start, int end) {
   return run_distances(g, start, end, GU_VARIANT_SHUFFLE, "shuffle-based neighbor gathering");
+}
+
+host_graph reverse(host_graph &g) {
+  int32_t *R = nullptr, *C = nullptr, *F = nullptr;
+  check(gu_reverse(g.R.data(), g.C.data(), g.n, g.m, &R, &C, &F), "reverse");
+  return take_graph(R, C, F, g.n, g.m, g.directed);
+}
+
+std::vector<std::set<int>> get_reachability_dag(host_graph &g) {
+  if (!g.directed) throw gu_error(GU_ERR_INVALID, "reachability needs a directed graph");
+  std::vector<std::set<int>> reach((size_t)g.n);
+  if (g.n == 0) return reach;
+  gu_graph_t h = nullptr;
+  check(gu_graph_create(g.R.data(), g.C.data(), g.F.data(), g.n, g.m, 1, &h), "gu_graph_create");
+  const int chunk = std::min(g.n, 256);
+  int32_t *dist = nullptr;
+  int rc = gu_malloc((void **)&dist, sizeof(int32_t) * (size_t)chunk * g.n);
+  std::vector<int32_t> rows((size_t)chunk * g.n), src((size_t)chunk);
+  for (int b0 = 0; rc == GU_OK && b0 < g.n; b0 += chunk) {
+    const int nb = std::min(chunk, g.n - b0);
+    for (int i = 0; i < nb; i++) src[(size_t)i] = b0 + i;
+    rc = gu_multi_search(h, src.data(), nb, dist, nullptr, nullptr, nullptr);
+    if (rc == GU_OK) rc = gu_memcpy_d2h(rows.data(), dist, sizeof(int32_t) * (size_t)nb * g.n);
+    for (int i = 0; rc == GU_OK && i < nb; i++) {
+      std::set<int> &s = reach[(size_t)(b0 + i)];
+      const int32_t *d = rows.data() + (size_t)i * g.n;
+      for (int v = 0; v < g.n; v++)
+        if (d[v] != INT_MAX && d[v] > 0) s.insert(s.end(), v);
+    }
+  }
+  if (dist) gu_free(dist);
+  gu_graph_destroy(h);
+  check(rc, "get_reachability_dag");
+  return reach;
 }
 
 // transitive_closure_setup (transitive_closure.cu:4-37): the reachability

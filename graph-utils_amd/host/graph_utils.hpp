@@ -17,6 +17,7 @@
 //   transitive_closure_setup              transitive_closure.cuh:13
 //   betweenness_centrality_setup          betweenness_centrality.cuh:14
 //   diameter_sampling_setup               diameter_sampling.cuh:13
+//   reverse, get_reachability_dag         graph-utils/reverse.h, dag_reachability.h
 //
 // Differences in error behaviour: where the reference prints and calls exit()
 // (checkCudaErrors, parse errors) these functions throw gu_error carrying the
@@ -26,6 +27,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <cstdint>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -131,6 +133,16 @@ void transitive_closure_setup(const device_graph &g, int start, int end, std::ve
 void betweenness_centrality_setup(const device_graph &g, int start, int end,
                                   std::vector<std::vector<float>> &delta_h);
 int diameter_sampling_setup(const device_graph &g, int start, int end);
+
+// reverse (graph-utils/reverse.h): the transposed graph, entries ordered by
+// (target, source).
+host_graph reverse(host_graph &g);
+// get_reachability_dag (graph-utils/dag_reachability.h): for every vertex of
+// a directed acyclic graph, the set of vertices reachable from it by at
+// least one edge.  Computed with multi-source searches on the GPU (256
+// sources per launch); throws gu_error for an undirected graph, as the
+// reference exits.
+std::vector<std::set<int>> get_reachability_dag(host_graph &g);
 
 // Rows of a [start,end) result and the source each row holds.
 std::vector<int> result_row_sources(int start, int end, int grid);

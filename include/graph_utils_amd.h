@@ -347,6 +347,10 @@ int gu_parse_market(const char *path, int32_t **R, int32_t **C, int32_t **F,
  * the reference (a header line is read as an entry). */
 int gu_parse_snap(const char *path, int32_t header, int32_t **R, int32_t **C,
                   int32_t **F, int32_t *n, int64_t *m);
+/* reverse (graph-utils/reverse.cpp:3-44): the transposed CSR, entries
+ * ordered by (target, source); F2 = rows of the result. */
+int gu_reverse(const int32_t *R, const int32_t *C, int32_t n, int64_t m,
+               int32_t **R2, int32_t **C2, int32_t **F2);
 /* host_graph::write_edgelist_to_file (parse.cpp:436-456). */
 int gu_write_edgelist(const char *path, int32_t header, const int32_t *F,
                       const int32_t *C, int32_t n, int64_t m);

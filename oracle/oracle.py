@@ -229,3 +229,34 @@ def ref_parse_snap(path, header=False):
                      ctypes.byref(R), ctypes.byref(C), ctypes.byref(F), ctypes.byref(dr))
     return (_take_ref(L, R, n.value + 1), _take_ref(L, C, m.value), _take_ref(L, F, m.value),
             n.value, m.value, bool(dr.value))
+
+
+def ref_reverse(R, C, F):
+    """The reference's reverse (graph-utils/reverse.cpp:3-44)."""
+    L, I32P = _ref_lib()
+    P = ctypes.c_void_p
+    L.ref_reverse.argtypes = [P, P, P, ctypes.c_int32, ctypes.c_int64, ctypes.POINTER(I32P),
+                              ctypes.POINTER(I32P), ctypes.POINTER(I32P)]
+    R, C, F = (np.ascontiguousarray(x, dtype=np.int32) for x in (R, C, F))
+    n, m = len(R) - 1, len(C)
+    R2, C2, F2 = I32P(), I32P(), I32P()
+    L.ref_reverse(_p(R), _p(C), _p(F), n, m, ctypes.byref(R2), ctypes.byref(C2), ctypes.byref(F2))
+    return _take_ref(L, R2, n + 1), _take_ref(L, C2, m), _take_ref(L, F2, m)
+
+
+def ref_reach_dag(R, C, F):
+    """The reference's get_reachability_dag (graph-utils/dag_reachability.cpp:31-65)
+    on a directed acyclic graph: list of sorted member arrays."""
+    L, I32P = _ref_lib()
+    P = ctypes.c_void_p
+    I64P = ctypes.POINTER(ctypes.c_int64)
+    L.ref_reach_dag.argtypes = [P, P, P, ctypes.c_int32, ctypes.c_int64, ctypes.POINTER(I64P),
+                                ctypes.POINTER(I32P)]
+    R, C, F = (np.ascontiguousarray(x, dtype=np.int32) for x in (R, C, F))
+    n, m = len(R) - 1, len(C)
+    off, mem = I64P(), I32P()
+    L.ref_reach_dag(_p(R), _p(C), _p(F), n, m, ctypes.byref(off), ctypes.byref(mem))
+    o = np.ctypeslib.as_array(off, shape=(n + 1,)).copy()
+    L.ref_free(ctypes.cast(off, ctypes.c_void_p))
+    members = _take_ref(L, mem, int(o[-1]))
+    return [members[o[i]:o[i + 1]] for i in range(n)]

@@ -25,3 +25,15 @@ def test_dropin_driver(gpu, name):
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
     assert "Test failed" not in r.stdout
     assert r.stdout.count("Test passed") >= 9
+
+
+def test_dag_reachability_dropin(gpu):
+    """get_reachability_dag / reverse (graph-utils/dag_reachability.h,
+    reverse.h) on random DAGs, against the oracle's BFS."""
+    exe = os.path.join(ROOT, "tests", "cpp", "_build", "reach_test")
+    if not os.path.exists(exe):
+        pytest.fail("tests/cpp/_build/reach_test is not built (run __graft_entry__.build())")
+    announce("reach_test")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:])
+    assert r.returncode == 0 and "Test passed" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]

@@ -400,3 +400,51 @@ def test_edgelist_round_trip(gu, orc, tmp_path, header):
     if orc.ref_parse_available():
         R, C, F, n, m, _ = orc.ref_parse_snap(str(p))
         assert np.array_equal(g.R, R) and np.array_equal(g.C, C)
+
+
+# --------------------------------------------------------------------------
+# reverse and DAG reachability (graph-utils/reverse.cpp, dag_reachability.cpp)
+# --------------------------------------------------------------------------
+def _random_dag(n, m, seed):
+    rng = np.random.default_rng(seed)
+    u = rng.integers(0, n, m)
+    v = rng.integers(0, n, m)
+    keep = u != v
+    a, b = np.minimum(u, v)[keep], np.maximum(u, v)[keep]
+    perm = rng.permutation(n)          # hide the topological order in the labels
+    a, b = perm[a], perm[b]
+    order = np.lexsort((b, a))
+    a, b = a[order], b[order]
+    R = np.zeros(n + 1, dtype=np.int32)
+    np.add.at(R, a + 1, 1)
+    R = np.cumsum(R).astype(np.int32)
+    return R, b.astype(np.int32), a.astype(np.int32)
+
+
+@pytest.mark.parametrize("case", ["jazz", "test3", "dag"])
+def test_reverse_matches_reference(gu, orc, case):
+    _ref_or_skip(orc)
+    if case == "dag":
+        R, C, F = _random_dag(300, 1500, 3)
+        hg = gu.HostGraph(R, C, F, directed=True)
+    else:
+        hg = gu.HostGraph.parse_metis(graph_path(case))
+    r = hg.reverse()
+    R2, C2, F2 = orc.ref_reverse(hg.R, hg.C, hg.F)
+    assert np.array_equal(r.R, R2) and np.array_equal(r.C, C2) and np.array_equal(r.F, F2)
+    rr = r.reverse()                  # reversing twice gives rows in ascending order
+    assert np.array_equal(rr.R, hg.R)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_oracle_reachability_matches_reference_dag(orc, seed):
+    """The oracle's BFS reachability (finite distance, not the source) equals
+    the reference's recursive DAG reachability: the check the GPU form of
+    get_reachability_dag is held to."""
+    _ref_or_skip(orc)
+    R, C, F = _random_dag(200, 700, seed)
+    ref = orc.ref_reach_dag(R, C, F)
+    for s in range(200):
+        d = orc.bfs(R, C, s)
+        got = np.nonzero((d != orc.INT_MAX) & (d > 0))[0]
+        assert np.array_equal(got, ref[s]), s
