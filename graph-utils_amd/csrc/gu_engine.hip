@@ -780,6 +780,15 @@ namespace {
 
 thread_local char g_last_err[256];
 
+// Two events destroyed with the scope, so early error returns leak nothing.
+struct EventPair {
+  hipEvent_t a = nullptr, b = nullptr;
+  ~EventPair() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+};
+
 int fail(hipError_t e) {
   std::snprintf(g_last_err, sizeof(g_last_err), "%s", hipGetErrorString(e));
   return e == hipErrorOutOfMemory ? GU_ERR_NOMEM : GU_ERR_HIP;
@@ -1552,10 +1561,10 @@ int run_job(gu_graph *g, const int32_t *sources, int32_t nsrc, Job job, const Ho
   const bool want_sigma = job == JOB_BC || (job == JOB_BFS && hr.sigma != nullptr);
   hipStream_t st = (hipStream_t)cfg.stream;
   GU_HIP(hipSetDevice(g->device));
-  hipEvent_t e0, e1;
-  GU_HIP(hipEventCreate(&e0));
-  GU_HIP(hipEventCreate(&e1));
-  GU_HIP(hipEventRecord(e0, st));
+  EventPair ev;
+  GU_HIP(hipEventCreate(&ev.a));
+  GU_HIP(hipEventCreate(&ev.b));
+  GU_HIP(hipEventRecord(ev.a, st));
   gu_stats s{};
   s.variant_used = variant;
   s.batch_width_used = width;
@@ -1599,12 +1608,10 @@ int run_job(gu_graph *g, const int32_t *sources, int32_t nsrc, Job job, const Ho
     s.dag_edges += (int64_t)info.dag;
     dmax = std::max(dmax, info.last_level);
   }
-  GU_HIP(hipEventRecord(e1, st));
-  GU_HIP(hipEventSynchronize(e1));
+  GU_HIP(hipEventRecord(ev.b, st));
+  GU_HIP(hipEventSynchronize(ev.b));
   float ms = 0;
-  GU_HIP(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  GU_HIP(hipEventElapsedTime(&ms, ev.a, ev.b));
   s.device_ms = ms;
   s.max_level = dmax;
   if (diam) *diam = dmax;
@@ -1694,12 +1701,19 @@ struct gu_timer_s {
 
 int gu_timer_start(void *stream, void **timer) {
   if (!timer) return GU_ERR_INVALID;
-  gu_timer_s *t = new (std::nothrow) gu_timer_s;
+  *timer = nullptr;
+  gu_timer_s *t = new (std::nothrow) gu_timer_s{};
   if (!t) return GU_ERR_NOMEM;
   t->s = (hipStream_t)stream;
-  GU_HIP(hipEventCreate(&t->a));
-  GU_HIP(hipEventCreate(&t->b));
-  GU_HIP(hipEventRecord(t->a, t->s));
+  hipError_t e = hipEventCreate(&t->a);
+  if (e == hipSuccess) e = hipEventCreate(&t->b);
+  if (e == hipSuccess) e = hipEventRecord(t->a, t->s);
+  if (e != hipSuccess) {
+    if (t->a) (void)hipEventDestroy(t->a);
+    if (t->b) (void)hipEventDestroy(t->b);
+    delete t;
+    return fail(e);
+  }
   *timer = t;
   return GU_OK;
 }
@@ -1707,14 +1721,15 @@ int gu_timer_start(void *stream, void **timer) {
 int gu_timer_stop(void *timer, double *ms) {
   if (!timer) return GU_ERR_INVALID;
   gu_timer_s *t = (gu_timer_s *)timer;
-  GU_HIP(hipEventRecord(t->b, t->s));
-  GU_HIP(hipEventSynchronize(t->b));
   float f = 0;
-  GU_HIP(hipEventElapsedTime(&f, t->a, t->b));
-  if (ms) *ms = f;
+  hipError_t e = hipEventRecord(t->b, t->s);
+  if (e == hipSuccess) e = hipEventSynchronize(t->b);
+  if (e == hipSuccess) e = hipEventElapsedTime(&f, t->a, t->b);
   (void)hipEventDestroy(t->a);
   (void)hipEventDestroy(t->b);
-  delete t;
+  delete t;  // the timer is consumed either way
+  if (e != hipSuccess) return fail(e);
+  if (ms) *ms = f;
   return GU_OK;
 }
 
@@ -2216,30 +2231,32 @@ int gu_lbs(const int32_t *counts, int64_t S, int32_t *scan, int32_t *result, int
   const int T = 256;
   const long long TS = (long long)T * kScanIPT;
   const long long nt = (S + TS - 1) / TS;
-  long long *tsum = nullptr;
-  GU_HIP(hipMalloc((void **)&tsum, sizeof(long long) * (size_t)(nt + 2)));
-  long long *total_d = tsum + nt + 1;
-  hipEvent_t e0, e1;
-  GU_HIP(hipEventCreate(&e0));
-  GU_HIP(hipEventCreate(&e1));
-  GU_HIP(hipEventRecord(e0, st));
+  struct Tile {
+    long long *p = nullptr;
+    ~Tile() {
+      if (p) (void)hipFree(p);
+    }
+  } tsum;
+  GU_HIP(hipMalloc((void **)&tsum.p, sizeof(long long) * (size_t)(nt + 2)));
+  long long *total_d = tsum.p + nt + 1;
+  EventPair ev;
+  GU_HIP(hipEventCreate(&ev.a));
+  GU_HIP(hipEventCreate(&ev.b));
+  GU_HIP(hipEventRecord(ev.a, st));
   const int grid = std::max(1, cus * 8);
-  k_cscan_reduce<<<grid, T, 0, st>>>(counts, S, tsum);
-  k_cscan_tiles<<<1, T, 0, st>>>(tsum, nt, total_d);
-  k_cscan_apply<<<grid, T, 0, st>>>(counts, S, tsum, scan, total_d);
+  k_cscan_reduce<<<grid, T, 0, st>>>(counts, S, tsum.p);
+  k_cscan_tiles<<<1, T, 0, st>>>(tsum.p, nt, total_d);
+  k_cscan_apply<<<grid, T, 0, st>>>(counts, S, tsum.p, scan, total_d);
   if (result) k_lbs_expand<<<grid, 256, 0, st>>>(scan, S, total_d, result_cap, result);
   GU_HIP(hipGetLastError());
-  GU_HIP(hipEventRecord(e1, st));
+  GU_HIP(hipEventRecord(ev.b, st));
   long long W = 0;
   GU_HIP(hipMemcpyAsync(&W, total_d, sizeof(long long), hipMemcpyDeviceToHost, st));
   GU_HIP(hipStreamSynchronize(st));
   int rc = GU_OK;
-  GU_HIP(hipEventSynchronize(e1));
+  GU_HIP(hipEventSynchronize(ev.b));
   float ms = 0;
-  GU_HIP(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipFree(tsum);
+  GU_HIP(hipEventElapsedTime(&ms, ev.a, ev.b));
   if (W >= INT_MAX || (W > 0 && (!result || W > result_cap))) rc = GU_ERR_CAPACITY;
   if (W_out) *W_out = W;
   if (stats) {
