@@ -414,6 +414,17 @@ __global__ __launch_bounds__(256) void k_lbs_expand(const int *scan, long long S
 // caller's source-major rows ([nb][n]); lanes that did not reach v get
 // `fill`.  A workgroup moves 64 vertices x 64 lanes through LDS; every
 // thread keeps 16 loads in flight.
+// Output transposes stream: rows are read and written once (non-temporal
+// loads and stores measured 0.7% faster per pass than cached ones).
+#ifndef GU_TR_NT
+#define GU_TR_NT 1
+#endif
+#ifndef GU_TR_NTL
+#define GU_TR_NTL 1
+#endif
+#ifndef GU_TR_GRID
+#define GU_TR_GRID 8
+#endif
 template <int NW, class T>
 __global__ __launch_bounds__(256) void k_transpose_out(Eng<NW> e, const T *src, T *const *rows, T fill) {
   __shared__ T tile[64][65];
@@ -429,7 +440,11 @@ __global__ __launch_bounds__(256) void k_transpose_out(Eng<NW> e, const T *src, 
     for (int k = 0; k < 16; k++) {
       const long long v = v0 + row0 + 4 * k;
       bool ok = v < e.n && ((e.visit[(size_t)v * NW + j] >> col) & 1ull);
+#if GU_TR_NTL
+      x[k] = ok ? __builtin_nontemporal_load(&src[(size_t)v * (kWL * NW) + j * kWL + col]) : fill;
+#else
       x[k] = ok ? src[(size_t)v * (kWL * NW) + j * kWL + col] : fill;
+#endif
     }
 #pragma unroll
     for (int k = 0; k < 16; k++) tile[row0 + 4 * k][col] = x[k];
@@ -440,7 +455,11 @@ __global__ __launch_bounds__(256) void k_transpose_out(Eng<NW> e, const T *src, 
       const long long v = v0 + col;
       if (b < e.nb && v < e.n) {
         T *r = rows[b];
+#if GU_TR_NT
+        if (r) __builtin_nontemporal_store(tile[col][row0 + 4 * k], &r[v]);
+#else
         if (r) r[v] = tile[col][row0 + 4 * k];
+#endif
       }
     }
     __syncthreads();
@@ -1268,7 +1287,7 @@ int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const g
   if (job == JOB_BFS && (out.dist || out.sigma))
   {
     const long long tiles = ((long long)g->n + 63) / 64 * ((nb + kWL - 1) / kWL);
-    const int og = (int)std::max(1LL, std::min<long long>(tiles, pick_grid(g, 8)));
+    const int og = (int)std::max(1LL, std::min<long long>(tiles, pick_grid(g, GU_TR_GRID)));
     if (out.dist) k_transpose_out<NW, int><<<og, 256, 0, st>>>(e, w.dvm, out.dist, INT_MAX);
     if (out.sigma) k_transpose_out<NW, u64><<<og, 256, 0, st>>>(e, w.svm, out.sigma, 0ull);
   }
