@@ -448,3 +448,23 @@ def test_oracle_reachability_matches_reference_dag(orc, seed):
         d = orc.bfs(R, C, s)
         got = np.nonzero((d != orc.INT_MAX) & (d > 0))[0]
         assert np.array_equal(got, ref[s]), s
+
+
+def test_missing_library_fails_loudly():
+    """Without the HIP library the package raises instead of computing
+    anything another way (no CPU fallback exists)."""
+    code = (
+        "import sys; sys.path.insert(0, %r)\n"
+        "import __graft_entry__\n"
+        "pkg = __graft_entry__.load_package()\n"
+        "pkg.LIB_PATH = '/nonexistent/libgraph_utils_amd.so'\n"
+        "pkg._lib = None\n"
+        "try:\n"
+        "    pkg.HostGraph.er(10, 5)\n"
+        "except ImportError as e:\n"
+        "    print('ImportError:', e)\n"
+        "    sys.exit(0)\n"
+        "sys.exit(1)\n" % ROOT)
+    r = subprocess.run(["python3", "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "not built" in r.stdout
