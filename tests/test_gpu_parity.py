@@ -512,3 +512,49 @@ def test_level_graph_replay_matches_direct(gu, orc, gpu, graphs):
             x1, b1, _ = dg.betweenness(src[:40], config=gu.make_config(level_graphs=graphs))
             x2, b2, _ = dg.betweenness(src[:40], config=gu.make_config(level_graphs=0))
             assert np.array_equal(x1, x2) and np.array_equal(b1, b2)
+
+
+# ---- randomized sweep: random graphs x random launch shapes ---------------
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_graphs_and_shapes(gu, orc, gpu, seed):
+    """Seeded random graphs (ER, R-MAT, grid, directed) and random source
+    sets, engines, expanders, widths, workgroup sizes and direction rules:
+    distances and path counts bit-exact, BC within 1e-5 of FP64."""
+    rng = np.random.default_rng(1000 + seed)
+    kind = ["er", "rmat", "grid", "directed"][seed % 4]
+    if kind == "er":
+        hg = gu.HostGraph.er(int(rng.integers(50, 600)), int(rng.integers(100, 3000)), seed=seed)
+    elif kind == "rmat":
+        hg = gu.HostGraph.rmat(int(rng.integers(8, 12)), int(rng.integers(4, 17)), seed=seed)
+    elif kind == "grid":
+        hg = gu.HostGraph.grid(int(rng.integers(8, 40)), float(rng.uniform(0, 0.3)), seed=seed)
+    else:
+        hg = _directed_graph(gu, n=int(rng.integers(200, 1500)), arcs=int(rng.integers(1000, 8000)),
+                             seed=seed)
+    k = int(rng.integers(1, min(300, hg.n) + 1))
+    src = rng.integers(0, hg.n, k).astype(np.int32)
+    variant = ["linear", "edge_parallel", "warp", "scan", "cta_warp", "shuffle", "auto"][int(rng.integers(7))]
+    cfg = gu.make_config(variant, direction=int(rng.integers(3)),
+                         batch_width=[0, 64, 128, 256][int(rng.integers(4))],
+                         wg_size=[0, 256, 512, 1024][int(rng.integers(4))],
+                         alpha=[0, 2, 16, 256][int(rng.integers(4))],
+                         beta=[0, 8, 64][int(rng.integers(3))],
+                         engine=int(rng.integers(3)))
+    announce("random %s seed=%d n=%d k=%d variant=%s engine=%d" % (kind, seed, hg.n, k, variant, cfg.engine))
+    with gu.DeviceGraph(hg) as dg:
+        d, s, st = dg.multi_search(src, True, True, cfg)
+        sub = src[:16]
+        od, os_ = _oracle_rows(orc, hg, sub)
+        assert np.array_equal(d[:len(sub)], od)
+        assert np.array_equal(s[:len(sub)], os_)
+        if not hg.directed:
+            bsub = src[:24]
+            _, bc, _ = dg.betweenness(bsub, delta=False, config=cfg)
+            ref = orc.bc_f64(hg.R, hg.C, bsub)
+            nz = ref > 0
+            assert np.all(bc[~nz] == 0)
+            if nz.any():
+                assert np.max(np.abs(bc[nz] - ref[nz]) / ref[nz]) <= BC_RTOL
+        diam, _ = dg.diameter(src[:32], cfg)
+        assert diam == max(orc.eccentricity(hg.R, hg.C, int(x)) for x in src[:32])
