@@ -551,8 +551,10 @@ __device__ __forceinline__ void neumaier(float &sum, float &comp, float x) {
 }
 
 // Accumulate the dependency terms of v's neighbours in [r0, r1) for lanes mk
-// (sv = float(sigma_b(v)) per lane).  Up to four successor rows are loaded
-// before any is used.
+// (sv = float(sigma_b(v)) per lane).  Up to D successor rows are loaded
+// before any is used (16 for one mask word, 4 for more: measured best on
+// R-MAT s21 at widths 64 and 128); the terms are still added in adjacency
+// order.
 template <int NW>
 __device__ __forceinline__ void bw_accum(const Eng<NW> &e, const Bw &b, int L, const u64 *mk,
                                          const float *sv, int r0, int r1, float *sum, float *comp) {
@@ -574,9 +576,10 @@ __device__ __forceinline__ void bw_accum(const Eng<NW> &e, const Bw &b, int L, c
     }
     uint64_t pm = ballot(has);
     while (pm) {
-      int k[4];
+      constexpr int D = NW == 1 ? 16 : 4;
+      int k[D];
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
+      for (int u = 0; u < D; u++) {
         if (pm) {
           k[u] = __ffsll((unsigned long long)pm) - 1;
           pm &= pm - 1;
@@ -584,10 +587,10 @@ __device__ __forceinline__ void bw_accum(const Eng<NW> &e, const Bw &b, int L, c
           k[u] = -1;
         }
       }
-      float sw[4][NW], dw[4][NW];
-      bool use[4][NW];
+      float sw[D][NW], dw[D][NW];
+      bool use[D][NW];
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
+      for (int u = 0; u < D; u++) {
         if (k[u] >= 0) {
           const int wk = readlane(w, k[u]);
           const size_t rowb = (size_t)wk * (kWL * NW);
@@ -604,7 +607,7 @@ __device__ __forceinline__ void bw_accum(const Eng<NW> &e, const Bw &b, int L, c
         }
       }
 #pragma unroll
-      for (int u = 0; u < 4; u++)
+      for (int u = 0; u < D; u++)
 #pragma unroll
         for (int j = 0; j < NW; j++)
           if (use[u][j]) neumaier(sum[j], comp[j], (sv[j] / sw[u][j]) * (1.0f + dw[u][j]));
