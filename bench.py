@@ -4,7 +4,9 @@ Default workload (the headline): batched multi-source BFS (distances +
 uint64 path counts) on R-MAT scale 20, edge factor 16, 64 sources per batch.
 A step is one pass of the search over one batch of 64 non-isolated sources
 with every input resident in HBM and both outputs ([64 x n] int32 distances,
-[64 x n] uint64 path counts) written to HBM.  Prints one JSON line.
+[64 x n] uint64 path counts) written to HBM.  Prints one JSON line; its
+"serving" object holds the served queries/s and p50/p99 latency of the mixed
+query stream (the serve workload below, --no-serve skips it).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--variant auto]
 
@@ -347,32 +349,26 @@ def _plain(gu, dgs, qs, arrivals_ms, to_host):
     return lat, makespan
 
 
-def bench_serve(args, gu, props):
-    """Serving: R-MAT s18 ef16, R-MAT s20 ef16 and a 1024^2 grid (-10% edges,
-    largest component) resident in HBM; a seeded stream of BFS / BC /
-    diameter queries of 1-512 sources.  Measures the served form's saturated
-    throughput (every query queued at t=0), then runs the stream with Poisson
-    arrivals at 70% of that rate through the server, and at 90% through the
-    server and through the plain path (one query per call, serial), reporting
-    queries/s and p50/p99 latency (scheduled arrival -> result ready)."""
-    t0 = time.time()
-    hgs = [gu.HostGraph.rmat(18, 16, 0.57, 0.19, 0.19, seed=18),
-           gu.HostGraph.rmat(20, 16, 0.57, 0.19, 0.19, seed=20),
-           gu.HostGraph.grid(1024, 0.10, seed=10)]
-    gen_s = time.time() - t0
+def serve_stream(args, gu, hgs, tune, plain_host=True):
+    """Runs the seeded serving stream over the resident graphs `hgs`:
+    saturated throughput (every query queued at t=0), then Poisson arrivals
+    at 70% and 90% of that rate through the server, and the 90% arrivals
+    through the plain path (one query per call, serial).  Returns the result
+    dict and the 90% run's makespan in ms."""
     nq = args.queries
     qs = _serve_queries(gu, hgs, nq, seed=4242)
     budget = args.budget_gb << 30
+    names = ("rmat18", "rmat20", "grid1024")
     dgs = [gu.DeviceGraph(h) for h in hgs]
     unit, tuned = {}, {}
     try:
-        if not args.no_tune:
-            for name, h, dg in zip(("rmat18", "rmat20", "grid1024"), hgs, dgs):
+        if tune:
+            for name, h, dg in zip(names, hgs, dgs):
                 smp = h.sample_sources(256, seed=17)
                 tuned[name] = {"bfs": dg.autotune(smp, gu.QUERY_BFS), "bc": dg.autotune(smp, gu.QUERY_BC)}
             print("serve: tuned %s" % json.dumps(tuned), file=sys.stderr, flush=True)
         # device time of one launch per graph and kind, 1 and 256 sources
-        for name, h, dg in zip(("rmat18", "rmat20", "grid1024"), hgs, dgs):
+        for name, h, dg in zip(names, hgs, dgs):
             src = h.sample_sources(256, seed=7)
             db, sb, bb = gu.DeviceBuffer(4 * 256 * h.n), gu.DeviceBuffer(8 * 256 * h.n), gu.DeviceBuffer(8 * h.n)
             row = {}
@@ -402,13 +398,15 @@ def bench_serve(args, gu, props):
         print("serve: served runs done", file=sys.stderr, flush=True)
         lat_pd, mk_pd = _plain(gu, dgs, qs, arr, False)
         print("serve: plain (device results) done", file=sys.stderr, flush=True)
-        lat_ph, mk_ph = _plain(gu, dgs, qs, arr, True)
-        print("serve: plain (host results) done", file=sys.stderr, flush=True)
+        if plain_host:
+            lat_ph, mk_ph = _plain(gu, dgs, qs, arr, True)
+            print("serve: plain (host results) done", file=sys.stderr, flush=True)
     finally:
         for dg in dgs:
             dg.close()
     served_qps = nq / (mk_l / 1e3)
     res = {
+        "queries": nq,
         "autotuned": tuned,
         "launch_ms": unit,
         "saturated": {"served_qps": round(sat_qps, 2), "p50_ms": _pct(lat_s, 50), "p99_ms": _pct(lat_s, 99),
@@ -420,16 +418,42 @@ def bench_serve(args, gu, props):
                    "server": st_l},
         "plain_device_results": {"qps": round(nq / (mk_pd / 1e3), 2), "p50_ms": _pct(lat_pd, 50),
                                  "p99_ms": _pct(lat_pd, 99)},
-        "plain_host_results": {"qps": round(nq / (mk_ph / 1e3), 2), "p50_ms": _pct(lat_ph, 50),
-                               "p99_ms": _pct(lat_ph, 99)},
     }
     res["qps_ratio_vs_plain_device"] = round(served_qps / res["plain_device_results"]["qps"], 2)
-    res["qps_ratio_vs_plain_host"] = round(served_qps / res["plain_host_results"]["qps"], 2)
-    _line("served queries/s, mixed BFS/BC/diameter stream over 3 resident graphs", round(served_qps, 2),
+    if plain_host:
+        res["plain_host_results"] = {"qps": round(nq / (mk_ph / 1e3), 2), "p50_ms": _pct(lat_ph, 50),
+                                     "p99_ms": _pct(lat_ph, 99)}
+        res["qps_ratio_vs_plain_host"] = round(served_qps / res["plain_host_results"]["qps"], 2)
+    return res, mk_l
+
+
+def serve_graphs(gu, rmat20=None):
+    """The three resident graphs of the serving stream: R-MAT s18 ef16,
+    R-MAT s20 ef16 (the same seeded graph as the BFS workload) and a 1024^2
+    grid with 10% of its edges deleted, largest component."""
+    return [gu.HostGraph.rmat(18, 16, 0.57, 0.19, 0.19, seed=18),
+            rmat20 if rmat20 is not None else gu.HostGraph.rmat(20, 16, 0.57, 0.19, 0.19, seed=20),
+            gu.HostGraph.grid(1024, 0.10, seed=10)]
+
+
+SERVE_WORKLOAD = ("R-MAT s18 ef16 + R-MAT s20 ef16 + 1024^2 grid -10%% LCC resident; %d queries, "
+                  "70%% BFS (d + sigma) / 20%% BC / 10%% diameter, 1-512 sources log-uniform, "
+                  "Poisson arrivals")
+
+
+def bench_serve(args, gu, props):
+    """Serving: R-MAT s18 ef16, R-MAT s20 ef16 and a 1024^2 grid (-10% edges,
+    largest component) resident in HBM; a seeded stream of BFS / BC /
+    diameter queries of 1-512 sources, reporting queries/s and p50/p99
+    latency (scheduled arrival -> result ready) served and plain."""
+    t0 = time.time()
+    hgs = serve_graphs(gu)
+    gen_s = time.time() - t0
+    res, mk_l = serve_stream(args, gu, hgs, not args.no_tune)
+    nq = args.queries
+    _line("served queries/s, mixed BFS/BC/diameter stream over 3 resident graphs", res["served"]["qps"],
           "queries/s", args, mk_l / nq, "int32+u64 / f32+f64 BC",
-          {"workload": "R-MAT s18 ef16 + R-MAT s20 ef16 + 1024^2 grid -10%% LCC resident; %d queries, "
-                       "70%% BFS (d + sigma) / 20%% BC / 10%% diameter, 1-512 sources log-uniform, "
-                       "Poisson arrivals" % nq,
+          {"workload": SERVE_WORKLOAD % nq,
            "hbm_budget_gb": args.budget_gb, "slots": args.slots, "max_pack": 256,
            "generator_seconds": round(gen_s, 2), "device": props["name"], "arch": props["arch"],
            "graphs": [{"n": h.n, "m_directed": h.m} for h in hgs]},
@@ -457,6 +481,8 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-variants", action="store_true", help="skip the per-variant sweep")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="bfs workload: skip the serving stream reported beside the GTEPS figure")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (degree-profile defaults)")
     ap.add_argument("--level-graphs", type=int, default=0,
                     help="batched engine: 1 replays captured chunks of levels as HIP graphs, 0 launches "
@@ -582,6 +608,16 @@ def main():
         "variants_gteps": variants,
         "pcie_inclusive_gteps": round(e2e_gteps, 2),
     }
+    if not args.no_serve:
+        # the metric's second half: the mixed query stream over three resident
+        # graphs (the R-MAT s20 above is one of them), served and plain
+        t0 = time.time()
+        hgs = serve_graphs(gu, hg)
+        res, _ = serve_stream(args, gu, hgs, not args.no_tune)
+        res["workload"] = SERVE_WORKLOAD % args.queries
+        res["hbm_budget_gb"], res["slots"], res["max_pack"] = args.budget_gb, args.slots, 256
+        res["seconds"] = round(time.time() - t0, 1)
+        out["serving"] = res
     if not args.no_cpu:
         from oracle import oracle as orc
 

@@ -118,6 +118,10 @@ __global__ __launch_bounds__(1024) void k_expand(Eng<NW> e) {
   Pusher pb{wave_pushbuf<NW>(), 0};
   Acc a;
   if (mode != 0) {
+    // bottom-up hub segments read only level-L rows and write only hub
+    // partials, so they run in this launch, ahead of the vertex blocks (one
+    // launch fewer per level: 3% per pass on R-MAT s20)
+    hub_segments<NW, SIGMA>(e, L, mode, wave_group<NW>(), a);
     bu_level<NW, SIGMA>(e, L, pb, wave_group<NW>(), a);
   } else {
     if (VAR == V_SHUFFLE) td_shuffle<NW>(e, L, pb);
@@ -137,8 +141,15 @@ __global__ __launch_bounds__(1024) void k_finalize(Eng<NW> e) {
   const int L = e.ctrl->level;
   const int mode = e.ctrl->mode;
   Acc a;
-  if (mode == 0) td_finalize<NW, SIGMA>(e, L, wave_group<NW>(), a);
-  hub_segments<NW, SIGMA>(e, L, mode, wave_group<NW>(), a);
+  if (mode == 0) {  // bottom-up levels ran their hub segments in k_expand
+    td_finalize<NW, SIGMA>(e, L, wave_group<NW>(), a);
+    // top-down hub segments only carry path counts (the expansion already
+    // pushed the hubs' new lanes); at level 0 the sources push them instead
+    if (SIGMA && L == 0)
+      hub_push_sources<NW>(e, a);
+    else if (SIGMA)
+      hub_segments<NW, SIGMA>(e, L, mode, wave_group<NW>(), a);
+  }
   write_acct<NW>(e, 1, a);
 }
 
@@ -415,7 +426,8 @@ __global__ __launch_bounds__(256) void k_lbs_expand(const int *scan, long long S
 // `fill`.  A workgroup moves 64 vertices x 64 lanes through LDS; every
 // thread keeps 16 loads in flight.
 // Output transposes stream: rows are read and written once (non-temporal
-// loads and stores measured 0.7% faster per pass than cached ones).
+// loads and stores measured 0.7% faster per pass than cached ones; issuing
+// the next tile's loads before storing the current one gained nothing).
 #ifndef GU_TR_NT
 #define GU_TR_NT 1
 #endif
@@ -425,6 +437,22 @@ __global__ __launch_bounds__(256) void k_lbs_expand(const int *scan, long long S
 #ifndef GU_TR_GRID
 #define GU_TR_GRID 8
 #endif
+template <int NW, class T>
+__device__ __forceinline__ void tr_load(const Eng<NW> &e, const T *src, T fill, long long t, int words, int row0,
+                                        int col, T (&x)[16]) {
+  const long long v0 = (t / words) * 64;
+  const int j = (int)(t % words);
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    const long long v = v0 + row0 + 4 * k;
+    bool ok = v < e.n && ((e.visit[(size_t)v * NW + j] >> col) & 1ull);
+#if GU_TR_NTL
+    x[k] = ok ? __builtin_nontemporal_load(&src[(size_t)v * (kWL * NW) + j * kWL + col]) : fill;
+#else
+    x[k] = ok ? src[(size_t)v * (kWL * NW) + j * kWL + col] : fill;
+#endif
+  }
+}
 template <int NW, class T>
 __global__ __launch_bounds__(256) void k_transpose_out(Eng<NW> e, const T *src, T *const *rows, T fill) {
   __shared__ T tile[64][65];
@@ -436,16 +464,7 @@ __global__ __launch_bounds__(256) void k_transpose_out(Eng<NW> e, const T *src, 
     const long long v0 = (t / words) * 64;
     const int j = (int)(t % words);
     T x[16];
-#pragma unroll
-    for (int k = 0; k < 16; k++) {
-      const long long v = v0 + row0 + 4 * k;
-      bool ok = v < e.n && ((e.visit[(size_t)v * NW + j] >> col) & 1ull);
-#if GU_TR_NTL
-      x[k] = ok ? __builtin_nontemporal_load(&src[(size_t)v * (kWL * NW) + j * kWL + col]) : fill;
-#else
-      x[k] = ok ? src[(size_t)v * (kWL * NW) + j * kWL + col] : fill;
-#endif
-    }
+    tr_load<NW, T>(e, src, fill, t, words, row0, col, x);
 #pragma unroll
     for (int k = 0; k < 16; k++) tile[row0 + 4 * k][col] = x[k];
     __syncthreads();
@@ -901,6 +920,7 @@ struct gu_graph {
   int *srcs_h = nullptr;      // pinned staging for batch sources
   int max_deg = 0;
   double avg_deg = 0;         // over vertices with out-degree > 0
+  bool simple = false;        // every adjacency row strictly increasing
   int *RT = nullptr, *CT = nullptr;  // in-edge CSR of a directed graph
   // Cluster-ordered copy of the graph for the per-source engine (bounded-
   // degree graphs): LR/LC in new ids, nof[old] = new id (device and host).
@@ -1112,6 +1132,7 @@ Eng<NW> make_eng(gu_graph *g, int nb, bool stack, bool sigma) {
   e.R = g->R;
   e.C = g->C;
   e.n = g->n;
+  e.simple = g->simple ? 1 : 0;
   e.nb = nb;
   e.visit = w.visit;
   e.fr = w.fr;
@@ -1875,8 +1896,12 @@ int gu_graph_create(const int32_t *R, const int32_t *C, const int32_t *F, int32_
     maxd = std::max(maxd, R[v + 1] - R[v]);
     nonzero += R[v + 1] > R[v];
   }
-  for (int64_t k = 0; k < m; k++)
-    if (C[k] < 0 || C[k] >= n) return GU_ERR_RANGE;
+  bool strict = true;  // rows strictly increasing: no parallel edges
+  for (int32_t v = 0; v < n; v++)
+    for (int64_t k = R[v]; k < R[v + 1]; k++) {
+      if (C[k] < 0 || C[k] >= n) return GU_ERR_RANGE;
+      strict &= k == R[v] || C[k] > C[k - 1];
+    }
   gu_graph *g = new (std::nothrow) gu_graph;
   if (!g) return GU_ERR_NOMEM;
   g->n = n;
@@ -1884,6 +1909,7 @@ int gu_graph_create(const int32_t *R, const int32_t *C, const int32_t *F, int32_
   g->directed = directed;
   g->max_deg = maxd;
   g->avg_deg = nonzero ? (double)m / (double)nonzero : 0.0;
+  g->simple = strict;
   int rc = GU_OK;
   do {
     if (hipGetDevice(&g->device) != hipSuccess) { rc = GU_ERR_HIP; break; }
@@ -1987,6 +2013,7 @@ int gu_graph_share(gu_graph_t g, gu_graph_t *out) {
   v->device = root->device;
   v->max_deg = root->max_deg;
   v->avg_deg = root->avg_deg;
+  v->simple = root->simple;
   v->RT = root->RT;
   v->CT = root->CT;
   v->LR = root->LR;

@@ -215,6 +215,37 @@ def test_hub_graph_search(gu, orc, gpu, direction, variant):
     _search_and_compare(gu, orc, hg, src, variant, direction, 64, "hubgraph")
 
 
+def _hub_multigraph(gu):
+    """The hub graph with parallel edges: source-to-hub edges doubled and
+    tripled, so path counts into a hub at level 1 are edge multiplicities
+    (the reference's adjacency walk counts every entry)."""
+    hg = _hub_graph(gu)
+    n = hg.n
+    adj = [list(hg.C[hg.R[v]:hg.R[v + 1]]) for v in range(n)]
+    for u, v, times in [(4000, 0, 1), (4001, 1, 2), (4000, 2, 1), (4001, 4000, 0), (4000, 4001, 1),
+                        (3, 4000, 0)]:
+        for _ in range(times):
+            adj[u].append(v)
+            adj[v].append(u)
+    if 4001 not in adj[4000]:
+        adj[4000].append(4001)
+        adj[4001].append(4000)
+    R = np.zeros(n + 1, dtype=np.int32)
+    R[1:] = np.cumsum([len(a) for a in adj])
+    C = np.array([c for a in adj for c in sorted(a)], dtype=np.int32)
+    return gu.HostGraph(R, C)
+
+
+@pytest.mark.parametrize("direction", ["auto", "topdown", "bottomup"])
+@pytest.mark.parametrize("width", [64, 128])
+def test_hub_multigraph_counts(gu, orc, gpu, direction, width):
+    hg = _hub_multigraph(gu)
+    src = np.array([0, 1, 2, 3, 4000, 4001, 3999, 7] * (width // 8), dtype=np.int32)
+    _search_and_compare(gu, orc, hg, src, "shuffle", direction, width, "hubmultigraph")
+    _, s1 = orc.sigma(hg.R, hg.C, 1)
+    assert s1[4001] == 3  # three parallel edges from source 1
+
+
 @pytest.mark.parametrize("width", [64, 256])
 def test_bc_hub_graph(gu, orc, gpu, width):
     hg = _hub_graph(gu)
