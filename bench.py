@@ -50,6 +50,17 @@ def model_bytes(n, k, st, sigma=True):
     return int(b)
 
 
+def model_bytes_bc(n, k, batches, st):
+    """Algorithmic bytes of a sampled Brandes call over k sources in
+    `batches` launches: the BFS + sigma forward model, plus the backward
+    20 n_s + 4 n_s (stack) + 8 m_s + 12 m'_s per source, plus the reduce
+    (4n read of delta per source, 16n FP64 read-modify-write per batch)."""
+    b = model_bytes(n, k, st)
+    b += 24 * st.reached + 8 * st.teps_units + 12 * st.dag_edges
+    b += 4 * n * k + 16 * n * batches
+    return int(b)
+
+
 def profile_traffic(name):
     """HBM bytes per step measured by the rocprofv3 counter runs summarised in
     profiles/<name> (line 'traffic_bytes_per_step: B'), or None."""
@@ -137,6 +148,16 @@ def bench_grid(args, gu, props):
     dev_ms = sum(s.device_ms for s in stats)
     levels = stats[-1].max_level + 1
     alg = sum(model_bytes(n, k, st) for st in stats)
+    extra = {}
+    if not args.no_cpu:
+        from oracle import oracle as orc
+
+        sample = batches[args.warmup][:96]
+        u, secs = orc.sigma_batch_timed(hg.R, hg.C, sample, 1)
+        extra["cpu_baseline"] = {
+            "value": round(u / secs / 1e9, 5), "unit": "GTEPS", "cores": 1, "kind": "port",
+            "sample": "the first 96 sources of the first timed batch, sequential BFS + path counts "
+                      "(oracle restating shortest_paths.cu:11-39), %.1f s" % secs}
     _line("GTEPS 256-source BFS + path counts, 2048^2 grid -10% LCC", round(units / wall / 1e9, 3),
           "GTEPS", args, wall / args.steps * 1e3, "int32+u64",
           {"workload": "2048x2048 4-neighbour grid, 10% edges deleted (seeded), largest component; "
@@ -151,7 +172,7 @@ def bench_grid(args, gu, props):
                         "frac": round(alg / (dev_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
                         "traffic": profile_traffic("grid2048_256src_traffic.txt")},
            "diameter_sampling": {"sources": 64, "diameter": diam, "ms": round(dwall * 1e3, 3),
-                                 "levels": dst.max_level + 1}})
+                                 "levels": dst.max_level + 1}, **extra})
 
 
 def bench_bc(args, gu, props):
@@ -201,6 +222,23 @@ def bench_bc(args, gu, props):
                      "zero_where_oracle_zero": bool(np.all(got[~nz] == 0)),
                      "oracle_seconds": round(time.time() - t2, 1)}
     wall, units, st = res[best]
+    alg = model_bytes_bc(n, k, (k + best - 1) // best, st)
+    extra = {"roofline": {"bound": "hbm", "achieved": round(alg / (st.device_ms / 1e3) / 1e9, 1),
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(alg / (st.device_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                          "traffic": None,
+                          "scope": "one 1024-source Brandes call at the best width (forward, "
+                                   "backward and FP64 reduce kernels)",
+                          "algorithmic_bytes_per_step": alg, "pass_ms": round(st.device_ms, 3)}}
+    if not args.no_cpu:
+        from oracle import oracle as orc
+
+        cs = src[:8]
+        u, secs = orc.brandes_timed(hg.R, hg.C, cs)
+        extra["cpu_baseline"] = {
+            "value": round(u / secs / 1e9, 5), "unit": "GTEPS", "cores": 1, "kind": "port",
+            "sample": "the first %d of the 1024 sources, sequential FP32 Brandes (oracle restating "
+                      "bc.cu:11-57), %.1f s; the rate applies linearly to all 1024" % (len(cs), secs)}
     _line("GTEPS sampled BC (1024 sources), R-MAT s22", round(units / wall / 1e9, 3), "GTEPS", args,
           wall * 1e3, "f32 delta / f64 BC / u64 sigma",
           {"workload": "R-MAT s22 ef16, 1024 sampled non-isolated sources per step", "n": n,
@@ -208,7 +246,7 @@ def bench_bc(args, gu, props):
            "arch": props["arch"]},
           {"batch_width": best, "autotuned": tuned,
            "ms_by_width": {str(w): round(res[w][0] * 1e3, 3) for w in res},
-           "levels_last_batch": st.max_level + 1, "bc_check": check})
+           "levels_last_batch": st.max_level + 1, "bc_check": check, **extra})
 
 
 def bench_lbs(args, gu, props):

@@ -646,6 +646,37 @@ __device__ __forceinline__ void bw_lanes(const Eng<NW> &e, int v, int L, u64 *mk
   }
 }
 
+// Hub segments of the backward pass: each wave writes its segment's
+// compensated partial sums; the combine kernel adds them in segment order,
+// so the result does not depend on scheduling.
+template <int NW>
+__device__ __forceinline__ void bw_hub_seg(const Eng<NW> &e, const Bw &b, int L, int si) {
+  const int lane = lane_id();
+  const int4 sg = e.segs[si];
+  const int v = sg.x;
+  u64 mk[NW];
+  bw_lanes<NW>(e, v, L, mk);
+  bool any = false;
+#pragma unroll
+  for (int j = 0; j < NW; j++) any |= mk[j] != 0;
+  if (!any) return;
+  float sv[NW], sum[NW], comp[NW];
+#pragma unroll
+  for (int j = 0; j < NW; j++) {
+    sum[j] = 0.0f;
+    comp[j] = 0.0f;
+    sv[j] = ((mk[j] >> lane) & 1ull) ? (float)e.svm[(size_t)v * (kWL * NW) + j * kWL + lane] : 0.0f;
+  }
+  bw_accum<NW>(e, b, L, mk, sv, sg.y, sg.z, sum, comp);
+#pragma unroll
+  for (int j = 0; j < NW; j++) b.part[(size_t)si * (kWL * NW) + j * kWL + lane] = make_float2(sum[j], comp[j]);
+}
+
+// One backward level.  Waves first take the hub segments (statically: they
+// are the long items), then claim 16-entry chunks of the level's stack
+// entries; both read only level L+1 rows and write disjoint data (segment
+// partials; the rows, masks and tags of non-hub vertices at level L), so
+// they share one launch.
 template <int NW>
 __global__ __launch_bounds__(256) void k_backward(Eng<NW> e, Bw b, int L) {
   const int lane = lane_id();
@@ -654,6 +685,9 @@ __global__ __launch_bounds__(256) void k_backward(Eng<NW> e, Bw b, int L) {
   const int pw = L & 1;
   int *tagw = b.btag + (size_t)pw * e.n;
   u64 *lmw = b.blm + (size_t)pw * e.n * NW;
+  const int waves = gridDim.x * (blockDim.x / kWave);
+  for (int si = blockIdx.x * (blockDim.x / kWave) + wave_id(); si < e.nsegs; si += waves)
+    bw_hub_seg<NW>(e, b, L, si);
   constexpr int CH = 16;
   while (true) {
     unsigned base = 0;
@@ -686,35 +720,6 @@ __global__ __launch_bounds__(256) void k_backward(Eng<NW> e, Bw b, int L) {
         tagw[v] = L;
       }
     }
-  }
-}
-
-// Hub segments of the backward pass: each wave writes its segment's
-// compensated partial sums; the combine kernel adds them in segment order,
-// so the result does not depend on scheduling.
-template <int NW>
-__global__ __launch_bounds__(256) void k_bw_hub_segs(Eng<NW> e, Bw b, int L) {
-  const int lane = lane_id();
-  const int waves = gridDim.x * (blockDim.x / kWave);
-  for (int si = blockIdx.x * (blockDim.x / kWave) + wave_id(); si < e.nsegs; si += waves) {
-    const int4 sg = e.segs[si];
-    const int v = sg.x;
-    u64 mk[NW];
-    bw_lanes<NW>(e, v, L, mk);
-    bool any = false;
-#pragma unroll
-    for (int j = 0; j < NW; j++) any |= mk[j] != 0;
-    if (!any) continue;
-    float sv[NW], sum[NW], comp[NW];
-#pragma unroll
-    for (int j = 0; j < NW; j++) {
-      sum[j] = 0.0f;
-      comp[j] = 0.0f;
-      sv[j] = ((mk[j] >> lane) & 1ull) ? (float)e.svm[(size_t)v * (kWL * NW) + j * kWL + lane] : 0.0f;
-    }
-    bw_accum<NW>(e, b, L, mk, sv, sg.y, sg.z, sum, comp);
-#pragma unroll
-    for (int j = 0; j < NW; j++) b.part[(size_t)si * (kWL * NW) + j * kWL + lane] = make_float2(sum[j], comp[j]);
   }
 }
 
@@ -1329,13 +1334,10 @@ int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const g
     eb.hubs = g->ou.hubs;
     eb.nhubs = g->ou.nhubs;
     Bw b{w.dl, w.btag, w.blm, w.bwork, w.part, g->ou.seg0};
-    const int hgrid = std::max(1, std::min(pick_grid(g, 8), (g->ou.nsegs + 3) / 4));
+    const int cgrid = std::max(1, std::min(pick_grid(g, 8), (g->ou.nhubs + 3) / 4));
     for (int L = last; L >= 1; L--) {
       k_backward<NW><<<pick_grid(g, 8), 256, 0, st>>>(eb, b, L);
-      if (g->ou.nhubs) {
-        k_bw_hub_segs<NW><<<hgrid, 256, 0, st>>>(eb, b, L);
-        k_bw_hub_combine<NW><<<std::max(1, std::min(hgrid, (g->ou.nhubs + 3) / 4)), 256, 0, st>>>(eb, b, L);
-      }
+      if (g->ou.nhubs) k_bw_hub_combine<NW><<<cgrid, 256, 0, st>>>(eb, b, L);
     }
     k_bc_out<NW><<<pick_grid(g, 8), 256, 0, st>>>(e, w.dl, out.delta, out.bc);
   }

@@ -148,6 +148,26 @@ def sigma_batch_timed(R, C, sources, nthreads=1):
     return int(units), secs.value
 
 
+def brandes_timed(R, C, sources):
+    """Times the reference's sequential FP32 Brandes (bc.cu:11-57) source
+    after source on one core; returns (TEPS units = edges scanned from the
+    reached vertices, summed over sources; seconds inside the calls)."""
+    import time
+
+    R, C = _csr(R, C)
+    n = len(R) - 1
+    d = np.empty(n, dtype=np.int32)
+    s = np.empty(n, dtype=np.uint64)
+    delta = np.empty(n, dtype=np.float32)
+    units, secs = 0, 0.0
+    for src in np.asarray(sources, dtype=np.int32):
+        t0 = time.perf_counter()
+        lib().or_brandes_f32(_p(R), _p(C), n, int(src), _p(d), _p(s), _p(delta))
+        secs += time.perf_counter() - t0
+        units += teps_units(R, d)
+    return units, secs
+
+
 def lbs_timed(counts):
     c = np.ascontiguousarray(counts, dtype=np.int32)
     W = int(c.astype(np.int64).sum())
