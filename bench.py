@@ -559,7 +559,8 @@ def main():
     batches = [hg.sample_sources(k, seed=1000 + i) for i in range(nb)]
     dist = gu.DeviceBuffer(4 * k * n)
     sig = gu.DeviceBuffer(8 * k * n)
-    cfg = gu.make_config(args.variant, alpha=args.alpha, beta=args.beta, level_graphs=args.level_graphs)
+    cfg = gu.make_config(args.variant, alpha=args.alpha, beta=args.beta, level_graphs=args.level_graphs,
+                         batch_width=args.width)
     with gu.DeviceGraph(hg) as dg:
         tuned = None
         if not args.no_tune and args.variant == "auto":

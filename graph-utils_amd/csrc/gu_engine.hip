@@ -150,18 +150,14 @@ __global__ __launch_bounds__(1024) void k_finalize(Eng<NW> e) {
     else if (SIGMA)
       hub_segments<NW, SIGMA>(e, L, mode, wave_group<NW>(), a);
   }
-  write_acct<NW>(e, 1, a);
-}
-
-template <int NW>
-__global__ __launch_bounds__(256) void k_hub_commit(Eng<NW> e) {
-  if (e.ctrl->done) return;
-  const int L = e.ctrl->level;
+  // The hub commit touches only hubs' masks, distances and pushes: top-down
+  // it reads the lanes the expansion pushed, bottom-up the lanes the
+  // segments in k_expand found, so it needs nothing computed above and
+  // shares this launch (0.9% per s20 pass over a launch of its own).
   Pusher pb{wave_pushbuf<NW>(), 0};
-  Acc a;
-  hub_commit<NW>(e, L, e.ctrl->mode, pb, a);
+  hub_commit<NW>(e, L, mode, pb, a);
   pb_flush<NW>(e, L, pb);
-  write_acct<NW>(e, 2, a);
+  write_acct<NW>(e, 1, a);
 }
 
 // Hubs take their path counts by atomic adds, so their rows start at zero.
@@ -595,7 +591,7 @@ __device__ __forceinline__ void bw_accum(const Eng<NW> &e, const Bw &b, int L, c
     }
     uint64_t pm = ballot(has);
     while (pm) {
-      constexpr int D = NW == 1 ? 16 : 4;
+      constexpr int D = NW == 1 ? 16 : 4;  // 8 for 128 lanes measured 4% slower
       int k[D];
 #pragma unroll
       for (int u = 0; u < D; u++) {
@@ -1188,7 +1184,6 @@ void launch_level(gu_graph *g, const Eng<NW> &e, int variant, int block, int gri
     default: k_expand<NW, V_SHUFFLE, SIGMA><<<e.grid_k[0], block, lx, st>>>(e); break;
   }
   k_finalize<NW, SIGMA><<<e.grid_k[1], 256, l256, st>>>(e);
-  if (e.grid_k[2]) k_hub_commit<NW><<<e.grid_k[2], 256, l256, st>>>(e);
   k_control<NW><<<1, 256, 0, st>>>(e);
 }
 
@@ -1271,7 +1266,7 @@ int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const g
   auto set_grids = [&](Eng<NW> &x) {
     x.grid_k[0] = grid;
     x.grid_k[1] = std::min(kAcctMax, pick_grid(g, 8));
-    x.grid_k[2] = g->pu.nhubs ? std::max(1, std::min(pick_grid(g, 2), (g->pu.nhubs + 255) / 256)) : 0;
+    x.grid_k[2] = 0;  // accounting slot of a third level kernel (none)
   };
   set_grids(e);
   int K = cfg.sync_every > 0 ? cfg.sync_every : 2;

@@ -334,11 +334,14 @@ __device__ void td_cta_warp(const Eng<NW> &e, int L, Pusher &pb) {
 }
 
 // scan_based.cu:105-169: a workgroup scans the degrees of its frontier
-// tile, then gathers the tile's edges blockDim at a time through LDS.
+// tile, then gathers the tile's edges blockDim at a time.  Each thread finds
+// the owner of its edge by binary search over the tile's scanned degrees in
+// LDS, so a hub in the tile costs its edges / blockDim rounds, not a serial
+// fill by the hub's own thread (2x slower BC on R-MAT s22 at 64 lanes).
 template <int NW>
 __device__ void td_scan(const Eng<NW> &e, int L, Pusher &pb) {
-  __shared__ int s_e[1024];
-  __shared__ int s_slot[1024];
+  __shared__ int s_off[1024];
+  __shared__ int s_r[1024];
   __shared__ int s_vert[1024];
   __shared__ int s_scan[17];
   const int tid = threadIdx.x;
@@ -352,31 +355,31 @@ __device__ void td_scan(const Eng<NW> &e, int L, Pusher &pb) {
     int v = have ? Q[k] : 0;
     int r = have ? e.R[v] : 0;
     int re = have ? e.R[v + 1] : 0;
-    s_vert[tid] = v;
     int total;
     int rsv = block_exclusive_sum(re - r, s_scan, &total);
+    s_vert[tid] = v;
+    s_r[tid] = r;
+    s_off[tid] = rsv;
+    __syncthreads();
     for (int progress = 0; progress < total; progress += T) {
-      while (rsv < progress + T && r < re) {
-        s_e[rsv - progress] = r;
-        s_slot[rsv - progress] = tid;
-        rsv++;
-        r++;
-      }
-      __syncthreads();
-      int cnt = total - progress < T ? total - progress : T;
+      const int cnt = total - progress < T ? total - progress : T;
       bool p = false;
       int w = 0;
       if (tid < cnt) {
-        int ed = s_e[tid];
-        int vv = s_vert[s_slot[tid]];
+        const int g = progress + tid;
+        int a = 0, b = T;  // last slot with s_off <= g
+        while (b - a > 1) {
+          const int mid = (a + b) >> 1;
+          if (s_off[mid] <= g) a = mid; else b = mid;
+        }
         u64 f[NW];
-        load_mask<NW>(fc + (size_t)vv * NW, f);
-        w = e.C[ed];
+        load_mask<NW>(fc + (size_t)s_vert[a] * NW, f);
+        w = e.C[s_r[a] + (g - s_off[a])];
         p = discover<NW>(e, L, f, w);
       }
       pb_push<NW>(e, L, pb, p, w);
-      __syncthreads();
     }
+    __syncthreads();
   }
 }
 
@@ -637,7 +640,10 @@ __device__ void pull_group(const Eng<NW> &e, int L, int w_l, const u64 *tgt_l, i
                            GroupLds<NW> &g, Pusher &pb, Acc &a, int aux_l = 0, bool seg_bu = false) {
   constexpr bool BU = MODE == PM_BU;
   constexpr bool OR_FOUND = MODE != PM_TD;
-  constexpr int D = NW == 1 ? 8 : (NW == 2 ? 4 : 2);  // predecessor rows in flight
+  // predecessor rows in flight, measured on s20 BFS: 64 lanes 12 (8: -1.7%,
+  // 16: +0.3% at 80 VGPRs), 128 lanes 8 (4: -5%, 12: -2%), 256 lanes 8
+  // (2: -7%, 4: -2%)
+  constexpr int D = NW == 1 ? 12 : 8;
   const int lane = lane_id();
   const int inc = wave_inclusive_sum(deg_l);
   const int T = wave_total(inc);
