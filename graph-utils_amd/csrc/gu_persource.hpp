@@ -22,6 +22,7 @@ namespace gu {
 struct PsArgs {
   const int *R;
   const int *C;
+  const int4 *LE;    // degree <= 4 graphs: neighbours per vertex, -1 padded
   int n;
   const int *srcs;   // [nsrc]
   int *const *dist;  // [nsrc] distance rows of n entries
@@ -150,6 +151,66 @@ __device__ __forceinline__ void ps_visit(const int *R, const int *C, int *d, u64
   if (SIGMA && PULL && v >= 0 && L >= 1) sg[v] = acc;
 }
 
+// ps_visit on a graph of degree at most 4 kept as one int4 of neighbours
+// per vertex (-1 padded): the visit is one 16-byte load, and a claimed
+// vertex enters the next frontier as its id alone, with no row-offset loads
+// before the level's barrier.
+template <bool SIGMA, bool PULL>
+__device__ __forceinline__ void ps_visit_ell(const int4 *LE, int *d, u64 *sg, int v, int L, u64 sv,
+                                             int &ovf, u64 &dag, u64 &teps, unsigned *cnt, PsFront &f,
+                                             int *spill, int nbuf) {
+  constexpr int B = 4;
+  if (v < 0) return;
+  const int4 q = LE[v];
+  int w[B] = {q.x, q.y, q.z, q.w}, o[B];
+#pragma unroll
+  for (int j = 0; j < B; j++) teps += w[j] >= 0;
+#pragma unroll
+  for (int j = 0; j < B; j++) o[j] = w[j] >= 0 ? ((SIGMA && PULL) ? ld_l2(&d[w[j]]) : d[w[j]]) : -1;
+#pragma unroll
+  for (int j = 0; j < B; j++)
+    if (o[j] == INT_MAX) o[j] = atomicCAS(&d[w[j]], INT_MAX, L + 1);  // INT_MAX back: claimed
+  unsigned claims = 0;
+#pragma unroll
+  for (int j = 0; j < B; j++) claims += o[j] == INT_MAX;
+  unsigned pos = claims ? atomicAdd(cnt, claims) : 0u;
+  u64 x[B];
+  if (SIGMA) {
+#pragma unroll
+    for (int j = 0; j < B; j++) {
+      if (PULL)
+        x[j] = (w[j] >= 0 && L >= 1 && o[j] == L - 1) ? ld_l2(&sg[w[j]]) : 0ull;
+      else
+        x[j] = (o[j] == INT_MAX || o[j] == L + 1) ? atomicAdd(&sg[w[j]], sv) : 0ull;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < B; j++) {
+    if (o[j] != INT_MAX) continue;
+    if (pos < (unsigned)kPsCapLds)
+      f.v[nbuf][pos] = w[j];
+    else
+      spill[pos - kPsCapLds] = w[j];
+    pos++;
+  }
+  if (SIGMA) {
+    u64 acc = 0;
+#pragma unroll
+    for (int j = 0; j < B; j++) {
+      if (PULL && w[j] >= 0 && L >= 1 && o[j] == L - 1) {
+        acc += x[j];
+        ovf |= acc < x[j];
+        dag++;
+      }
+      if (!PULL && (o[j] == INT_MAX || o[j] == L + 1)) {
+        ovf |= x[j] + sv < sv;
+        dag++;
+      }
+    }
+    if (PULL && L >= 1) sg[v] = acc;
+  }
+}
+
 // A wave strip-mines the adjacency lists longer than kPsSerialDeg of its
 // lanes, electing the highest lane that still has one (race_and_resolve,
 // warp_based.cu:109-145); in PULL mode the lanes' partial path counts are
@@ -215,9 +276,10 @@ __device__ __forceinline__ void ps_visit_wave(const int *R, const int *C, int *d
 // level's vertices are also appended to S in HBM (E[l] = start of level l),
 // and a frontier's entries beyond the LDS capacity are read back from S;
 // otherwise they go to the two `spill` queues.  Returns the deepest level.
-template <bool SIGMA, bool PULL, bool BC>
+template <bool SIGMA, bool PULL, bool BC, bool ELL = false>
 __device__ int ps_levels(const int *R, const int *C, int n, int s, int *d, u64 *sg, int *spill0, int *S,
-                         int *E, PsFront &f, unsigned *s_cnt, int &ovf, u64 &teps, u64 &reached, u64 &dag) {
+                         int *E, PsFront &f, unsigned *s_cnt, int &ovf, u64 &teps, u64 &reached, u64 &dag,
+                         const int4 *LE = nullptr) {
   const int tid = threadIdx.x, T = blockDim.x;
   if (tid < 3) s_cnt[tid] = 0;
   if (tid == 0) {
@@ -244,6 +306,12 @@ __device__ int ps_levels(const int *R, const int *C, int n, int s, int *d, u64 *
     int *sp_next = BC ? S + qs + kPsCapLds : spill0 + (size_t)(b ^ 1) * n;
     for (unsigned base = 0; base < qn; base += T) {
       const unsigned k = base + tid;
+      if (ELL) {
+        const int v = k >= qn ? -1 : (k < (unsigned)kPsCapLds ? f.v[b][k] : sp_cur[k - kPsCapLds]);
+        const u64 sv = (SIGMA && !PULL && v >= 0) ? ld_l2(&sg[v]) : 0ull;
+        ps_visit_ell<SIGMA, PULL>(LE, d, sg, v, L, sv, ovf, dag, teps, cnt, f, sp_next, b ^ 1);
+        continue;
+      }
       int v = -1, r = 0, re = 0;
       if (k < qn) ps_entry(f, sp_cur, R, b, k, v, r, re);
       const u64 sv = (SIGMA && !PULL && v >= 0) ? ld_l2(&sg[v]) : 0ull;
@@ -297,7 +365,7 @@ __device__ __forceinline__ void ps_finish(u64 teps, u64 dag, u64 reached, int L,
   }
 }
 
-template <bool SIGMA, bool PULL>
+template <bool SIGMA, bool PULL, bool ELL = false>
 __global__ __launch_bounds__(1024) void k_persource(PsArgs a) {
   __shared__ PsFront f;
   __shared__ unsigned s_cnt[3];
@@ -316,8 +384,8 @@ __global__ __launch_bounds__(1024) void k_persource(PsArgs a) {
   __syncthreads();
   int ovf = 0;
   u64 teps = 0, reached = 1, dag = 0;
-  const int L = ps_levels<SIGMA, PULL, false>(a.R, a.C, a.n, s, d, sg, spill, nullptr, nullptr, f, s_cnt,
-                                              ovf, teps, reached, dag);
+  const int L = ps_levels<SIGMA, PULL, false, ELL>(a.R, a.C, a.n, s, d, sg, spill, nullptr, nullptr, f, s_cnt,
+                                                   ovf, teps, reached, dag, a.LE);
   ps_finish(teps, dag, reached, L, ovf, &s_ovf, s_red, a.part, a.depth, a.ovf);
 }
 

@@ -421,6 +421,26 @@ def test_persource_directed(gu, orc, gpu):
                 "directed")
 
 
+def test_persource_directed_degree4(gu, orc, gpu):
+    """Directed graph of out-degree <= 4 (some vertices without out-edges):
+    the per-source kernel reads it as one int4 of neighbours per vertex and
+    pushes path counts along arcs."""
+    rng = np.random.default_rng(77)
+    n = 5000
+    rows = []
+    for v in range(n):
+        k = int(rng.integers(0, 5))
+        t = rng.choice(n - 1, size=k, replace=False)
+        rows.append(np.sort(np.where(t >= v, t + 1, t)))
+    R = np.zeros(n + 1, dtype=np.int32)
+    R[1:] = np.cumsum([len(r) for r in rows])
+    C = np.concatenate(rows).astype(np.int32)
+    hg = gu.HostGraph(R, C, directed=True)
+    assert hg.degrees().max() <= 4
+    _ps_compare(gu, orc, hg, hg.sample_sources(64, seed=78), "directed-deg4")
+    _ps_compare(gu, orc, hg, hg.sample_sources(16, seed=79), "directed-deg4-d", sigma=False)
+
+
 def test_persource_distances_only(gu, orc, gpu):
     hg = gu.HostGraph.grid(32, 0.10, seed=9)
     _ps_compare(gu, orc, hg, hg.sample_sources(40, seed=10), "grid32-d", sigma=False)
