@@ -1522,10 +1522,15 @@ int run_persource_bc(gu_graph *g, const int *sources, int nsrc, const HostRows &
     GU_HIP(hipMemcpyAsync(p.srcs, g->srcs_h, sizeof(int) * nb, hipMemcpyHostToDevice, st));
     GU_HIP(hipMemcpyAsync(g->rowtab, t, sizeof(void *) * kRowTab, hipMemcpyHostToDevice, st));
     if (b0 == 0) k_fill_i32<<<1, 64, 0, st>>>(p.ovf, 1, 0);
-    PsBcArgs a{rl ? g->LR : g->R, rl ? g->LC : g->C, g->n, p.srcs, (int *const *)g->rowtab,
+    const int4 *LE = rl ? g->LE : nullptr;
+    PsBcArgs a{rl ? g->LR : g->R, rl ? g->LC : g->C, LE, g->n, p.srcs, (int *const *)g->rowtab,
                (u64 *const *)(g->rowtab + 256), (float *const *)(g->rowtab + 512), p.queues, p.depth,
                p.part, p.ovf};
-    if (g->directed)
+    if (LE && g->directed)
+      k_ps_bc<false, true><<<nb, 1024, 0, st>>>(a);
+    else if (LE)
+      k_ps_bc<true, true><<<nb, 1024, 0, st>>>(a);
+    else if (g->directed)
       k_ps_bc<false><<<nb, 1024, 0, st>>>(a);
     else
       k_ps_bc<true><<<nb, 1024, 0, st>>>(a);

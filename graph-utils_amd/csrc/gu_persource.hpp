@@ -158,7 +158,7 @@ __device__ __forceinline__ void ps_visit(const int *R, const int *C, int *d, u64
 template <bool SIGMA, bool PULL, int NV>
 __device__ __forceinline__ void ps_visit_ell(const int4 *LE, int *d, u64 *sg, const int *vv, int L,
                                              const u64 *sv, int &ovf, u64 &dag, u64 &teps, unsigned *cnt,
-                                             PsFront &f, int *spill, int nbuf) {
+                                             PsFront &f, int *spill, int *S, int nbuf) {
   constexpr int B = 4 * NV;
   int w[B], o[B];
 #pragma unroll
@@ -193,6 +193,7 @@ __device__ __forceinline__ void ps_visit_ell(const int4 *LE, int *d, u64 *sg, co
 #pragma unroll
   for (int j = 0; j < B; j++) {
     if (o[j] != INT_MAX) continue;
+    if (S) S[pos] = w[j];
     if (pos < (unsigned)kPsCapLds)
       f.v[nbuf][pos] = w[j];
     else
@@ -325,7 +326,7 @@ __device__ int ps_levels(const int *R, const int *C, int n, int s, int *d, u64 *
           vv[i] = k >= qn ? -1 : (k < (unsigned)kPsCapLds ? f.v[b][k] : sp_cur[k - kPsCapLds]);
           sv[i] = (SIGMA && !PULL && vv[i] >= 0) ? ld_l2(&sg[vv[i]]) : 0ull;
         }
-        ps_visit_ell<SIGMA, PULL, NV>(LE, d, sg, vv, L, sv, ovf, dag, teps, cnt, f, sp_next, b ^ 1);
+        ps_visit_ell<SIGMA, PULL, NV>(LE, d, sg, vv, L, sv, ovf, dag, teps, cnt, f, sp_next, Snext, b ^ 1);
       }
     }
     for (unsigned base = 0; !ELL && base < qn; base += T) {
@@ -419,6 +420,7 @@ __global__ __launch_bounds__(1024) void k_persource(PsArgs a) {
 struct PsBcArgs {
   const int *R;
   const int *C;
+  const int4 *LE;       // degree <= 4 graphs: the forward's neighbour table
   int n;
   const int *srcs;
   int *const *dist;     // [nsrc] scratch rows
@@ -439,7 +441,7 @@ __device__ __forceinline__ void ps_neumaier(float &sum, float &comp, float x) {
   sum = t;
 }
 
-template <bool PULL>
+template <bool PULL, bool ELL = false>
 __global__ __launch_bounds__(1024) void k_ps_bc(PsBcArgs a) {
   __shared__ PsFront f;
   __shared__ unsigned s_cnt[3];
@@ -463,8 +465,8 @@ __global__ __launch_bounds__(1024) void k_ps_bc(PsBcArgs a) {
   __syncthreads();
   int ovf = 0;
   u64 teps = 0, reached = 1, dag = 0;
-  const int L = ps_levels<true, PULL, true>(a.R, a.C, a.n, s, d, sg, nullptr, S, E, f, s_cnt, ovf, teps,
-                                            reached, dag);
+  const int L = ps_levels<true, PULL, true, ELL>(a.R, a.C, a.n, s, d, sg, nullptr, S, E, f, s_cnt, ovf, teps,
+                                                 reached, dag, a.LE);
   __syncthreads();
   // backward, deepest level first (its vertices have no successors): each
   // thread sums the successors of one vertex in adjacency order, loading a

@@ -421,12 +421,10 @@ def test_persource_directed(gu, orc, gpu):
                 "directed")
 
 
-def test_persource_directed_degree4(gu, orc, gpu):
-    """Directed graph of out-degree <= 4 (some vertices without out-edges):
-    the per-source kernel reads it as one int4 of neighbours per vertex and
-    pushes path counts along arcs."""
-    rng = np.random.default_rng(77)
-    n = 5000
+def _directed_degree4(gu, n=5000, seed=77):
+    """Random directed graph of out-degree 0-4, no self-loops or parallel
+    arcs."""
+    rng = np.random.default_rng(seed)
     rows = []
     for v in range(n):
         k = int(rng.integers(0, 5))
@@ -437,8 +435,39 @@ def test_persource_directed_degree4(gu, orc, gpu):
     C = np.concatenate(rows).astype(np.int32)
     hg = gu.HostGraph(R, C, directed=True)
     assert hg.degrees().max() <= 4
+    return hg
+
+
+def test_persource_directed_degree4(gu, orc, gpu):
+    """Directed graph of out-degree <= 4 (some vertices without out-edges):
+    the per-source kernel reads it as one int4 of neighbours per vertex and
+    pushes path counts along arcs."""
+    hg = _directed_degree4(gu)
     _ps_compare(gu, orc, hg, hg.sample_sources(64, seed=78), "directed-deg4")
     _ps_compare(gu, orc, hg, hg.sample_sources(16, seed=79), "directed-deg4-d", sigma=False)
+
+
+def _binary_tree(gu, depth):
+    """Complete binary tree with 2^depth - 1 vertices (heap numbering): the
+    deepest level holds 2^(depth-1) vertices, more than the per-source LDS
+    frontier, so the HBM spill path runs."""
+    n = (1 << depth) - 1
+    adj = [[] for _ in range(n)]
+    for v in range(1, n):
+        p = (v - 1) // 2
+        adj[p].append(v)
+        adj[v].append(p)
+    R = np.zeros(n + 1, dtype=np.int32)
+    R[1:] = np.cumsum([len(a) for a in adj])
+    C = np.array([c for a in adj for c in sorted(a)], dtype=np.int32)
+    return gu.HostGraph(R, C)
+
+
+def test_persource_tree_frontier_spill(gu, orc, gpu):
+    hg = _binary_tree(gu, 13)  # 4096 leaves > 3072 LDS frontier entries
+    src = np.array([0, 1, 2, hg.n - 1, hg.n // 2, 77], dtype=np.int32)
+    _ps_compare(gu, orc, hg, src, "tree13")
+    _check_bc(gu, orc, hg, src, 64, "tree13", engine=PS)
 
 
 def test_persource_distances_only(gu, orc, gpu):
@@ -501,9 +530,9 @@ def _brandes_directed_f64(R, C, n, s):
     return delta
 
 
-@pytest.mark.parametrize("engine", [1, PS])
-def test_bc_directed(gu, gpu, engine):
-    hg = _directed_graph(gu)
+@pytest.mark.parametrize("engine,graph", [(1, "hubs"), (PS, "hubs"), (PS, "degree4")])
+def test_bc_directed(gu, gpu, engine, graph):
+    hg = _directed_graph(gu) if graph == "hubs" else _directed_degree4(gu, 3000, 91)
     src = np.concatenate([[0, 1], hg.sample_sources(30, seed=7)]).astype(np.int32)
     announce("bc directed engine=%d" % engine)
     with gu.DeviceGraph(hg) as dg:
