@@ -190,7 +190,7 @@ def bench_bc(args, gu, props):
         tuned = None
         if not args.no_tune and args.variant == "auto":
             tuned = dg.autotune(src[:256], gu.QUERY_BC)
-        for width in (64, 128, 256):
+        for width in ((args.width,) if args.width else (64, 128, 256)):
             cfg = gu.make_config(args.variant, batch_width=width)
             bc.zero()
             dg.betweenness_device(src[:width], None, bc, cfg)  # warm-up
@@ -226,9 +226,9 @@ def bench_bc(args, gu, props):
     extra = {"roofline": {"bound": "hbm", "achieved": round(alg / (st.device_ms / 1e3) / 1e9, 1),
                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": round(alg / (st.device_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
-                          "traffic": None,
+                          "traffic": profile_traffic("bc_rmat22_1024src_traffic.txt"),
                           "scope": "one 1024-source Brandes call at the best width (forward, "
-                                   "backward and FP64 reduce kernels)",
+                                   "backward and FP64 reduce kernels); traffic measured at width 128",
                           "algorithmic_bytes_per_step": alg, "pass_ms": round(st.device_ms, 3)}}
     if not args.no_cpu:
         from oracle import oracle as orc
