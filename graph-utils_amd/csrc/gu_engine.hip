@@ -561,12 +561,17 @@ __global__ __launch_bounds__(256) void k_teps(Eng<NW> e) {
 // ---- Brandes backward (shuffle_based.cu:217-354 pull form) ----------------
 // Level L, lanes mk of vertex v (lanes for which d(v) = L):
 //   delta_b(v) = sum over neighbours w with d_b(w) = L+1 of
-//                (float)sigma_b(v) / (float)sigma_b(w) * (1 + delta_b(w))
-// summed in FP32 with Neumaier compensation.  Successor lanes come from a
+//                (float)sigma_b(v) * q_b(w),  q_b(w) = (1 + delta_b(w)) / (float)sigma_b(w)
+// summed in FP32 with Neumaier compensation.  q is stored when delta(w) is
+// final, so a successor costs one FP32 row instead of its u64 path count and
+// FP32 dependency rows (the reference's sigma_v / sigma_w * (1 + delta_w),
+// bc.cu:52, reassociated: both are within a few ulp of the FP64 value).  Successor lanes come from a
 // dense per-parity mask array tagged with the level that wrote it, so a
 // level never needs a clearing pass.
 struct Bw {
   float *dl;          // [n][64*NW] dependencies
+  float *ql;          // [n][64*NW] (1 + delta) / sigma of each vertex's lanes
+                      //   once its delta is final: one row per successor
   int *btag;          // [2][n] level that last wrote each parity's mask
   u64 *blm;           // [2][n][NW] lanes at that level
   unsigned *bwork;    // per-level work counters
@@ -620,7 +625,7 @@ __device__ __forceinline__ void bw_accum(const Eng<NW> &e, const Bw &b, int L, c
           k[u] = -1;
         }
       }
-      float sw[D][NW], dw[D][NW];
+      float qw[D][NW];
       bool use[D][NW];
 #pragma unroll
       for (int u = 0; u < D; u++) {
@@ -631,8 +636,7 @@ __device__ __forceinline__ void bw_accum(const Eng<NW> &e, const Bw &b, int L, c
           for (int j = 0; j < NW; j++) {
             u64 tk = readlane64(t[j], k[u]);
             use[u][j] = (tk >> lane) & 1ull;
-            sw[u][j] = use[u][j] ? (float)e.svm[rowb + j * kWL + lane] : 1.0f;
-            dw[u][j] = use[u][j] ? b.dl[rowb + j * kWL + lane] : 0.0f;
+            qw[u][j] = use[u][j] ? b.ql[rowb + j * kWL + lane] : 0.0f;
           }
         } else {
 #pragma unroll
@@ -643,7 +647,7 @@ __device__ __forceinline__ void bw_accum(const Eng<NW> &e, const Bw &b, int L, c
       for (int u = 0; u < D; u++)
 #pragma unroll
         for (int j = 0; j < NW; j++)
-          if (use[u][j]) neumaier(sum[j], comp[j], (sv[j] / sw[u][j]) * (1.0f + dw[u][j]));
+          if (use[u][j]) neumaier(sum[j], comp[j], sv[j] * qw[u][j]);
     }
   }
 }
@@ -727,7 +731,11 @@ __global__ __launch_bounds__(256) void k_backward(Eng<NW> e, Bw b, int L) {
       const size_t vrow = (size_t)v * (kWL * NW);
 #pragma unroll
       for (int j = 0; j < NW; j++)
-        if ((mk[j] >> lane) & 1ull) b.dl[vrow + j * kWL + lane] = sum[j] + comp[j];
+        if ((mk[j] >> lane) & 1ull) {
+          const float dv = sum[j] + comp[j];
+          b.dl[vrow + j * kWL + lane] = dv;
+          b.ql[vrow + j * kWL + lane] = (1.0f + dv) / sv[j];
+        }
       if (lane == 0) {
 #pragma unroll
         for (int j = 0; j < NW; j++) lmw[(size_t)v * NW + j] = mk[j];
@@ -761,7 +769,11 @@ __global__ __launch_bounds__(256) void k_bw_hub_combine(Eng<NW> e, Bw b, int L) 
         neumaier(sum, comp, p.x);
         comp += p.y;
       }
-      if ((mk[j] >> lane) & 1ull) b.dl[vrow + j * kWL + lane] = sum + comp;
+      if ((mk[j] >> lane) & 1ull) {
+        const float dv = sum + comp;
+        b.dl[vrow + j * kWL + lane] = dv;
+        b.ql[vrow + j * kWL + lane] = (1.0f + dv) / (float)e.svm[vrow + j * kWL + lane];
+      }
     }
     if (lane == 0) {
 #pragma unroll
@@ -869,14 +881,14 @@ struct Workspace {
   u64 *hub_found = nullptr;
   u64 *acct = nullptr;
   int *q = nullptr, *tag = nullptr, *dvm = nullptr, *off = nullptr, *srcs = nullptr, *btag = nullptr;
-  float *dl = nullptr;
+  float *dl = nullptr, *ql = nullptr;
   float2 *part = nullptr;
   long long *ep = nullptr, *tsum = nullptr;
   unsigned *bwork = nullptr;
   size_t S_cap = 0;   // entries of q in stack mode
   size_t bytes = 0;
   void release() {
-    void *ps[] = {visit, fr, svm, smask, blm, hub_found, acct, q, tag, dvm, off, srcs, btag, dl, part, ep, tsum, bwork};
+    void *ps[] = {visit, fr, svm, smask, blm, hub_found, acct, q, tag, dvm, off, srcs, btag, dl, ql, part, ep, tsum, bwork};
     for (void *p : ps)
       if (p) (void)hipFree(p);
     *this = Workspace();
@@ -893,12 +905,13 @@ struct PsWorkspace {
   int *dscratch = nullptr;  // [cap][n] distances when the caller keeps none
   u64 *sscratch = nullptr;  // [cap][n] path counts of a Brandes launch
   float *xscratch = nullptr;  // [cap][n] dependencies when the caller keeps none
+  float *qscratch = nullptr;  // [cap][n] (1 + delta) / sigma of a Brandes launch
   int *depth = nullptr, *ovf = nullptr;
   int *srcs = nullptr;      // [cap] sources of the launch
   u64 *part = nullptr;
   size_t bytes = 0;
   void release() {
-    void *ps[] = {queues, dscratch, sscratch, xscratch, depth, ovf, srcs, part};
+    void *ps[] = {queues, dscratch, sscratch, xscratch, qscratch, depth, ovf, srcs, part};
     for (void *p : ps)
       if (p) (void)hipFree(p);
     *this = PsWorkspace();
@@ -1101,11 +1114,12 @@ int ensure_ws(gu_graph *g, int NW, bool stack, size_t want_S, hipStream_t st) {
       w.stack = true;
       GU_HIP(dalloc(&w.smask, w.S_cap * NW));
       GU_HIP(dalloc(&w.dl, n * WL));
+      GU_HIP(dalloc(&w.ql, n * WL));
       GU_HIP(dalloc(&w.btag, 2 * n));
       GU_HIP(dalloc(&w.blm, 2 * n * NW));
       GU_HIP(dalloc(&w.bwork, n + 3));
       GU_HIP(dalloc(&w.part, (size_t)std::max(g->ou.nsegs, 1) * WL));
-      w.bytes += w.S_cap * NW * 8 + n * WL * 4 + 2 * n * 4 + 2 * n * NW * 8 + (n + 3) * 4;
+      w.bytes += w.S_cap * NW * 8 + n * WL * 8 + 2 * n * 4 + 2 * n * NW * 8 + (n + 3) * 4;
     }
   }
   if (stack && want_S > w.S_cap) {  // grow the stack, keeping its contents
@@ -1350,7 +1364,7 @@ int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const g
     eb.nsegs = g->ou.nsegs;
     eb.hubs = g->ou.hubs;
     eb.nhubs = g->ou.nhubs;
-    Bw b{w.dl, w.btag, w.blm, w.bwork, w.part, g->ou.seg0};
+    Bw b{w.dl, w.ql, w.btag, w.blm, w.bwork, w.part, g->ou.seg0};
     const int cgrid = std::max(1, std::min(pick_grid(g, 8), (g->ou.nhubs + 3) / 4));
     for (int L = last; L >= 1; L--) {
       k_backward<NW><<<pick_grid(g, 8), 256, 0, st>>>(eb, b, L);
@@ -1510,6 +1524,10 @@ int run_persource_bc(gu_graph *g, const int *sources, int nsrc, const HostRows &
     GU_HIP(dalloc(&p.sscratch, (size_t)kPsCap * n));
     p.bytes += (size_t)kPsCap * n * 8;
   }
+  if (!p.qscratch) {
+    GU_HIP(dalloc(&p.qscratch, (size_t)kPsCap * n));
+    p.bytes += (size_t)kPsCap * n * 4;
+  }
   const bool rl = g->LR != nullptr;
   bool need_x = rl || hr.delta == nullptr;
   for (int i = 0; !need_x && i < nsrc; i++) need_x = hr.delta[i] == nullptr;
@@ -1533,14 +1551,15 @@ int run_persource_bc(gu_graph *g, const int *sources, int nsrc, const HostRows &
       t[512 + i] = (x && !rl) ? (void *)x : (void *)(p.xscratch + (size_t)i * n);
       t[768 + i] = hr.bc ? (void *)hr.bc[b0 + i] : nullptr;
       t[1024 + i] = x;
+      t[1280 + i] = p.qscratch + (size_t)i * n;
     }
     GU_HIP(hipMemcpyAsync(p.srcs, g->srcs_h, sizeof(int) * nb, hipMemcpyHostToDevice, st));
     GU_HIP(hipMemcpyAsync(g->rowtab, t, sizeof(void *) * kRowTab, hipMemcpyHostToDevice, st));
     if (b0 == 0) k_fill_i32<<<1, 64, 0, st>>>(p.ovf, 1, 0);
     const int4 *LE = rl ? g->LE : nullptr;
     PsBcArgs a{rl ? g->LR : g->R, rl ? g->LC : g->C, LE, g->n, p.srcs, (int *const *)g->rowtab,
-               (u64 *const *)(g->rowtab + 256), (float *const *)(g->rowtab + 512), p.queues, p.depth,
-               p.part, p.ovf};
+               (u64 *const *)(g->rowtab + 256), (float *const *)(g->rowtab + 512),
+               (float *const *)(g->rowtab + 1280), p.queues, p.depth, p.part, p.ovf};
     if (LE && g->directed)
       k_ps_bc<false, true><<<nb, 1024, 0, st>>>(a);
     else if (LE)

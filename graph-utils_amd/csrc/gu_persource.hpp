@@ -430,6 +430,8 @@ struct PsBcArgs {
   int *const *dist;     // [nsrc] scratch rows
   u64 *const *sigma;    // [nsrc] scratch rows
   float *const *delta;  // [nsrc] dependency rows
+  float *const *quot;   // [nsrc] scratch rows: (1 + delta) / sigma of each
+                        //   vertex once its delta is final
   int *stack;           // [nsrc][2n+2]: S, then level starts E (HBM)
   int *depth;
   u64 *part;
@@ -456,6 +458,7 @@ __global__ __launch_bounds__(1024) void k_ps_bc(PsBcArgs a) {
   int *d = a.dist[blockIdx.x];
   u64 *sg = a.sigma[blockIdx.x];
   float *dl = a.delta[blockIdx.x];
+  float *ql = a.quot[blockIdx.x];
   int *S = a.stack + (size_t)blockIdx.x * (2 * (size_t)a.n + 2);
   int *E = S + a.n;
   // frontier entries beyond the LDS capacity are read back from S itself:
@@ -472,10 +475,16 @@ __global__ __launch_bounds__(1024) void k_ps_bc(PsBcArgs a) {
   const int L = ps_levels<true, PULL, true, ELL>(a.R, a.C, a.n, s, d, sg, nullptr, S, E, f, s_cnt, ovf, teps,
                                                  reached, dag, a.LE);
   __syncthreads();
-  // backward, deepest level first (its vertices have no successors): each
-  // thread sums the successors of one vertex in adjacency order, loading a
-  // chunk's distances together, then its successors' path counts and
-  // dependencies together
+  // backward, deepest level first: its vertices have no successors (delta
+  // 0, q = 1 / sigma); then each thread sums the successors of one vertex in
+  // adjacency order, sigma_v * q_w (q_w = (1 + delta_w) / sigma_w, stored
+  // when delta_w became final), loading a chunk's distances together, then
+  // its successors' q together
+  for (int i = E[L] + tid; i >= 0 && L >= 1 && i < E[L + 1]; i += T) {
+    const int v = S[i];
+    ql[v] = (1.0f + 0.0f) / (float)ld_l2(&sg[v]);
+  }
+  __syncthreads();
   for (int l = L - 1; l >= 1; l--) {
     const int b0 = E[l], b1 = E[l + 1];
     for (int i = b0 + tid; i < b1; i += T) {
@@ -488,21 +497,20 @@ __global__ __launch_bounds__(1024) void k_ps_bc(PsBcArgs a) {
       for (; r < re; r += B) {
         int w[B];
         bool on[B];
-        float sw[B], xw[B];
+        float qw[B];
 #pragma unroll
         for (int j = 0; j < B; j++) w[j] = r + j < re ? a.C[r + j] : -1;
 #pragma unroll
         for (int j = 0; j < B; j++) on[j] = w[j] >= 0 && ld_l2(&d[w[j]]) == l + 1;
 #pragma unroll
-        for (int j = 0; j < B; j++) {
-          sw[j] = on[j] ? (float)ld_l2(&sg[w[j]]) : 1.0f;
-          xw[j] = on[j] ? dl[w[j]] : 0.0f;
-        }
+        for (int j = 0; j < B; j++) qw[j] = on[j] ? ql[w[j]] : 0.0f;
 #pragma unroll
         for (int j = 0; j < B; j++)
-          if (on[j]) ps_neumaier(sum, comp, (sv / sw[j]) * (1.0f + xw[j]));
+          if (on[j]) ps_neumaier(sum, comp, sv * qw[j]);
       }
-      dl[v] = sum + comp;
+      const float dv = sum + comp;
+      dl[v] = dv;
+      ql[v] = (1.0f + dv) / sv;
     }
     __syncthreads();
   }
