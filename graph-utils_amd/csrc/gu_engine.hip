@@ -431,6 +431,63 @@ __global__ __launch_bounds__(256) void k_lbs_expand(const int *scan, long long S
   }
 }
 
+// Load-balanced search for up to kLbsSmall segments in one launch: every
+// workgroup scans the counts itself in LDS (a few KB, read from L2), so the
+// three scan launches and their dependency disappear; workgroup 0 also
+// writes the scan and the total.  Larger inputs take the three-phase scan
+// and k_lbs_expand.
+constexpr int kLbsSmall = 8192;
+__global__ __launch_bounds__(256) void k_lbs_small(const int *counts, int S, int *scan, long long cap,
+                                                   int *result, long long *total_out) {
+  __shared__ int s_off[kLbsSmall];
+  __shared__ long long s_wsum[4];
+  __shared__ int s_lo, s_hi;
+  const int tid = threadIdx.x, T = blockDim.x, lane = lane_id(), wid = wave_id();
+  const int per = (S + T - 1) / T;
+  const int i0 = tid * per < S ? tid * per : S;
+  const int i1 = i0 + per < S ? i0 + per : S;
+  for (int i = tid; i < S; i += T) s_off[i] = counts[i];  // coalesced
+  __syncthreads();
+  long long acc = 0;
+  for (int i = i0; i < i1; i++) acc += s_off[i];
+  long long inc = acc;
+  for (int o = 1; o < kWave; o <<= 1) {
+    const long long y = __shfl_up(inc, o, kWave);
+    if (lane >= o) inc += y;
+  }
+  if (lane == kWave - 1) s_wsum[wid] = inc;
+  __syncthreads();
+  long long run = inc - acc, W = 0;
+  for (int w = 0; w < T / kWave; w++) {
+    if (w < wid) run += s_wsum[w];
+    W += s_wsum[w];
+  }
+  for (int i = i0; i < i1; i++) {
+    const int c = s_off[i];
+    s_off[i] = (int)run;  // prefixes fit in int whenever W < INT_MAX
+    run += c;
+  }
+  __syncthreads();
+  if (blockIdx.x == 0) {
+    for (int i = tid; i < S; i += T) scan[i] = s_off[i];
+    if (tid == 0) *total_out = W;
+  }
+  if (W > cap || W >= INT_MAX) return;  // reported as a capacity error by the host
+  const long long TE = (long long)T * kLbsIPT;
+  for (long long tile = blockIdx.x; tile * TE < W; tile += gridDim.x) {
+    const long long e0 = tile * TE;
+    const long long e1 = e0 + TE < W ? e0 + TE : W;
+    if (tid == 0) {
+      s_lo = seg_search(s_off, 0, S - 1, e0);
+      s_hi = seg_search(s_off, s_lo, S - 1, e1 - 1);
+    }
+    __syncthreads();
+    const int lo = s_lo, hi = s_hi;
+    for (long long k = e0 + tid; k < e1; k += T) result[k] = lo == hi ? lo : seg_search(s_off, lo, hi, k);
+    __syncthreads();
+  }
+}
+
 // ---- outputs ----------------------------------------------------------------
 // Transpose one vertex-major per-lane array of the batch ([n][64*NW]) into the
 // caller's source-major rows ([nb][n]); lanes that did not reach v get
@@ -2360,10 +2417,19 @@ int gu_lbs(const int32_t *counts, int64_t S, int32_t *scan, int32_t *result, int
   GU_HIP(hipEventCreate(&ev.b));
   GU_HIP(hipEventRecord(ev.a, st));
   const int grid = std::max(1, cus * 8);
-  k_cscan_reduce<<<grid, T, 0, st>>>(counts, S, tsum.p);
-  k_cscan_tiles<<<1, T, 0, st>>>(tsum.p, nt, total_d);
-  k_cscan_apply<<<grid, T, 0, st>>>(counts, S, tsum.p, scan, total_d);
-  if (result) k_lbs_expand<<<grid, 256, 0, st>>>(scan, S, total_d, result_cap, result);
+  if (S <= kLbsSmall) {
+    // without a result buffer only the scan and the total are wanted
+    // 4 workgroups per CU: 2^26 items in Zipf segments ran at 1104 G items/s
+    // against 1023 with 8 per CU (2, 3, 6 and 16 per CU, and 32 or 64 items
+    // per thread, were slower still)
+    k_lbs_small<<<result ? std::max(1, cus * 4) : 1, 256, 0, st>>>(counts, (int)S, scan,
+                                                                    result ? result_cap : -1, result, total_d);
+  } else {
+    k_cscan_reduce<<<grid, T, 0, st>>>(counts, S, tsum.p);
+    k_cscan_tiles<<<1, T, 0, st>>>(tsum.p, nt, total_d);
+    k_cscan_apply<<<grid, T, 0, st>>>(counts, S, tsum.p, scan, total_d);
+    if (result) k_lbs_expand<<<grid, 256, 0, st>>>(scan, S, total_d, result_cap, result);
+  }
   GU_HIP(hipGetLastError());
   GU_HIP(hipEventRecord(ev.b, st));
   long long W = 0;

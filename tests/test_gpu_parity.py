@@ -328,6 +328,35 @@ def test_lbs_random_with_zero_segments(gu, orc, gpu, seed):
     assert np.array_equal(res, orc.lbs(oW, oscan))
 
 
+@pytest.mark.parametrize("S", [8191, 8192, 8193])
+def test_lbs_single_launch_boundary(gu, orc, gpu, S):
+    """Up to 8192 segments one launch scans and expands; one more takes the
+    three-phase scan.  Both sides of the boundary give the oracle's output."""
+    rng = np.random.default_rng(S)
+    c = rng.integers(0, 40, size=S).astype(np.int32)
+    c[rng.random(S) < 0.2] = 0
+    announce("lbs boundary S=%d" % S)
+    scan, res, W = gu.lbs(c)
+    oscan, oW = orc.exclusive_scan(c)
+    assert W == oW and np.array_equal(scan, oscan)
+    assert np.array_equal(res, orc.lbs(oW, oscan))
+
+
+def test_lbs_total_overflow_is_capacity_error(gu, gpu):
+    """A total of 2^31 + 2^30 items exceeds the int32 output index: refused
+    with a capacity error and nothing written."""
+    c = np.array([1 << 30, 1 << 30, 1 << 30], dtype=np.int32)
+    cb = gu.DeviceBuffer.from_array(c)
+    sb = gu.DeviceBuffer(4 * 3)
+    rb = gu.DeviceBuffer(4 * 16)
+    rb.zero()
+    announce("lbs total overflow")
+    with pytest.raises(gu.GuError) as ei:
+        gu.lbs_device(cb, 3, sb, rb, 16)
+    assert ei.value.status == gu.ERR_CAPACITY
+    assert np.all(rb.to_host(np.int32) == 0)
+
+
 def test_lbs_zipf_2_22(gu, orc, gpu):
     c = gu.zipf_counts(1.1, 1 << 20, 1 << 22, seed=9)
     announce("lbs zipf W=2^22 S=%d" % len(c))
