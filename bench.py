@@ -228,7 +228,7 @@ def bench_bc(args, gu, props):
                           "frac": round(alg / (st.device_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
                           "traffic": profile_traffic("bc_rmat22_1024src_traffic.txt"),
                           "scope": "one 1024-source Brandes call at the best width (forward, "
-                                   "backward and FP64 reduce kernels); traffic measured at width 128",
+                                   "backward and FP64 reduce kernels); traffic measured at width 256",
                           "algorithmic_bytes_per_step": alg, "pass_ms": round(st.device_ms, 3)}}
     if not args.no_cpu:
         from oracle import oracle as orc
