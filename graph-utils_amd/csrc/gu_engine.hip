@@ -1073,10 +1073,14 @@ const std::vector<int32_t> &nof_host(const gu_graph *g) { return g->root ? g->ro
 // Work-unit shape: a vertex of degree above kHubDeg is a hub whose adjacency
 // is pulled in segments of kSeg entries by separate waves; every other
 // vertex belongs to a block of at most 64 consecutive vertices holding at
-// most kBlockEdges entries (or a single vertex).
+// most kBlockEdges entries (or a single vertex).  Measured on R-MAT s20,
+// 64-source BFS + path counts: blocks of 1024 edges and segments of 2048
+// ran at 501-510 GTEPS against 435 with 2048 / 1024 (block bounds 512, 768
+// and 1280 and segments of 1536-3072 were equal or slower; hubs from 768 or
+// 2048 edges no better); sampled BC on s22 gained 2%.
 constexpr int kHubDeg = 1024;
-constexpr int kSeg = 1024;
-constexpr int kBlockEdges = 2048;
+constexpr int kSeg = 2048;
+constexpr int kBlockEdges = 1024;
 
 int build_units(int n, const int32_t *R, Units *u) {
   std::vector<int2> blocks;
