@@ -763,7 +763,7 @@ __global__ __launch_bounds__(256) void k_backward(Eng<NW> e, Bw b, int L) {
   const int waves = gridDim.x * (blockDim.x / kWave);
   for (int si = blockIdx.x * (blockDim.x / kWave) + wave_id(); si < e.nsegs; si += waves)
     bw_hub_seg<NW>(e, b, L, si);
-  constexpr int CH = 16;
+  constexpr int CH = 16;  // stack entries per claim (8 and 32 measured no better)
   while (true) {
     unsigned base = 0;
     if (lane == 0) base = atomicAdd(&b.bwork[L], (unsigned)CH);
@@ -1359,7 +1359,11 @@ int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const g
   }
   k_seed<NW><<<1, 256, 0, st>>>(e);
   const int block = cfg.wg_size > 0 ? cfg.wg_size : 256;
-  const int grid = std::min(kAcctMax, pick_grid(g, block >= 1024 ? 2 : (block >= 512 ? 4 : 8)));
+  // 64 waves per compute unit, twice what is resident at once: waves take
+  // their blocks statically, so the second round of workgroups soaks up the
+  // first round's imbalance (s20 BFS: 542 GTEPS against 510 with 32 waves,
+  // 455 with 16, 521 with 128)
+  const int grid = std::min(kAcctMax, pick_grid(g, std::max(1, 64 * kWave / block)));
   auto set_grids = [&](Eng<NW> &x) {
     x.grid_k[0] = grid;
     x.grid_k[1] = std::min(kAcctMax, pick_grid(g, 8));
@@ -1428,7 +1432,8 @@ int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const g
     Bw b{w.dl, w.ql, w.btag, w.blm, w.bwork, w.part, g->ou.seg0};
     const int cgrid = std::max(1, std::min(pick_grid(g, 8), (g->ou.nhubs + 3) / 4));
     for (int L = last; L >= 1; L--) {
-      k_backward<NW><<<pick_grid(g, 8), 256, 0, st>>>(eb, b, L);
+      // 16 workgroups per compute unit: s22 BC 305 GTEPS against 292 with 8
+      k_backward<NW><<<pick_grid(g, 16), 256, 0, st>>>(eb, b, L);
       if (g->ou.nhubs) k_bw_hub_combine<NW><<<cgrid, 256, 0, st>>>(eb, b, L);
     }
     k_bc_out<NW><<<pick_grid(g, 8), 256, 0, st>>>(e, w.dl, out.delta, out.bc);
