@@ -476,14 +476,14 @@ def test_persource_directed_degree4(gu, orc, gpu):
     _ps_compare(gu, orc, hg, hg.sample_sources(16, seed=79), "directed-deg4-d", sigma=False)
 
 
-def _binary_tree(gu, depth):
-    """Complete binary tree with 2^depth - 1 vertices (heap numbering): the
-    deepest level holds 2^(depth-1) vertices, more than the per-source LDS
-    frontier, so the HBM spill path runs."""
-    n = (1 << depth) - 1
+def _kary_tree(gu, k, depth):
+    """Complete k-ary tree of the given depth (heap numbering): its deepest
+    level holds k^(depth-1) vertices, more than the per-source LDS frontier,
+    so the HBM spill path runs."""
+    n = (k ** depth - 1) // (k - 1)
     adj = [[] for _ in range(n)]
     for v in range(1, n):
-        p = (v - 1) // 2
+        p = (v - 1) // k
         adj[p].append(v)
         adj[v].append(p)
     R = np.zeros(n + 1, dtype=np.int32)
@@ -492,11 +492,15 @@ def _binary_tree(gu, depth):
     return gu.HostGraph(R, C)
 
 
-def test_persource_tree_frontier_spill(gu, orc, gpu):
-    hg = _binary_tree(gu, 13)  # 4096 leaves > 3072 LDS frontier entries
+@pytest.mark.parametrize("k,depth", [(2, 15), (4, 7)])
+def test_persource_tree_frontier_spill(gu, orc, gpu, k, depth):
+    """Binary tree (degree-4 neighbour table): 16384 leaves; 4-ary tree
+    (degree 5, adjacency bounds in the frontier): 4096.  Both exceed the
+    3072-entry LDS frontier."""
+    hg = _kary_tree(gu, k, depth)
     src = np.array([0, 1, 2, hg.n - 1, hg.n // 2, 77], dtype=np.int32)
-    _ps_compare(gu, orc, hg, src, "tree13")
-    _check_bc(gu, orc, hg, src, 64, "tree13", engine=PS)
+    _ps_compare(gu, orc, hg, src, "tree%d-%d" % (k, depth))
+    _check_bc(gu, orc, hg, src, 64, "tree%d-%d" % (k, depth), engine=PS)
 
 
 def test_persource_distances_only(gu, orc, gpu):
