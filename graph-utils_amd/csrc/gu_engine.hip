@@ -647,9 +647,7 @@ __device__ __forceinline__ void neumaier(float &sum, float &comp, float x) {
 
 // Accumulate the dependency terms of v's neighbours in [r0, r1) for lanes mk
 // (sv = float(sigma_b(v)) per lane).  Up to D successor rows are loaded
-// before any is used (16 for one mask word, 4 for more: measured best on
-// R-MAT s21 at widths 64 and 128); the terms are still added in adjacency
-// order.
+// before any is used; the terms are still added in adjacency order.
 template <int NW>
 __device__ __forceinline__ void bw_accum(const Eng<NW> &e, const Bw &b, int L, const u64 *mk,
                                          const float *sv, int r0, int r1, float *sum, float *comp) {
@@ -671,7 +669,10 @@ __device__ __forceinline__ void bw_accum(const Eng<NW> &e, const Bw &b, int L, c
     }
     uint64_t pm = ballot(has);
     while (pm) {
-      constexpr int D = NW == 1 ? 16 : 4;  // 8 for 128 lanes measured 4% slower
+      // successor rows in flight, measured on s22 BC: 16 for 64 lanes (32 no
+      // better), 2 for 128 (4: -3%, 8: -8%), 1 for 256 (2: -2%, 4: -5%):
+      // wider lanes hold more registers per row, so fewer rows keep occupancy
+      constexpr int D = NW == 1 ? 16 : (NW == 2 ? 2 : 1);
       int k[D];
 #pragma unroll
       for (int u = 0; u < D; u++) {
