@@ -640,9 +640,9 @@ __device__ void pull_group(const Eng<NW> &e, int L, int w_l, const u64 *tgt_l, i
                            GroupLds<NW> &g, Pusher &pb, Acc &a, int aux_l = 0, bool seg_bu = false) {
   constexpr bool BU = MODE == PM_BU;
   constexpr bool OR_FOUND = MODE != PM_TD;
-  // predecessor rows in flight, measured on s20 BFS: 64 lanes 12 (8: -1.7%,
-  // 16: +0.3% at 80 VGPRs), 128 lanes 8 (4: -5%, 12: -2%), 256 lanes 8
-  // (2: -7%, 4: -2%)
+  // predecessor rows in flight, measured on s20 BFS: 64 lanes 12 (8: -0.5%,
+  // 16: equal at 80 VGPRs), 128 lanes 8 (4: -2%, 12: -4%), 256 lanes 8
+  // (4: -0.5%, 12: -45%)
   constexpr int D = NW == 1 ? 12 : 8;
   const int lane = lane_id();
   const int inc = wave_inclusive_sum(deg_l);
