@@ -611,10 +611,13 @@ template <int NW>
 struct GroupLds {
   u64 tgt[kWave][NW];
   u64 found[kWave][NW];
+  // visit mask of an owner vertex, loaded with the group (up to two mask
+  // words: for four the larger LDS cost 5% of occupancy-bound throughput)
+  u64 vis[kWave][NW <= 2 ? NW : 1];
   int off[kWave + 1];
   int w[kWave];
   int r0[kWave];
-  int aux[kWave];  // hub index of a segment owner
+  int aux[kWave];   // hub index of a segment owner; out-degree of a vertex owner
 };
 
 // What a group pull does with a finished owner.
@@ -635,9 +638,14 @@ __device__ __forceinline__ int owner_of(const int *off, int t) {
   return a;
 }
 
+// A vertex owner's out-degree (aux_l) and, for up to two mask words, its
+// visit mask (vis_l) are loaded by the caller for all 64 owners at once:
+// only the owner's own commit writes the mask during the level, so the
+// commits need no dependent load of their own (+1.9% per s20 pass).
 template <int NW, int MODE>
 __device__ void pull_group(const Eng<NW> &e, int L, int w_l, const u64 *tgt_l, int r0_l, int deg_l,
-                           GroupLds<NW> &g, Pusher &pb, Acc &a, int aux_l = 0, bool seg_bu = false) {
+                           GroupLds<NW> &g, Pusher &pb, Acc &a, int aux_l = 0, bool seg_bu = false,
+                           const u64 *vis_l = nullptr) {
   constexpr bool BU = MODE == PM_BU;
   constexpr bool OR_FOUND = MODE != PM_TD;
   // predecessor rows in flight, measured on s20 BFS: 64 lanes 12 (8: -0.5%,
@@ -657,6 +665,7 @@ __device__ void pull_group(const Eng<NW> &e, int L, int w_l, const u64 *tgt_l, i
   for (int j = 0; j < NW; j++) {
     g.tgt[lane][j] = tgt_l[j];
     g.found[lane][j] = 0;
+    if constexpr (MODE != PM_SEG && NW <= 2) g.vis[lane][j] = vis_l[j];
   }
   __builtin_amdgcn_wave_barrier();
   const u64 *fc = frs(e, L);
@@ -689,14 +698,17 @@ __device__ void pull_group(const Eng<NW> &e, int L, int w_l, const u64 *tgt_l, i
 #pragma unroll
     for (int j = 0; j < NW; j++) {
       nw[j] = BU ? g.found[slot][j] : g.tgt[slot][j];
-      vo[j] = e.visit[(size_t)w * NW + j];
+      if constexpr (NW <= 2)
+        vo[j] = g.vis[slot][j];
+      else
+        vo[j] = e.visit[(size_t)w * NW + j];
     }
     commit<NW, true>(e, L, w, vo, nw, acc);
     if (BU && lane == 0) {
 #pragma unroll
       for (int j = 0; j < NW; j++) fn[(size_t)w * NW + j] = nw[j];
     }
-    const u64 deg = (u64)(e.R[w + 1] - e.R[w]);
+    const u64 deg = (u64)g.aux[slot];
     a.fe += deg;
     if (full_mask<NW>(e, vo, nw)) a.ud += deg;
     disc |= 1ull << slot;
@@ -817,16 +829,18 @@ __device__ void bu_level(const Eng<NW> &e, int L, Pusher &pb, GroupLds<NW> &g, A
     if (SIGMA) {
       const bool have = lane < nblk;
       const int w = blk.x + (have ? lane : 0);
-      u64 tgt[NW];
+      u64 tgt[NW], vis[NW];
       bool any = false;
 #pragma unroll
       for (int j = 0; j < NW; j++) {
-        tgt[j] = have ? (e.all[j] & ~e.visit[(size_t)w * NW + j]) : 0ull;
+        vis[j] = have ? e.visit[(size_t)w * NW + j] : 0ull;
+        tgt[j] = have ? (e.all[j] & ~vis[j]) : 0ull;
         any |= tgt[j] != 0;
       }
       const int r0 = have ? e.RP[w] : 0;
       const int deg = any ? e.RP[w + 1] - r0 : 0;
-      pull_group<NW, PM_BU>(e, L, w, tgt, r0, deg, g, pb, a);
+      const int odeg = any ? e.R[w + 1] - e.R[w] : 0;
+      pull_group<NW, PM_BU>(e, L, w, tgt, r0, deg, g, pb, a, odeg, false, vis);
       continue;
     }
     // distances only: one vertex at a time, stopping as soon as every
@@ -892,7 +906,11 @@ __device__ void td_finalize(const Eng<NW> &e, int L, GroupLds<NW> &g, Acc &a) {
     // path (its edge from the source) in each lane that reached it: no pull
     const bool unit = SIGMA && L == 0 && e.simple;
     if (SIGMA && !unit) {
-      pull_group<NW, PM_TD>(e, L, w, nw, r0, hub ? 0 : deg, g, none, a);
+      u64 vis[NW];
+#pragma unroll
+      for (int j = 0; j < NW; j++) vis[j] = (have && !hub) ? e.visit[(size_t)w * NW + j] : 0ull;
+      const int odeg = (have && !hub) ? e.R[w + 1] - e.R[w] : 0;
+      pull_group<NW, PM_TD>(e, L, w, nw, r0, hub ? 0 : deg, g, none, a, odeg, false, vis);
       continue;
     }
     u64 one[NW];
