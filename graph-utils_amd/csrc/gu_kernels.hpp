@@ -916,7 +916,14 @@ __device__ void td_finalize(const Eng<NW> &e, int L, GroupLds<NW> &g, Acc &a) {
     u64 one[NW];
 #pragma unroll
     for (int j = 0; j < NW; j++) one[j] = 1;
-    uint64_t todo = ballot(have && !hub);
+    // every entry's visit mask and out-degree are loaded at once, lane by
+    // lane; the commits below then only store
+    const bool mine = have && !hub;
+    u64 vis[NW];
+#pragma unroll
+    for (int j = 0; j < NW; j++) vis[j] = mine ? e.visit[(size_t)w * NW + j] : 0ull;
+    const int odeg = mine ? e.R[w + 1] - e.R[w] : 0;
+    uint64_t todo = ballot(mine);
     while (todo) {
       const int k = __ffsll((unsigned long long)todo) - 1;
       todo &= todo - 1;
@@ -924,7 +931,7 @@ __device__ void td_finalize(const Eng<NW> &e, int L, GroupLds<NW> &g, Acc &a) {
       u64 vo[NW], nk[NW];
 #pragma unroll
       for (int j = 0; j < NW; j++) {
-        vo[j] = e.visit[(size_t)wk * NW + j];
+        vo[j] = readlane64(vis[j], k);
         nk[j] = readlane64(nw[j], k);
       }
       commit<NW, SIGMA>(e, L, wk, vo, nk, one);
@@ -932,7 +939,7 @@ __device__ void td_finalize(const Eng<NW> &e, int L, GroupLds<NW> &g, Acc &a) {
 #pragma unroll
         for (int j = 0; j < NW; j++) a.dag += (u64)__popcll(nk[j]);
       }
-      const u64 d = (u64)(e.R[wk + 1] - e.R[wk]);
+      const u64 d = (u64)readlane(odeg, k);
       a.fe += d;
       if (full_mask<NW>(e, vo, nk)) a.ud += d;
     }
