@@ -489,10 +489,27 @@ __global__ __launch_bounds__(1024) void k_ps_bc(PsBcArgs a) {
     const int b0 = E[l], b1 = E[l + 1];
     for (int i = b0 + tid; i < b1; i += T) {
       const int v = S[i];
-      int r = a.R[v];
-      const int re = a.R[v + 1];
       const float sv = (float)ld_l2(&sg[v]);
       float sum = 0.0f, comp = 0.0f;
+      if (ELL) {  // the neighbours in one load, in adjacency order
+        const int4 q4 = a.LE[v];
+        const int w[4] = {q4.x, q4.y, q4.z, q4.w};
+        bool on[4];
+        float qw[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) on[j] = w[j] >= 0 && ld_l2(&d[w[j]]) == l + 1;
+#pragma unroll
+        for (int j = 0; j < 4; j++) qw[j] = on[j] ? ql[w[j]] : 0.0f;
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          if (on[j]) ps_neumaier(sum, comp, sv * qw[j]);
+        const float dv = sum + comp;
+        dl[v] = dv;
+        ql[v] = (1.0f + dv) / sv;
+        continue;
+      }
+      int r = a.R[v];
+      const int re = a.R[v + 1];
       constexpr int B = 8;
       for (; r < re; r += B) {
         int w[B];
