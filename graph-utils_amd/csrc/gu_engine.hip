@@ -1371,7 +1371,9 @@ int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const g
     x.grid_k[2] = 0;  // accounting slot of a third level kernel (none)
   };
   set_grids(e);
-  int K = cfg.sync_every > 0 ? cfg.sync_every : 2;
+  // first chunk of 3 levels: on R-MAT the poll after level 1 came too early
+  // (s20 BFS and the serving stream about 1% ahead of 2; 4 no better)
+  int K = cfg.sync_every > 0 ? cfg.sync_every : 3;
   // stack mode reserves room for every level of a chunk, so its chunks stay short
   const int Kmax = cfg.sync_every > 0 ? cfg.sync_every : (stack ? 8 : 64);
   int levels_launched = 0;
