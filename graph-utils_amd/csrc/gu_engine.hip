@@ -54,9 +54,11 @@ __global__ void k_validate(const int *R, const int *C, int n, long long m, int *
 
 // ---- batch seeding (shuffle_based.cuh:104-153 per-source initialisation) --
 template <int NW>
-__global__ void k_seed(Eng<NW> e) {
+__global__ __launch_bounds__(256) void k_seed(Eng<NW> e) {
+  __shared__ u64 s_deg[4];
   const int tid = threadIdx.x;
   u64 *f0 = frs(e, 0);
+  u64 deg = 0;
   if (tid < e.nb) {
     int s = e.srcs[tid];
     int j = tid / kWL;
@@ -65,8 +67,16 @@ __global__ void k_seed(Eng<NW> e) {
     atomicOr(&f0[(size_t)s * NW + j], bit);
     e.dvm[(size_t)s * (kWL * NW) + tid] = 0;
     e.svm[(size_t)s * (kWL * NW) + tid] = 1;
+    deg = (u64)(e.R[s + 1] - e.R[s]);
   }
+  // every lane reaches its source: the first reached vertex and TEPS units
+  for (int o = 32; o >= 1; o >>= 1) deg += __shfl_xor(deg, o, kWave);
+  if (lane_id() == 0) s_deg[wave_id()] = deg;
   __syncthreads();
+  if (tid == 0) {
+    e.ctrl->teps = s_deg[0] + s_deg[1] + s_deg[2] + s_deg[3];
+    e.ctrl->reached = (u64)e.nb;
+  }
   if (tid < e.nb) {
     int s = e.srcs[tid];
     if (atomicMax(&e.tag[s], 0) < 0) {  // first lane with this source pushes it
@@ -174,52 +184,49 @@ __global__ void k_zero_hub_rows(Eng<NW> e) {
 // the queues and choose the next level's direction.
 template <int NW>
 __global__ __launch_bounds__(256) void k_control(Eng<NW> e) {
-  __shared__ u64 s_red[4][4];
+  __shared__ u64 s_red[4][kAcctW];
   Ctrl *c = e.ctrl;
   if (c->done) return;
-  u64 t[4] = {0, 0, 0, 0};
+  u64 t[kAcctW] = {};
   // every slot a thread sums is loaded before any is added (8 per step), so
   // the reduction costs two or three L2 round trips, not one per slot
   constexpr int U = 8;
   for (int kid = 0; kid < 3; kid++) {
     const int nk = e.grid_k[kid];
     for (int b0 = threadIdx.x; b0 < nk; b0 += U * blockDim.x) {
-      u64 x[U][4];
+      u64 x[U][kAcctW];
 #pragma unroll
       for (int i = 0; i < U; i++) {
         const int b = b0 + i * blockDim.x;
-        const u64 *slot = e.acct + ((size_t)kid * kAcctMax + (b < nk ? b : 0)) * 4;
+        const u64 *slot = e.acct + ((size_t)kid * kAcctMax + (b < nk ? b : 0)) * kAcctW;
 #pragma unroll
-        for (int c = 0; c < 4; c++) x[i][c] = b < nk ? slot[c] : 0ull;
+        for (int c = 0; c < kAcctW; c++) x[i][c] = b < nk ? slot[c] : 0ull;
       }
 #pragma unroll
-      for (int i = 0; i < U; i++) {
-        t[0] += x[i][0];
-        t[1] += x[i][1];
-        t[2] += x[i][2];
-        t[3] |= x[i][3];
-      }
+      for (int i = 0; i < U; i++)
+#pragma unroll
+        for (int c = 0; c < kAcctW; c++) t[c] = c == 3 ? (t[c] | x[i][c]) : t[c] + x[i][c];
     }
   }
   for (int o = 32; o >= 1; o >>= 1)
-    for (int i = 0; i < 4; i++) {
+    for (int i = 0; i < kAcctW; i++) {
       u64 y = __shfl_xor(t[i], o, kWave);
       t[i] = i == 3 ? (t[i] | y) : t[i] + y;
     }
   if (lane_id() == 0)
-    for (int i = 0; i < 4; i++) s_red[wave_id()][i] = t[i];
+    for (int i = 0; i < kAcctW; i++) s_red[wave_id()][i] = t[i];
   __syncthreads();
   if (threadIdx.x != 0) return;
-  for (int i = 0; i < 3; i++) {
+  for (int i = 0; i < kAcctW; i++) {
     t[i] = 0;
-    for (int w = 0; w < (int)blockDim.x / kWave; w++) t[i] += s_red[w][i];
+    for (int w = 0; w < (int)blockDim.x / kWave; w++) t[i] = i == 3 ? (t[i] | s_red[w][i]) : t[i] + s_red[w][i];
   }
-  t[3] = 0;
-  for (int w = 0; w < (int)blockDim.x / kWave; w++) t[3] |= s_red[w][3];
   const int L = c->level;
   const unsigned qn = c->qlen[slot_of(L + 1)];
   c->unvis_edges -= t[1];
   c->dag += t[2];
+  c->teps += t[4];
+  c->reached += t[5];
   if (t[3]) c->overflow = 1;
   if (c->mode == 0)
     c->td_levels++;
@@ -580,38 +587,6 @@ __global__ __launch_bounds__(256) void k_lane_ecc(Eng<NW> e, int *ecc) {
   for (int b = tid; b < e.nb; b += blockDim.x) {
     const int m = max(max(s_m[0][b], s_m[1][b]), max(s_m[2][b], s_m[3][b]));
     if (m > 0) atomicMax(&ecc[b], m);
-  }
-}
-
-// Sum over lanes of out-degrees (TEPS units) and of reached vertices.
-template <int NW>
-__global__ __launch_bounds__(256) void k_teps(Eng<NW> e) {
-  __shared__ u64 s_a[4], s_b[4];
-  u64 a = 0, b = 0;
-  size_t st = (size_t)gridDim.x * blockDim.x;
-  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < (size_t)e.n; v += st) {
-    int pc = 0;
-    for (int j = 0; j < NW; j++) pc += __popcll(e.visit[v * NW + j]);
-    a += (u64)pc * (u64)(e.R[v + 1] - e.R[v]);
-    b += (u64)pc;
-  }
-  for (int o = 32; o >= 1; o >>= 1) {
-    a += __shfl_xor(a, o, kWave);
-    b += __shfl_xor(b, o, kWave);
-  }
-  if (lane_id() == 0) {
-    s_a[wave_id()] = a;
-    s_b[wave_id()] = b;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    u64 ta = 0, tb = 0;
-    for (int w = 0; w < (int)blockDim.x / kWave; w++) {
-      ta += s_a[w];
-      tb += s_b[w];
-    }
-    atomicAdd(&e.ctrl->teps, ta);
-    atomicAdd(&e.ctrl->reached, tb);
   }
 }
 
@@ -1167,7 +1142,7 @@ int ensure_ws(gu_graph *g, int NW, bool stack, size_t want_S, hipStream_t st) {
     GU_HIP(dalloc(&w.srcs, 256));
     GU_HIP(dalloc(&w.ep, n + 3));
     GU_HIP(dalloc(&w.hub_found, (size_t)std::max(g->pu.nhubs, 1) * NW));
-    GU_HIP(dalloc(&w.acct, (size_t)3 * kAcctMax * 4));
+    GU_HIP(dalloc(&w.acct, (size_t)3 * kAcctMax * kAcctW));
     w.bytes = n * NW * 8 * 4 + n * 4 * 2 + n * WL * 12 + (n + 8) * 8 + 1024 + (n + 3) * 8;
     w.S_cap = 3 * n;
     GU_HIP(dalloc(&w.q, w.S_cap));
@@ -1441,7 +1416,6 @@ int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const g
     }
     k_bc_out<NW><<<pick_grid(g, 8), 256, 0, st>>>(e, w.dl, out.delta, out.bc);
   }
-  k_teps<NW><<<pick_grid(g, 4), 256, 0, st>>>(e);
   GU_HIP(hipGetLastError());
   if (out.ecc) GU_HIP(hipMemcpyAsync(g->ecc_h, out.ecc, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
   GU_HIP(hipMemcpyAsync(g->ctrl_h, g->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));

@@ -139,19 +139,33 @@ __device__ __forceinline__ bool discover(const Eng<NW> &e, int L, const u64 *f, 
 
 // Per-level accounting of one wave: out-degree sum of the vertices it
 // pushed (next frontier edges), of the vertices it completed (every lane
-// reached), shortest-path DAG edges pulled, path-count overflow.  Summed per
-// workgroup into a slot of e.acct (no shared counter), then by k_control.
+// reached), shortest-path DAG edges pulled, path-count overflow, and the
+// (vertex, lane) pairs it committed, plain and weighted by out-degree (the
+// reached count and TEPS units).  Summed per workgroup into a slot of
+// e.acct (no shared counter), then by k_control.
 struct Acc {
-  u64 fe = 0, ud = 0, dag = 0;
+  u64 fe = 0, ud = 0, dag = 0, tp = 0, rc = 0;
   int ovf = 0;
 };
+
+constexpr int kAcctW = 6;  // words per accounting slot
+
+// Lanes newly committed for a vertex of out-degree deg.
+template <int NW>
+__device__ __forceinline__ void acct_commit(Acc &a, const u64 *nw, u64 deg) {
+  u64 pc = 0;
+#pragma unroll
+  for (int j = 0; j < NW; j++) pc += (u64)__popcll(nw[j]);
+  a.rc += pc;
+  a.tp += pc * deg;
+}
 
 constexpr int kAcctMax = 8192;  // largest grid of an accounting kernel
 
 // Whole workgroup: every thread must call it once.
 template <int NW>
 __device__ void write_acct(const Eng<NW> &e, int kid, const Acc &a) {
-  __shared__ u64 s_acc[16][4];
+  __shared__ u64 s_acc[16][kAcctW];
   const int lane = lane_id(), wid = wave_id();
   const bool ovf = ballot(a.ovf != 0) != 0;
   if (lane == 0) {
@@ -159,14 +173,16 @@ __device__ void write_acct(const Eng<NW> &e, int kid, const Acc &a) {
     s_acc[wid][1] = a.ud;
     s_acc[wid][2] = a.dag;
     s_acc[wid][3] = ovf ? 1ull : 0ull;
+    s_acc[wid][4] = a.tp;
+    s_acc[wid][5] = a.rc;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    u64 t[4] = {0, 0, 0, 0};
+    u64 t[kAcctW] = {};
     for (int w = 0; w < (int)blockDim.x / kWave; w++)
-      for (int i = 0; i < 4; i++) t[i] = i == 3 ? (t[i] | s_acc[w][i]) : t[i] + s_acc[w][i];
-    u64 *slot = e.acct + ((size_t)kid * kAcctMax + blockIdx.x) * 4;
-    for (int i = 0; i < 4; i++) slot[i] = t[i];
+      for (int i = 0; i < kAcctW; i++) t[i] = i == 3 ? (t[i] | s_acc[w][i]) : t[i] + s_acc[w][i];
+    u64 *slot = e.acct + ((size_t)kid * kAcctMax + blockIdx.x) * kAcctW;
+    for (int i = 0; i < kAcctW; i++) slot[i] = t[i];
   }
 }
 
@@ -711,6 +727,7 @@ __device__ void pull_group(const Eng<NW> &e, int L, int w_l, const u64 *tgt_l, i
     const u64 deg = (u64)g.aux[slot];
     a.fe += deg;
     if (full_mask<NW>(e, vo, nw)) a.ud += deg;
+    acct_commit<NW>(a, nw, deg);
     disc |= 1ull << slot;
   };
 
@@ -874,6 +891,7 @@ __device__ void bu_level(const Eng<NW> &e, int L, Pusher &pb, GroupLds<NW> &g, A
       const u64 deg = (u64)(e.R[w + 1] - e.R[w]);
       a.fe += deg;
       if (full_mask<NW>(e, vo, found)) a.ud += deg;
+      acct_commit<NW>(a, found, deg);
     }
     pb_push<NW>(e, L, pb, (disc >> lane) & 1ull, blk.x + lane);
   }
@@ -942,6 +960,7 @@ __device__ void td_finalize(const Eng<NW> &e, int L, GroupLds<NW> &g, Acc &a) {
       const u64 d = (u64)readlane(odeg, k);
       a.fe += d;
       if (full_mask<NW>(e, vo, nk)) a.ud += d;
+      acct_commit<NW>(a, nk, d);
     }
   }
 }
@@ -1108,12 +1127,19 @@ __device__ void hub_commit(const Eng<NW> &e, int L, int mode, Pusher &pb, Acc &a
     if (mode) pb_push<NW>(e, L, pb, any, w);
     u64 fe = any ? (u64)(e.R[w + 1] - e.R[w]) : 0ull;
     u64 ud = (any && full_mask<NW>(e, vo, nw)) ? fe : 0ull;
+    Acc hc;  // this lane's hub
+    if (any) acct_commit<NW>(hc, nw, fe);
+    u64 tp = hc.tp, rc = hc.rc;
     for (int o = 32; o >= 1; o >>= 1) {
       fe += __shfl_xor(fe, o, kWave);
       ud += __shfl_xor(ud, o, kWave);
+      tp += __shfl_xor(tp, o, kWave);
+      rc += __shfl_xor(rc, o, kWave);
     }
     a.fe += fe;
     a.ud += ud;
+    a.tp += tp;
+    a.rc += rc;
   }
 }
 

@@ -671,3 +671,29 @@ def test_random_graphs_and_shapes(gu, orc, gpu, seed):
                 assert np.max(np.abs(bc[nz] - ref[nz]) / ref[nz]) <= BC_RTOL
         diam, _ = dg.diameter(src[:32], cfg)
         assert diam == max(orc.eccentricity(hg.R, hg.C, int(x)) for x in src[:32])
+
+
+@pytest.mark.parametrize("graph,engine,sigma", [("rmat12", 1, True), ("rmat12", 1, False), ("rmat12", PS, True),
+                                                ("grid40", 1, True), ("grid40", PS, False),
+                                                ("directed", 1, True), ("directed", PS, True)])
+def test_teps_units_and_reached(gu, orc, gpu, graph, engine, sigma):
+    """The GTEPS numerator: teps_units = sum over sources of the out-degrees
+    of the vertices each reaches (shortest_paths.cu / multi_search_test.cu
+    convention), and reached = sum over sources of reached vertices, as the
+    oracle counts them from its distances."""
+    if graph == "rmat12":
+        hg = gu.HostGraph.rmat(12, 16, seed=12)
+    elif graph == "grid40":
+        hg = gu.HostGraph.grid(40, 0.1, seed=40)
+    else:
+        hg = _directed_graph(gu)
+    src = hg.sample_sources(70, seed=3)
+    announce("teps units %s engine=%d sigma=%d" % (graph, engine, sigma))
+    with gu.DeviceGraph(hg) as dg:
+        d, _, st = dg.multi_search(src, dist=True, sigma=sigma, config=gu.make_config(engine=engine))
+        _, _, stb = dg.betweenness(src, delta=False, config=gu.make_config(engine=engine))
+    units = sum(orc.teps_units(hg.R, orc.sigma(hg.R, hg.C, int(s))[0]) for s in src)
+    reached = int(np.sum(d != np.iinfo(np.int32).max))
+    assert st.engine_used == engine
+    assert st.teps_units == units and st.reached == reached
+    assert stb.teps_units == units and stb.reached == reached
