@@ -53,6 +53,23 @@ __global__ void k_validate(const int *R, const int *C, int n, long long m, int *
 }
 
 // ---- batch seeding (shuffle_based.cuh:104-153 per-source initialisation) --
+// Clears a batch's state in one launch: lane masks and the three frontier
+// slots to 0, claim tags to -1, hub found-masks to 0 and, with path counts,
+// the hubs' rows (hubs take their path counts by atomic adds).
+template <int NW>
+__global__ __launch_bounds__(256) void k_begin(Eng<NW> e, bool sigma) {
+  const size_t n = (size_t)e.n;
+  const size_t st = (size_t)gridDim.x * blockDim.x;
+  const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (size_t i = i0; i < n * NW; i += st) e.visit[i] = 0;
+  for (size_t i = i0; i < 3 * n * NW; i += st) e.fr[i] = 0;
+  for (size_t i = i0; i < n; i += st) e.tag[i] = -1;
+  for (size_t i = i0; i < (size_t)e.nhubs * NW; i += st) e.hub_found[i] = 0;
+  if (sigma)
+    for (size_t i = i0; i < (size_t)e.nhubs * (kWL * NW); i += st)
+      e.svm[(size_t)e.hubs[i / (kWL * NW)] * (kWL * NW) + i % (kWL * NW)] = 0;
+}
+
 template <int NW>
 __global__ __launch_bounds__(256) void k_seed(Eng<NW> e) {
   __shared__ u64 s_deg[4];
@@ -168,16 +185,6 @@ __global__ __launch_bounds__(1024) void k_finalize(Eng<NW> e) {
   hub_commit<NW>(e, L, mode, pb, a);
   pb_flush<NW>(e, L, pb);
   write_acct<NW>(e, 1, a);
-}
-
-// Hubs take their path counts by atomic adds, so their rows start at zero.
-template <int NW>
-__global__ void k_zero_hub_rows(Eng<NW> e) {
-  const int waves = gridDim.x * (blockDim.x / kWave);
-  for (int h = blockIdx.x * (blockDim.x / kWave) + wave_id(); h < e.nhubs; h += waves) {
-    u64 *row = e.svm + (size_t)e.hubs[h] * (kWL * NW);
-    for (int j = 0; j < NW; j++) row[j * kWL + lane_id()] = 0;
-  }
 }
 
 // One workgroup: sum the level's accounting slots, then terminate or rotate
@@ -1316,9 +1323,6 @@ int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const g
   const int fgrid = pick_grid(g, 4);
   std::memcpy(g->srcs_h, srcs, sizeof(int) * nb);
   GU_HIP(hipMemcpyAsync(w.srcs, g->srcs_h, sizeof(int) * nb, hipMemcpyHostToDevice, st));
-  k_fill_u64<<<fgrid, 256, 0, st>>>(w.visit, n * NW, 0ull);
-  k_fill_u64<<<fgrid, 256, 0, st>>>(w.fr, 3 * n * NW, 0ull);
-  k_fill_i32<<<fgrid, 256, 0, st>>>(w.tag, n, -1);
   Ctrl c0{};
   c0.direction = cfg.direction;
   c0.alpha = default_alpha(cfg);
@@ -1329,10 +1333,7 @@ int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const g
   *g->ctrl_h = c0;
   GU_HIP(hipMemcpyAsync(g->ctrl, g->ctrl_h, sizeof(Ctrl), hipMemcpyHostToDevice, st));
   Eng<NW> e = make_eng<NW>(g, nb, stack, sigma);
-  if (g->pu.nhubs) {
-    k_fill_u64<<<1, 256, 0, st>>>(w.hub_found, (size_t)g->pu.nhubs * NW, 0ull);
-    if (sigma) k_zero_hub_rows<NW><<<std::max(1, std::min(fgrid, (g->pu.nhubs + 3) / 4)), 256, 0, st>>>(e);
-  }
+  k_begin<NW><<<fgrid, 256, 0, st>>>(e, sigma);
   k_seed<NW><<<1, 256, 0, st>>>(e);
   const int block = cfg.wg_size > 0 ? cfg.wg_size : 256;
   // 64 waves per compute unit, twice what is resident at once: waves take
