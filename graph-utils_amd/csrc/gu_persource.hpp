@@ -358,7 +358,7 @@ __device__ int ps_levels(const int *R, const int *C, int n, int s, int *d, u64 *
     int *sp_cur = BC ? S + qb + kPsCapLds : spill0 + (size_t)b * n;
     int *sp_next = BC ? S + qs + kPsCapLds : spill0 + (size_t)(b ^ 1) * n;
     if (ELL) {  // one frontier vertex per thread and slice (2 or 3, with
-                // their loads side by side, measured 5-6% slower on a grid)
+                // their loads side by side, measured 8-9% slower on a grid)
       constexpr int NV = 1;
       for (unsigned base = 0; base < qn; base += NV * T) {
         int vv[NV];
