@@ -342,6 +342,41 @@ def test_lbs_single_launch_boundary(gu, orc, gpu, S):
     assert np.array_equal(res, orc.lbs(oW, oscan))
 
 
+class _PtrView:
+    """A device address inside another buffer (for the ctypes entry points)."""
+
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+
+@pytest.mark.parametrize("off", [1, 2, 3])
+def test_lbs_unaligned_result(gu, orc, gpu, off):
+    """A result pointer that is not 16-byte aligned takes one-item stores
+    (aligned ones store four items at a time); the items either side of the
+    result stay untouched."""
+    import ctypes
+
+    rng = np.random.default_rng(40 + off)
+    c = rng.integers(0, 50, size=3000).astype(np.int32)
+    c[rng.random(c.size) < 0.1] = 0
+    W = int(c.sum())
+    announce("lbs unaligned result off=%d W=%d" % (off, W))
+    cb = gu.DeviceBuffer.from_array(c)
+    sb = gu.DeviceBuffer(4 * c.size)
+    rb = gu.DeviceBuffer.from_array(np.full(W + 8, -7, dtype=np.int32))
+    try:
+        view = _PtrView(ctypes.c_void_p(rb.ptr.value + 4 * off))
+        Wd, _ = gu.lbs_device(cb, c.size, sb, view, W)
+        full = rb.to_host(np.int32)
+    finally:
+        for b in (cb, sb, rb):
+            b.close()
+    oscan, oW = orc.exclusive_scan(c)
+    assert Wd == oW
+    assert np.array_equal(full[off:off + W], orc.lbs(oW, oscan))
+    assert (full[:off] == -7).all() and (full[off + W:] == -7).all()
+
+
 def test_lbs_total_overflow_is_capacity_error(gu, gpu):
     """A total of 2^31 + 2^30 items exceeds the int32 output index: refused
     with a capacity error and nothing written."""
