@@ -409,39 +409,67 @@ __global__ __launch_bounds__(1024) void k_cscan_apply(const int *c, long long S,
 }
 
 // load_balanced_search.cu:175-191 as a device-wide merge: result[k] =
-// upper_bound(scan, k) - 1 for k < W.  A workgroup takes kLbsIPT*blockDim
-// consecutive items; the scan entries spanning them are staged in LDS; a
-// tile inside one segment is a plain fill.
-constexpr int kLbsIPT = 16;
-__global__ __launch_bounds__(256) void k_lbs_expand(const int *scan, long long S, const long long *total,
-                                                    long long cap, int *result) {
+// upper_bound(scan, k) - 1 for k < W, in wave tiles of kLbsWT items (64
+// lanes x 4 items x kLbsWR steps, 16-byte stores when the result is 16-byte
+// aligned, no workgroup barrier).  k_lbs_tiles first finds every tile's
+// first segment, one binary search per thread, so tile t spans segments
+// [tlo[t], tlo[t+1]]; a tile inside one segment is a plain fill, and
+// otherwise the wave stages those scan entries in its LDS window.  2^26
+// items in 100,000 Zipf segments: 81.7 us for the five launches, against
+// 103.9 us with workgroup tiles (two barriers each) whose first and last
+// segments one thread searched in the global scan; 10^6 segments: 124
+// against 155 us.
+constexpr int kLbsWR = 4;
+constexpr long long kLbsWT = (long long)kWave * 4 * kLbsWR;
+constexpr int kLbsWin = 512;  // scan entries a wave stages for one tile
+
+__global__ __launch_bounds__(256) void k_lbs_tiles(const int *scan, long long S, const long long *total,
+                                                   long long cap, int *tlo) {
   const long long W = *total;
   if (W > cap || W >= INT_MAX) return;  // reported as a capacity error by the host
-  constexpr int kCap = 256 * kLbsIPT + 1;
-  __shared__ int s_off[kCap];
-  __shared__ int s_lo, s_hi;
-  const int T = blockDim.x, tid = threadIdx.x;
-  const long long TE = (long long)T * kLbsIPT;
-  for (long long tile = blockIdx.x; tile * TE < W; tile += gridDim.x) {
-    const long long e0 = tile * TE;
-    const long long e1 = e0 + TE < W ? e0 + TE : W;
-    if (tid == 0) {
-      s_lo = seg_search(scan, 0, (int)S - 1, e0);
-      s_hi = seg_search(scan, s_lo, (int)S - 1, e1 - 1);
+  const long long nt = (W + kLbsWT - 1) / kLbsWT;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t <= nt; t += stride)
+    tlo[t] = t == nt ? (int)S - 1 : seg_search(scan, 0, (int)S - 1, t * kLbsWT);
+}
+
+__global__ __launch_bounds__(256) void k_lbs_expand(const int *scan, long long S, const long long *total,
+                                                    long long cap, const int *tlo, int *result) {
+  const long long W = *total;
+  if (W > cap || W >= INT_MAX) return;  // reported as a capacity error by the host
+  __shared__ int s_win[256 / kWave][kLbsWin];
+  const int lane = lane_id(), wid = wave_id();
+  int *win = s_win[wid];
+  const int WPB = blockDim.x / kWave;
+  const long long nwave = (long long)gridDim.x * WPB;
+  const bool v4 = (reinterpret_cast<uintptr_t>(result) & 15) == 0;
+  for (long long tile = (long long)blockIdx.x * WPB + wid; tile * kLbsWT < W; tile += nwave) {
+    const long long e0 = tile * kLbsWT;
+    const long long e1 = e0 + kLbsWT < W ? e0 + kLbsWT : W;
+    const int lo = tlo[tile];
+    const int hi = tlo[tile + 1];  // at least the segment of item e1 - 1
+    const bool one = lo == hi || (long long)scan[lo + 1] > e1 - 1;
+    const int nwin = hi - lo + 1;
+    const bool staged = !one && nwin <= kLbsWin;
+    if (staged) {
+      __builtin_amdgcn_wave_barrier();  // the previous tile's window reads are done
+      for (int i = lane; i < nwin; i += kWave) win[i] = scan[lo + i];
+      __builtin_amdgcn_wave_barrier();
     }
-    __syncthreads();
-    const int lo = s_lo, hi = s_hi;
-    if (lo == hi) {
-      for (long long k = e0 + tid; k < e1; k += T) result[k] = lo;
-    } else {
-      const bool in_lds = hi - lo + 1 <= kCap;
-      if (in_lds)
-        for (int i = tid; i <= hi - lo; i += T) s_off[i] = scan[lo + i];
-      __syncthreads();
-      for (long long k = e0 + tid; k < e1; k += T)
-        result[k] = in_lds ? lo + seg_search(s_off, 0, hi - lo, k) : seg_search(scan, lo, hi, k);
+    auto seg = [&](long long k) -> int {
+      if (one) return lo;
+      return staged ? lo + seg_search(win, 0, nwin - 1, k) : seg_search(scan, lo, hi, k);
+    };
+#pragma unroll
+    for (int r = 0; r < kLbsWR; r++) {
+      const long long b = e0 + ((long long)r * kWave + lane) * 4;
+      if (v4 && b + 4 <= e1) {
+        *reinterpret_cast<int4 *>(result + b) = make_int4(seg(b), seg(b + 1), seg(b + 2), seg(b + 3));
+      } else {
+        for (int q = 0; q < 4; q++)
+          if (b + q < e1) result[b + q] = seg(b + q);
+      }
     }
-    __syncthreads();
   }
 }
 
@@ -460,7 +488,6 @@ __global__ __launch_bounds__(256) void k_lbs_expand(const int *scan, long long S
 // (1086), contiguous tile runs per wave (1140), and the last segment
 // searched from the first with a one-read check (1196).
 constexpr int kLbsSmall = 8192;
-constexpr int kLbsWR = 4;
 __global__ __launch_bounds__(256) void k_lbs_small(const int *counts, int S, int *scan, long long cap,
                                                    int *result, long long *total_out) {
   __shared__ int s_off[kLbsSmall];
@@ -496,7 +523,7 @@ __global__ __launch_bounds__(256) void k_lbs_small(const int *counts, int S, int
     if (tid == 0) *total_out = W;
   }
   if (W > cap || W >= INT_MAX) return;  // reported as a capacity error by the host
-  constexpr long long WT = (long long)kWave * 4 * kLbsWR;
+  constexpr long long WT = kLbsWT;
   const int WPB = T / kWave;
   const long long nwave = (long long)gridDim.x * WPB;
   const bool v4 = (reinterpret_cast<uintptr_t>(result) & 15) == 0;
@@ -2425,25 +2452,30 @@ int gu_lbs(const int32_t *counts, int64_t S, int32_t *scan, int32_t *result, int
       if (p) (void)hipFree(p);
     }
   } tsum;
-  GU_HIP(hipMalloc((void **)&tsum.p, sizeof(long long) * (size_t)(nt + 2)));
+  // tile starts of the large-S expansion: W < INT_MAX items in kLbsWT tiles
+  const long long ntl = (result && S > kLbsSmall) ? std::min<long long>(result_cap, INT_MAX) / kLbsWT + 2 : 0;
+  GU_HIP(hipMalloc((void **)&tsum.p, sizeof(long long) * (size_t)(nt + 2) + sizeof(int) * (size_t)ntl));
   long long *total_d = tsum.p + nt + 1;
+  int *tlo = reinterpret_cast<int *>(tsum.p + nt + 2);
   EventPair ev;
   GU_HIP(hipEventCreate(&ev.a));
   GU_HIP(hipEventCreate(&ev.b));
   GU_HIP(hipEventRecord(ev.a, st));
   const int grid = std::max(1, cus * 8);
   if (S <= kLbsSmall) {
-    // without a result buffer only the scan and the total are wanted
-    // 4 workgroups per CU: 2^26 items in Zipf segments ran at 1104 G items/s
-    // against 1023 with 8 per CU (2, 3, 6 and 16 per CU, and 32 or 64 items
-    // per thread, were slower still)
+    // without a result buffer only the scan and the total are wanted; 4
+    // workgroups per CU (2 and 8 were slower: 848 and 1141 against 1230 G
+    // items/s for 2^26 items in 1805 Zipf segments)
     k_lbs_small<<<result ? std::max(1, cus * 4) : 1, 256, 0, st>>>(counts, (int)S, scan,
                                                                     result ? result_cap : -1, result, total_d);
   } else {
     k_cscan_reduce<<<grid, T, 0, st>>>(counts, S, tsum.p);
     k_cscan_tiles<<<1, T, 0, st>>>(tsum.p, nt, total_d);
     k_cscan_apply<<<grid, T, 0, st>>>(counts, S, tsum.p, scan, total_d);
-    if (result) k_lbs_expand<<<grid, 256, 0, st>>>(scan, S, total_d, result_cap, result);
+    if (result) {
+      k_lbs_tiles<<<grid, 256, 0, st>>>(scan, S, total_d, result_cap, tlo);
+      k_lbs_expand<<<std::max(1, cus * 4), 256, 0, st>>>(scan, S, total_d, result_cap, tlo, result);
+    }
   }
   GU_HIP(hipGetLastError());
   GU_HIP(hipEventRecord(ev.b, st));

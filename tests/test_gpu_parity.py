@@ -392,6 +392,24 @@ def test_lbs_total_overflow_is_capacity_error(gu, gpu):
     assert np.all(rb.to_host(np.int32) == 0)
 
 
+@pytest.mark.parametrize("S", [100000, 1000000])
+def test_lbs_many_zipf_segments(gu, orc, gpu, S):
+    """The reference's own LBS example size (100,000 segments) and ten times
+    that: tiles that cross many small segments stage them in LDS, or search
+    the scan when more than the window; most tiles lie in one segment."""
+    rng = np.random.default_rng(S)
+    w = 1.0 / np.arange(1, S + 1) ** 1.1
+    c = np.floor(w / w.sum() * (1 << 22)).astype(np.int64)
+    c[0] += (1 << 22) - c.sum()
+    rng.shuffle(c)
+    c = c.astype(np.int32)
+    announce("lbs zipf S=%d W=%d" % (S, int(c.sum())))
+    scan, res, W = gu.lbs(c)
+    oscan, oW = orc.exclusive_scan(c)
+    assert W == oW and np.array_equal(scan, oscan)
+    assert np.array_equal(res, orc.lbs(oW, oscan))
+
+
 def test_lbs_zipf_2_22(gu, orc, gpu):
     c = gu.zipf_counts(1.1, 1 << 20, 1 << 22, seed=9)
     announce("lbs zipf W=2^22 S=%d" % len(c))
