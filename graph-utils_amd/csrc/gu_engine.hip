@@ -488,10 +488,11 @@ __global__ __launch_bounds__(256) void k_lbs_expand(const int *scan, long long S
 // (1086), contiguous tile runs per wave (1140), and the last segment
 // searched from the first with a one-read check (1196).
 constexpr int kLbsSmall = 8192;
-__global__ __launch_bounds__(256) void k_lbs_small(const int *counts, int S, int *scan, long long cap,
-                                                   int *result, long long *total_out) {
+constexpr int kLbsSmallT = 512;  // threads per workgroup, two per CU
+__global__ __launch_bounds__(kLbsSmallT) void k_lbs_small(const int *counts, int S, int *scan, long long cap,
+                                                          int *result, long long *total_out) {
   __shared__ int s_off[kLbsSmall];
-  __shared__ long long s_wsum[4];
+  __shared__ long long s_wsum[kLbsSmallT / kWave];
   const int tid = threadIdx.x, T = blockDim.x, lane = lane_id(), wid = wave_id();
   const int per = (S + T - 1) / T;
   const int i0 = tid * per < S ? tid * per : S;
@@ -2463,10 +2464,11 @@ int gu_lbs(const int32_t *counts, int64_t S, int32_t *scan, int32_t *result, int
   GU_HIP(hipEventRecord(ev.a, st));
   const int grid = std::max(1, cus * 8);
   if (S <= kLbsSmall) {
-    // without a result buffer only the scan and the total are wanted; 4
-    // workgroups per CU (2 and 8 were slower: 848 and 1141 against 1230 G
-    // items/s for 2^26 items in 1805 Zipf segments)
-    k_lbs_small<<<result ? std::max(1, cus * 4) : 1, 256, 0, st>>>(counts, (int)S, scan,
+    // without a result buffer only the scan and the total are wanted.  Two
+    // workgroups of 512 per CU: 2^26 items in 1805 Zipf segments took 50.1
+    // us, against 52.0 with four of 256 and 50.2 with one of 1024 (2 and 8
+    // workgroups of 256 per CU ran at 848 and 1141 against 1230 G items/s)
+    k_lbs_small<<<result ? std::max(1, cus * 2) : 1, kLbsSmallT, 0, st>>>(counts, (int)S, scan,
                                                                     result ? result_cap : -1, result, total_d);
   } else {
     k_cscan_reduce<<<grid, T, 0, st>>>(counts, S, tsum.p);
