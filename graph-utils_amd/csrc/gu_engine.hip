@@ -696,13 +696,20 @@ __device__ __forceinline__ void bw_accum(const Eng<NW> &e, const Bw &b, int L, c
   for (int e0 = r0; e0 < r1; e0 += kWave) {
     int ed = e0 + lane;
     bool valid = ed < r1;
-    int w = valid ? e.C[ed] : 0;
-    bool ok = valid && tagr[w] == L + 1;
+    // branch-free: lanes past the range re-read entry r0, and every lane
+    // loads its neighbour's tag and lane masks, selected afterwards (under
+    // branches each load waited for the previous one)
+    const int w = e.C[valid ? ed : r0];
+    const int tg = tagr[w];
+    u64 lm[NW];
+#pragma unroll
+    for (int j = 0; j < NW; j++) lm[j] = lmr[(size_t)w * NW + j];
+    const bool ok = valid && tg == L + 1;
     u64 t[NW];
     bool has = false;
 #pragma unroll
     for (int j = 0; j < NW; j++) {
-      t[j] = ok ? (lmr[(size_t)w * NW + j] & mk[j]) : 0ull;
+      t[j] = ok ? (lm[j] & mk[j]) : 0ull;
       has |= t[j] != 0;
     }
     uint64_t pm = ballot(has);
@@ -753,11 +760,17 @@ __device__ __forceinline__ void bw_lanes(const Eng<NW> &e, int v, int L, u64 *mk
   // lanes b with d_b(v) = L, from v's own distance row
   const int lane = lane_id();
   const int *drow = e.dvm + (size_t)v * (kWL * NW);
+  // every word is loaded before any is tested (the short-circuit form put a
+  // wait after each of the 2*NW dependent loads)
+  u64 vis[NW];
+  int dl[NW];
 #pragma unroll
   for (int j = 0; j < NW; j++) {
-    bool on = ((e.visit[(size_t)v * NW + j] >> lane) & 1ull) && drow[j * kWL + lane] == L;
-    mk[j] = ballot(on);
+    vis[j] = e.visit[(size_t)v * NW + j];
+    dl[j] = drow[j * kWL + lane];
   }
+#pragma unroll
+  for (int j = 0; j < NW; j++) mk[j] = ballot(((vis[j] >> lane) & 1ull) && dl[j] == L);
 }
 
 // Hub segments of the backward pass: each wave writes its segment's
@@ -779,7 +792,8 @@ __device__ __forceinline__ void bw_hub_seg(const Eng<NW> &e, const Bw &b, int L,
   for (int j = 0; j < NW; j++) {
     sum[j] = 0.0f;
     comp[j] = 0.0f;
-    sv[j] = ((mk[j] >> lane) & 1ull) ? (float)e.svm[(size_t)v * (kWL * NW) + j * kWL + lane] : 0.0f;
+    const u64 x = e.svm[(size_t)v * (kWL * NW) + j * kWL + lane];  // loaded by every lane
+    sv[j] = ((mk[j] >> lane) & 1ull) ? (float)x : 0.0f;
   }
   bw_accum<NW>(e, b, L, mk, sv, sg.y, sg.z, sum, comp);
 #pragma unroll
@@ -821,7 +835,8 @@ __global__ __launch_bounds__(256) void k_backward(Eng<NW> e, Bw b, int L) {
         mk[j] = e.smask[(size_t)pos * NW + j];
         sum[j] = 0.0f;
         comp[j] = 0.0f;
-        sv[j] = ((mk[j] >> lane) & 1ull) ? (float)e.svm[(size_t)v * (kWL * NW) + j * kWL + lane] : 0.0f;
+        const u64 x = e.svm[(size_t)v * (kWL * NW) + j * kWL + lane];  // loaded by every lane
+        sv[j] = ((mk[j] >> lane) & 1ull) ? (float)x : 0.0f;
       }
       bw_accum<NW>(e, b, L, mk, sv, r0, r1, sum, comp);
       const size_t vrow = (size_t)v * (kWL * NW);
