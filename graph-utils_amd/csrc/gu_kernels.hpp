@@ -846,17 +846,24 @@ __device__ void bu_level(const Eng<NW> &e, int L, Pusher &pb, GroupLds<NW> &g, A
     if (SIGMA) {
       const bool have = lane < nblk;
       const int w = blk.x + (have ? lane : 0);
+      // every lane loads (w is a vertex of the block either way) and the
+      // values are selected afterwards: under branches the row offsets
+      // waited for the visit masks
+      u64 vraw[NW];
+#pragma unroll
+      for (int j = 0; j < NW; j++) vraw[j] = e.visit[(size_t)w * NW + j];
+      const int rp0 = e.RP[w], rp1 = e.RP[w + 1], ro0 = e.R[w], ro1 = e.R[w + 1];
       u64 tgt[NW], vis[NW];
       bool any = false;
 #pragma unroll
       for (int j = 0; j < NW; j++) {
-        vis[j] = have ? e.visit[(size_t)w * NW + j] : 0ull;
+        vis[j] = have ? vraw[j] : 0ull;
         tgt[j] = have ? (e.all[j] & ~vis[j]) : 0ull;
         any |= tgt[j] != 0;
       }
-      const int r0 = have ? e.RP[w] : 0;
-      const int deg = any ? e.RP[w + 1] - r0 : 0;
-      const int odeg = any ? e.R[w + 1] - e.R[w] : 0;
+      const int r0 = have ? rp0 : 0;
+      const int deg = any ? rp1 - r0 : 0;
+      const int odeg = any ? ro1 - ro0 : 0;
       pull_group<NW, PM_BU>(e, L, w, tgt, r0, deg, g, pb, a, odeg, false, vis);
       continue;
     }
