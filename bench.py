@@ -598,7 +598,7 @@ def main():
                 vc = gu.make_config(name)
                 dg.multi_search_device(batches[0], dist, sig, vc)
                 u, ms = 0, 0.0
-                for i in range(args.warmup, min(nb, args.warmup + 3)):
+                for i in range(args.warmup, nb):  # the timed batches of the line itself
                     st = dg.multi_search_device(batches[i], dist, sig, vc)
                     u += st.teps_units
                     ms += st.device_ms
