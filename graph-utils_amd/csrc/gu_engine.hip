@@ -569,15 +569,7 @@ __global__ __launch_bounds__(kLbsSmallT) void k_lbs_small(const int *counts, int
 // Loading each value only after its vertex's lane mask, and storing it only
 // after loading its row pointer, ran at 3.3 TB/s; with the three loads side
 // by side a standalone probe of this tile moves 5.5 TB/s.
-#ifndef GU_TR_NT
-#define GU_TR_NT 1
-#endif
-#ifndef GU_TR_NTL
-#define GU_TR_NTL 1
-#endif
-#ifndef GU_TR_GRID
-#define GU_TR_GRID 8
-#endif
+constexpr int kTransposeWgPerCu = 8;
 template <int NW, class T>
 __global__ __launch_bounds__(256) void k_transpose_out(Eng<NW> e, const T *src, T *const *rows, T fill) {
   __shared__ T tile[64][65];
@@ -598,11 +590,7 @@ __global__ __launch_bounds__(256) void k_transpose_out(Eng<NW> e, const T *src, 
     for (int k = 0; k < 16; k++) {
       const long long v = v0 + row0 + 4 * k;
       const bool in = v < e.n;
-#if GU_TR_NTL
       x[k] = in ? __builtin_nontemporal_load(&src[(size_t)v * (kWL * NW) + j * kWL + col]) : fill;
-#else
-      x[k] = in ? src[(size_t)v * (kWL * NW) + j * kWL + col] : fill;
-#endif
       vm[k] = in ? e.visit[(size_t)v * NW + j] : 0ull;
       const int b = j * kWL + row0 + 4 * k;
       rp[k] = b < e.nb ? rows[b] : nullptr;
@@ -615,11 +603,7 @@ __global__ __launch_bounds__(256) void k_transpose_out(Eng<NW> e, const T *src, 
 #pragma unroll
       for (int k = 0; k < 16; k++) {
         T *r = rp[k];
-#if GU_TR_NT
         if (r) __builtin_nontemporal_store(tile[col][row0 + 4 * k], &r[v]);
-#else
-        if (r) r[v] = tile[col][row0 + 4 * k];
-#endif
       }
     }
     __syncthreads();
@@ -1462,7 +1446,7 @@ int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const g
   if (job == JOB_BFS && (out.dist || out.sigma))
   {
     const long long tiles = ((long long)g->n + 63) / 64 * ((nb + kWL - 1) / kWL);
-    const int og = (int)std::max(1LL, std::min<long long>(tiles, pick_grid(g, GU_TR_GRID)));
+    const int og = (int)std::max(1LL, std::min<long long>(tiles, pick_grid(g, kTransposeWgPerCu)));
     if (out.dist) k_transpose_out<NW, int><<<og, 256, 0, st>>>(e, w.dvm, out.dist, INT_MAX);
     if (out.sigma) k_transpose_out<NW, u64><<<og, 256, 0, st>>>(e, w.svm, out.sigma, 0ull);
   }
