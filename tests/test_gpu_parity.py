@@ -349,18 +349,20 @@ class _PtrView:
         self.ptr = ptr
 
 
+@pytest.mark.parametrize("S", [3000, 20000])
 @pytest.mark.parametrize("off", [1, 2, 3])
-def test_lbs_unaligned_result(gu, orc, gpu, off):
+def test_lbs_unaligned_result(gu, orc, gpu, off, S):
     """A result pointer that is not 16-byte aligned takes one-item stores
-    (aligned ones store four items at a time); the items either side of the
-    result stay untouched."""
+    (aligned ones store four items at a time), in the one-launch form and
+    beyond 8192 segments; the items either side of the result stay
+    untouched."""
     import ctypes
 
-    rng = np.random.default_rng(40 + off)
-    c = rng.integers(0, 50, size=3000).astype(np.int32)
+    rng = np.random.default_rng(40 + off + S)
+    c = rng.integers(0, 50, size=S).astype(np.int32)
     c[rng.random(c.size) < 0.1] = 0
     W = int(c.sum())
-    announce("lbs unaligned result off=%d W=%d" % (off, W))
+    announce("lbs unaligned result S=%d off=%d W=%d" % (S, off, W))
     cb = gu.DeviceBuffer.from_array(c)
     sb = gu.DeviceBuffer(4 * c.size)
     rb = gu.DeviceBuffer.from_array(np.full(W + 8, -7, dtype=np.int32))
