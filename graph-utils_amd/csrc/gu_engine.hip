@@ -479,14 +479,33 @@ __global__ __launch_bounds__(256) void k_lbs_expand(const int *scan, long long S
 // writes the scan and the total.  Larger inputs take the three-phase scan
 // and k_lbs_expand.
 // The expansion is a write stream: each wave takes tiles of 64 lanes x 4
-// items x kLbsWR steps with no workgroup barrier, lane 0 finds the tile's
-// first and last segment, and lanes store 4 items at a time (16 bytes when
+// items x kLbsWR steps with no workgroup barrier, finds the tile's first
+// and last segment by a wave search, and lanes store 4 items at a time (16 bytes when
 // the result is 16-byte aligned).  Against workgroup tiles with two
 // barriers and one-item stores, 2^26 items in Zipf segments went from 1090
 // to 1220-1237 G items/s.  Measured and slower: 2 or 8 steps per tile
 // (887, 1154), 2 or 8 workgroups per CU (848, 1141), non-temporal stores
 // (1086), contiguous tile runs per wave (1140), and the last segment
 // searched from the first with a one-read check (1196).
+// Largest i in [lo, lo + n) with off[i] <= x (off nondecreasing, off[lo] <=
+// x), found by the whole wave: 64 evenly spaced samples and a ballot narrow
+// the range 64-fold a step, so 8192 entries take three LDS reads per lane.
+// Every lane returns the same value.
+__device__ __forceinline__ int wave_seg_search(const int *off, int lo, int n, long long x) {
+  const int lane = lane_id();
+  while (n > kWave) {
+    const int step = (n + kWave - 1) / kWave;
+    const int idx = lo + lane * step;
+    const bool le = idx < lo + n && (long long)off[idx] <= x;
+    const int c = __popcll(ballot(le));  // samples at or below x form a prefix
+    const int nlo = lo + (c - 1) * step;
+    n = (lo + n - nlo) < step ? (lo + n - nlo) : step;
+    lo = nlo;
+  }
+  const bool le = lane < n && (long long)off[lo + lane] <= x;
+  return lo + __popcll(ballot(le)) - 1;
+}
+
 constexpr int kLbsSmall = 8192;
 constexpr int kLbsSmallT = 512;  // threads per workgroup, two per CU
 __global__ __launch_bounds__(kLbsSmallT) void k_lbs_small(const int *counts, int S, int *scan, long long cap,
@@ -531,13 +550,10 @@ __global__ __launch_bounds__(kLbsSmallT) void k_lbs_small(const int *counts, int
   for (long long tile = (long long)blockIdx.x * WPB + wid; tile * WT < W; tile += nwave) {
     const long long e0 = tile * WT;
     const long long e1 = e0 + WT < W ? e0 + WT : W;
-    int lo = 0, hi = 0;
-    if (lane == 0) {
-      lo = seg_search(s_off, 0, S - 1, e0);
-      hi = seg_search(s_off, lo, S - 1, e1 - 1);
-    }
-    lo = __shfl(lo, 0, kWave);
-    hi = __shfl(hi, 0, kWave);
+    // the tile's first and last segment, by the whole wave (lane 0 alone,
+    // with two binary searches, took 50.2 against 48.6 us per pass)
+    const int lo = wave_seg_search(s_off, 0, S, e0);
+    const int hi = wave_seg_search(s_off, lo, S - lo, e1 - 1);
 #pragma unroll
     for (int r = 0; r < kLbsWR; r++) {
       const long long b = e0 + ((long long)r * kWave + lane) * 4;
