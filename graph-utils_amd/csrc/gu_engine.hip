@@ -409,17 +409,23 @@ __global__ __launch_bounds__(1024) void k_cscan_apply(const int *c, long long S,
 }
 
 // load_balanced_search.cu:175-191 as a device-wide merge: result[k] =
-// upper_bound(scan, k) - 1 for k < W, in wave tiles of kLbsWT items (64
-// lanes x 4 items x kLbsWR steps, 16-byte stores when the result is 16-byte
+// upper_bound(scan, k) - 1 for k < W, in wave tiles of kLbsXT items (64
+// lanes x 4 items x kLbsXR steps, 16-byte stores when the result is 16-byte
 // aligned, no workgroup barrier).  k_lbs_tiles first finds every tile's
 // first segment, one binary search per thread, so tile t spans segments
 // [tlo[t], tlo[t+1]]; a tile inside one segment is a plain fill, and
 // otherwise the wave stages those scan entries in its LDS window.  2^26
-// items in 100,000 Zipf segments: 81.7 us for the five launches, against
+// items in 100,000 Zipf segments: 72.0 us for the five launches, against
 // 103.9 us with workgroup tiles (two barriers each) whose first and last
-// segments one thread searched in the global scan; 10^6 segments: 124
-// against 155 us.
-constexpr int kLbsWR = 4;
+// segments one thread searched in the global scan; 10^6 segments: 103
+// against 155 us.  (A coarse index of the scan in every workgroup's LDS
+// for the tile-start searches was slower: 87 against 81 us.)
+// steps per tile beyond kLbsSmall segments: 100,000 Zipf segments took
+// 72.0 us with 2, against 76.6 with 1 and 81.3 with 4 (10^6 segments: 103,
+// 102 and 124 us)
+constexpr int kLbsXR = 2;
+constexpr long long kLbsXT = (long long)kWave * 4 * kLbsXR;
+constexpr int kLbsWR = 4;  // steps per tile of the one-launch form
 constexpr long long kLbsWT = (long long)kWave * 4 * kLbsWR;
 constexpr int kLbsWin = 512;  // scan entries a wave stages for one tile
 
@@ -427,10 +433,10 @@ __global__ __launch_bounds__(256) void k_lbs_tiles(const int *scan, long long S,
                                                    long long cap, int *tlo) {
   const long long W = *total;
   if (W > cap || W >= INT_MAX) return;  // reported as a capacity error by the host
-  const long long nt = (W + kLbsWT - 1) / kLbsWT;
+  const long long nt = (W + kLbsXT - 1) / kLbsXT;
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t <= nt; t += stride)
-    tlo[t] = t == nt ? (int)S - 1 : seg_search(scan, 0, (int)S - 1, t * kLbsWT);
+    tlo[t] = t == nt ? (int)S - 1 : seg_search(scan, 0, (int)S - 1, t * kLbsXT);
 }
 
 __global__ __launch_bounds__(256) void k_lbs_expand(const int *scan, long long S, const long long *total,
@@ -443,9 +449,9 @@ __global__ __launch_bounds__(256) void k_lbs_expand(const int *scan, long long S
   const int WPB = blockDim.x / kWave;
   const long long nwave = (long long)gridDim.x * WPB;
   const bool v4 = (reinterpret_cast<uintptr_t>(result) & 15) == 0;
-  for (long long tile = (long long)blockIdx.x * WPB + wid; tile * kLbsWT < W; tile += nwave) {
-    const long long e0 = tile * kLbsWT;
-    const long long e1 = e0 + kLbsWT < W ? e0 + kLbsWT : W;
+  for (long long tile = (long long)blockIdx.x * WPB + wid; tile * kLbsXT < W; tile += nwave) {
+    const long long e0 = tile * kLbsXT;
+    const long long e1 = e0 + kLbsXT < W ? e0 + kLbsXT : W;
     const int lo = tlo[tile];
     const int hi = tlo[tile + 1];  // at least the segment of item e1 - 1
     const bool one = lo == hi || (long long)scan[lo + 1] > e1 - 1;
@@ -461,7 +467,7 @@ __global__ __launch_bounds__(256) void k_lbs_expand(const int *scan, long long S
       return staged ? lo + seg_search(win, 0, nwin - 1, k) : seg_search(scan, lo, hi, k);
     };
 #pragma unroll
-    for (int r = 0; r < kLbsWR; r++) {
+    for (int r = 0; r < kLbsXR; r++) {
       const long long b = e0 + ((long long)r * kWave + lane) * 4;
       if (v4 && b + 4 <= e1) {
         *reinterpret_cast<int4 *>(result + b) = make_int4(seg(b), seg(b + 1), seg(b + 2), seg(b + 3));
@@ -2468,8 +2474,8 @@ int gu_lbs(const int32_t *counts, int64_t S, int32_t *scan, int32_t *result, int
       if (p) (void)hipFree(p);
     }
   } tsum;
-  // tile starts of the large-S expansion: W < INT_MAX items in kLbsWT tiles
-  const long long ntl = (result && S > kLbsSmall) ? std::min<long long>(result_cap, INT_MAX) / kLbsWT + 2 : 0;
+  // tile starts of the large-S expansion: W < INT_MAX items in kLbsXT tiles
+  const long long ntl = (result && S > kLbsSmall) ? std::min<long long>(result_cap, INT_MAX) / kLbsXT + 2 : 0;
   GU_HIP(hipMalloc((void **)&tsum.p, sizeof(long long) * (size_t)(nt + 2) + sizeof(int) * (size_t)ntl));
   long long *total_d = tsum.p + nt + 1;
   int *tlo = reinterpret_cast<int *>(tsum.p + nt + 2);
