@@ -17,6 +17,8 @@ Other workloads (one JSON line each, same fields):
     --workload bc     sampled Brandes BC on R-MAT scale 22: 1024 sources per
                       step in batches of 64/128/256 (best reported)
     --workload lbs    load-balanced search over 2^26 items, Zipf(1.1) counts
+                      (--lbs-segments S: over S Zipf-weighted segments, e.g.
+                      the reference example's 100,000)
     --workload serve  query server over R-MAT s18 / s20 and a 1024^2 grid:
                       served vs plain queries/s and p50/p99 latency
 """
@@ -253,7 +255,17 @@ def bench_lbs(args, gu, props):
     """Device-wide load-balanced search: 2^26 work items, Zipf(1.1) segment
     counts capped at 2^20 (int32); a step is scan + search."""
     W = 1 << 26
-    c = gu.zipf_counts(1.1, 1 << 20, W, seed=26)
+    if args.lbs_segments:
+        # a fixed segment count (the reference's lbs.cu example uses 100,000):
+        # 2^26 items spread by Zipf(1.1) weights over S segments, shuffled
+        S = args.lbs_segments
+        w = 1.0 / np.arange(1, S + 1) ** 1.1
+        c = np.floor(w / w.sum() * W).astype(np.int64)
+        c[0] += W - c.sum()
+        np.random.default_rng(26).shuffle(c)
+        c = c.astype(np.int32)
+    else:
+        c = gu.zipf_counts(1.1, 1 << 20, W, seed=26)
     S = len(c)
     cb = gu.DeviceBuffer.from_array(c)
     sb = gu.DeviceBuffer(4 * S)
@@ -272,7 +284,7 @@ def bench_lbs(args, gu, props):
     extra = {"roofline": {"bound": "hbm", "achieved": round(alg / (dev / args.steps / 1e3) / 1e9, 1),
                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": round(alg / (dev / args.steps / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
-                          "traffic": profile_traffic("lbs_zipf_2p26_traffic.txt")},
+                          "traffic": None if args.lbs_segments else profile_traffic("lbs_zipf_2p26_traffic.txt")},
              "device_ms_per_step": round(dev / args.steps, 4), "segments": S}
     if not args.no_cpu:
         from oracle import oracle as orc
@@ -283,7 +295,8 @@ def bench_lbs(args, gu, props):
                                  "scan + merge (lbs.cu:18-31), %.2f s" % secs}
     _line("G items/s load-balanced search, 2^26 items, Zipf(1.1)", round(W / (dev / args.steps / 1e3) / 1e9, 3),
           "G items/s", args, wall / args.steps * 1e3, "int32",
-          {"workload": "2^26 work items in %d Zipf(1.1) segments (cap 2^20)" % S,
+          {"workload": ("2^26 work items in %d Zipf(1.1)-weighted segments" % S if args.lbs_segments
+                        else "2^26 work items in %d Zipf(1.1) segments (cap 2^20)" % S),
            "device": props["name"], "arch": props["arch"]}, extra)
 
 
@@ -518,6 +531,9 @@ def main():
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--lbs-segments", type=int, default=0,
+                    help="--workload lbs: spread the 2^26 items over this many Zipf-weighted segments "
+                         "(default: Zipf(1.1) counts capped at 2^20, 1805 segments)")
     ap.add_argument("--no-variants", action="store_true", help="skip the per-variant sweep")
     ap.add_argument("--no-serve", action="store_true",
                     help="bfs workload: skip the serving stream reported beside the GTEPS figure")
