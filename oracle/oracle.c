@@ -306,3 +306,142 @@ double or_lbs_timed(const int32_t *counts, int64_t S, int32_t *scan,
   or_lbs(tot < W ? tot : W, scan, S, result);
   return now_s() - t0;
 }
+
+/* ---- threaded helpers for full-size parity checks ----------------------- */
+
+typedef struct {
+  const int32_t *R, *C, *sources;
+  int32_t n, lo, hi;
+  int32_t *dist;     /* [nsrc][n] or NULL */
+  uint64_t *sigma;   /* [nsrc][n] or NULL */
+  int32_t *ecc;      /* [nsrc] or NULL */
+} rows_arg;
+
+static void *rows_worker(void *p) {
+  rows_arg *a = (rows_arg *)p;
+  size_t n = (size_t)a->n;
+  int32_t *dtmp = (int32_t *)malloc(sizeof(int32_t) * (n > 0 ? n : 1));
+  for (int32_t k = a->lo; k < a->hi; k++) {
+    int32_t *d = a->dist ? a->dist + (size_t)k * n : dtmp;
+    if (a->sigma)
+      or_sigma(a->R, a->C, a->n, a->sources[k], d, a->sigma + (size_t)k * n);
+    else
+      or_bfs(a->R, a->C, a->n, a->sources[k], d);
+    if (a->ecc) {
+      int32_t mx = 0;
+      for (size_t i = 0; i < n; i++)
+        if (d[i] != INT_MAX && d[i] > mx) mx = d[i];
+      a->ecc[k] = mx;
+    }
+  }
+  free(dtmp);
+  return NULL;
+}
+
+void or_rows_mt(const int32_t *R, const int32_t *C, int32_t n,
+                const int32_t *sources, int32_t nsrc, int32_t nthreads,
+                int32_t *dist, uint64_t *sigma, int32_t *ecc) {
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > nsrc) nthreads = nsrc > 0 ? nsrc : 1;
+  pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nthreads);
+  rows_arg *args = (rows_arg *)malloc(sizeof(rows_arg) * (size_t)nthreads);
+  for (int32_t t = 0; t < nthreads; t++) {
+    rows_arg x = {R, C, sources, n, (int32_t)((int64_t)nsrc * t / nthreads),
+                  (int32_t)((int64_t)nsrc * (t + 1) / nthreads), dist, sigma, ecc};
+    args[t] = x;
+    pthread_create(&th[t], NULL, rows_worker, &args[t]);
+  }
+  for (int32_t t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+  free(th);
+  free(args);
+}
+
+/* One row of or_check_search_rows.  Returns the violations; *bad gets the
+ * first offending vertex. */
+static int64_t check_row(const int32_t *RT, const int32_t *CT, int32_t n,
+                         int32_t src, const int32_t *d, const uint64_t *s,
+                         int64_t *bad) {
+  int64_t viol = 0;
+  *bad = -1;
+#define GU_BAD(v)               \
+  do {                          \
+    if (*bad < 0) *bad = (v);   \
+    viol++;                     \
+  } while (0)
+  if (src < 0 || src >= n) {
+    GU_BAD(0);
+    return viol;
+  }
+  for (int32_t v = 0; v < n; v++) {
+    const int32_t dv = d[v];
+    if (v == src) {
+      if (dv != 0 || (s && s[v] != 1)) GU_BAD(v);
+      continue;
+    }
+    int32_t mn = INT_MAX;
+    uint64_t acc = 0;
+    for (int32_t k = RT[v]; k < RT[v + 1]; k++) {
+      const int32_t du = d[CT[k]];
+      if (du < mn) mn = du;
+      if (s && dv != INT_MAX && du == dv - 1) acc += s[CT[k]]; /* mod 2^64 */
+    }
+    if (dv == INT_MAX) {
+      /* unreached: no in-neighbour reached, no paths */
+      if (mn != INT_MAX || (s && s[v] != 0)) GU_BAD(v);
+    } else {
+      /* reached: one more than the nearest in-neighbour */
+      if (dv < 1 || mn == INT_MAX || mn != dv - 1 || (s && s[v] != acc)) GU_BAD(v);
+    }
+  }
+#undef GU_BAD
+  return viol;
+}
+
+typedef struct {
+  const int32_t *RT, *CT, *sources, *dist;
+  const uint64_t *sigma;
+  int32_t n, lo, hi;
+  int64_t viol, bad;
+} check_arg;
+
+static void *check_worker(void *p) {
+  check_arg *a = (check_arg *)p;
+  size_t n = (size_t)a->n;
+  a->viol = 0;
+  a->bad = -1;
+  for (int32_t k = a->lo; k < a->hi; k++) {
+    int64_t b;
+    int64_t v = check_row(a->RT, a->CT, a->n, a->sources[k], a->dist + (size_t)k * n,
+                          a->sigma ? a->sigma + (size_t)k * n : NULL, &b);
+    if (v && a->bad < 0) a->bad = (int64_t)k * (int64_t)n + b;
+    a->viol += v;
+  }
+  return NULL;
+}
+
+int64_t or_check_search_rows(const int32_t *RT, const int32_t *CT, int32_t n,
+                             const int32_t *sources, int32_t nrows,
+                             const int32_t *dist, const uint64_t *sigma,
+                             int32_t nthreads, int64_t *first_bad) {
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > nrows) nthreads = nrows > 0 ? nrows : 1;
+  pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nthreads);
+  check_arg *args = (check_arg *)malloc(sizeof(check_arg) * (size_t)nthreads);
+  for (int32_t t = 0; t < nthreads; t++) {
+    check_arg x = {RT, CT, sources, dist, sigma, n,
+                   (int32_t)((int64_t)nrows * t / nthreads),
+                   (int32_t)((int64_t)nrows * (t + 1) / nthreads), 0, -1};
+    args[t] = x;
+    pthread_create(&th[t], NULL, check_worker, &args[t]);
+  }
+  int64_t viol = 0, bad = -1;
+  for (int32_t t = 0; t < nthreads; t++) {
+    pthread_join(th[t], NULL);
+    viol += args[t].viol;
+    if (bad < 0 && args[t].bad >= 0) bad = args[t].bad;
+  }
+  if (first_bad) *first_bad = bad;
+  free(th);
+  free(args);
+  return viol;
+}

@@ -89,6 +89,35 @@ int64_t or_sigma_batch_timed(const int32_t *R, const int32_t *C, int32_t n,
 double or_lbs_timed(const int32_t *counts, int64_t S, int32_t *scan,
                     int32_t *result, int64_t W);
 
+
+/* Rows of or_sigma (sigma != NULL) or or_bfs (sigma == NULL) for a list of
+ * sources on nthreads threads, plus each source's eccentricity (largest
+ * finite distance, diameter_sampling_test.cu:9-38) when ecc != NULL.
+ * dist may be NULL when only eccentricities are wanted (then sigma must be
+ * NULL too).  Rows are [nsrc][n]. */
+void or_rows_mt(const int32_t *R, const int32_t *C, int32_t n,
+                const int32_t *sources, int32_t nsrc, int32_t nthreads,
+                int32_t *dist, uint64_t *sigma, int32_t *ecc);
+
+/* Full-size check of multi-search rows without recomputing them.  RT/CT is
+ * the in-edge CSR (the adjacency itself for undirected graphs).  Row k
+ * (source sources[k]) passes when
+ *   d[src] = 0 and sigma[src] = 1;
+ *   every other vertex with a finite distance has d[v] = 1 + min of d over
+ *   its in-neighbours, and sigma[v] = sum of sigma over the in-neighbours u
+ *   with d[u] = d[v] - 1 (mod 2^64);
+ *   every vertex with d = INT_MAX has only INT_MAX in-neighbours and
+ *   sigma = 0.
+ * These conditions have exactly one solution, the BFS distances and
+ * shortest-path counts (induction over the distance), so a passing row is
+ * the oracle's row.  sigma may be NULL (distances only).  Returns the number
+ * of violating (row, vertex) pairs; *first_bad = row * n + vertex of one of
+ * them, or -1. */
+int64_t or_check_search_rows(const int32_t *RT, const int32_t *CT, int32_t n,
+                             const int32_t *sources, int32_t nrows,
+                             const int32_t *dist, const uint64_t *sigma,
+                             int32_t nthreads, int64_t *first_bad);
+
 #ifdef __cplusplus
 }
 #endif

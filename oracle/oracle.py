@@ -45,6 +45,10 @@ def lib():
         L.or_sigma_batch_timed.restype = i64
         L.or_lbs_timed.argtypes = [_P, i64, _P, _P, i64]
         L.or_lbs_timed.restype = ctypes.c_double
+        L.or_rows_mt.argtypes = [_P, _P, i32, _P, i32, i32, _P, _P, _P]
+        L.or_check_search_rows.argtypes = [_P, _P, i32, _P, i32, _P, _P, i32,
+                                           ctypes.POINTER(ctypes.c_int64)]
+        L.or_check_search_rows.restype = i64
         _lib = L
     return _lib
 
@@ -175,6 +179,59 @@ def lbs_timed(counts):
     res = np.empty(W, dtype=np.int32)
     secs = lib().or_lbs_timed(_p(c), len(c), _p(scan), _p(res), W)
     return scan, res, secs
+
+
+def default_threads():
+    """Threads for the checker: the cores this process may use, at most 16
+    (the GPU box's share of its host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def rows_mt(R, C, sources, sigma=True, ecc=False, nthreads=None):
+    """Oracle rows for many sources on threads: (dist, sigma or None, ecc or
+    None), rows [len(sources)][n]."""
+    R, C = _csr(R, C)
+    n = len(R) - 1
+    src = np.ascontiguousarray(sources, dtype=np.int32)
+    k = len(src)
+    d = np.empty((k, n), dtype=np.int32)
+    s = np.empty((k, n), dtype=np.uint64) if sigma else None
+    e = np.empty(k, dtype=np.int32) if ecc else None
+    lib().or_rows_mt(_p(R), _p(C), n, _p(src), k, int(nthreads or default_threads()), _p(d),
+                     _p(s) if sigma else None, _p(e) if ecc else None)
+    return d, s, e
+
+
+def eccentricities_mt(R, C, sources, nthreads=None):
+    R, C = _csr(R, C)
+    src = np.ascontiguousarray(sources, dtype=np.int32)
+    e = np.empty(len(src), dtype=np.int32)
+    lib().or_rows_mt(_p(R), _p(C), len(R) - 1, _p(src), len(src), int(nthreads or default_threads()),
+                     None, None, _p(e))
+    return e
+
+
+def check_search_rows(RT, CT, sources, dist, sigma=None, nthreads=None):
+    """Checks search rows against the BFS equations (see oracle.h).  Returns
+    (violations, (row, vertex) of one violation or None)."""
+    RT, CT = _csr(RT, CT)
+    n = len(RT) - 1
+    src = np.ascontiguousarray(sources, dtype=np.int32)
+    d = np.ascontiguousarray(dist, dtype=np.int32)
+    assert d.shape == (len(src), n)
+    if sigma is not None:
+        sigma = np.ascontiguousarray(sigma, dtype=np.uint64)
+        assert sigma.shape == d.shape
+    bad = ctypes.c_int64(-1)
+    v = lib().or_check_search_rows(_p(RT), _p(CT), n, _p(src), len(src), _p(d),
+                                   _p(sigma) if sigma is not None else None,
+                                   int(nthreads or default_threads()), ctypes.byref(bad))
+    where = None if bad.value < 0 else divmod(int(bad.value), n)
+    return int(v), where
 
 
 def ref_parse_available():

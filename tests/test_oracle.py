@@ -334,3 +334,98 @@ def test_bc_f64_multithreaded_matches_serial(orc):
     R, C = load_ref_graph(orc, "jazz")
     src = np.arange(0, 198, 3)
     assert np.allclose(orc.bc_f64_mt(R, C, src, 4), orc.bc_f64(R, C, src), rtol=1e-13, atol=1e-10)
+
+
+# ---- the full-size row checker (or_check_search_rows) and the threaded rows
+
+
+def _er_rows(orc, n=300, p=0.012, seed=5, k=12):
+    R, C = random_graph(n, p, seed)
+    src = np.random.default_rng(seed).integers(0, n, k).astype(np.int32)
+    d, s, e = orc.rows_mt(R, C, src, sigma=True, ecc=True, nthreads=3)
+    return R, C, src, d, s, e
+
+
+def test_rows_mt_matches_serial_oracle(orc):
+    R, C, src, d, s, e = _er_rows(orc)
+    for i, x in enumerate(src):
+        od, os_ = orc.sigma(R, C, int(x))
+        assert np.array_equal(d[i], od) and np.array_equal(s[i], os_)
+        assert e[i] == orc.eccentricity(R, C, int(x))
+    assert np.array_equal(orc.eccentricities_mt(R, C, src, nthreads=4), e)
+
+
+def test_check_rows_accepts_oracle_and_scipy_rows(orc):
+    R, C, src, d, s, _ = _er_rows(orc)
+    assert orc.check_search_rows(R, C, src, d, s, nthreads=2) == (0, None)
+    # distances from an independent BFS (scipy), distances-only check
+    n = len(R) - 1
+    A = sp.csr_matrix((np.ones(len(C)), C, R), shape=(n, n))
+    sd = shortest_path(A, unweighted=True, indices=src)
+    sd = np.where(np.isinf(sd), INT_MAX, sd).astype(np.int32)
+    assert orc.check_search_rows(R, C, src, sd, None) == (0, None)
+    # the intact grid's closed forms: Manhattan distance and binomials
+    W = 30
+    Rg, Cg = grid_graph(W)
+    r, c = np.divmod(np.arange(W * W), W)
+    gd = (r + c).astype(np.int32)[None, :]
+    gs = np.array([comb(int(a + b), int(a)) % (1 << 64) for a, b in zip(r, c)], dtype=np.uint64)[None, :]
+    assert orc.check_search_rows(Rg, Cg, np.array([0], np.int32), gd, gs) == (0, None)
+
+
+@pytest.mark.parametrize("mistake", ["d_plus_one", "d_minus_one", "unreached_finite", "reached_inf",
+                                     "sigma_off", "src_dist", "src_sigma", "unreached_sigma",
+                                     "swap_rows", "level_shift"])
+def test_check_rows_catches_planted_mistakes(orc, mistake):
+    """Each plausible kernel mistake, planted in oracle rows on the CPU, must
+    be reported, with its location."""
+    R, C, src, d, s, _ = _er_rows(orc)
+    d, s = d.copy(), s.copy()
+    row = 3
+    reach = np.flatnonzero((d[row] != INT_MAX) & (np.arange(len(R) - 1) != src[row]))
+    unreach = np.flatnonzero(d[row] == INT_MAX)
+    v = int(reach[len(reach) // 2])
+    if mistake == "d_plus_one":
+        d[row, v] += 1
+    elif mistake == "d_minus_one":
+        d[row, v] -= 1
+    elif mistake == "unreached_finite":
+        assert len(unreach) > 0
+        d[row, unreach[0]] = 5
+    elif mistake == "reached_inf":
+        d[row, v] = INT_MAX
+    elif mistake == "sigma_off":
+        s[row, v] += np.uint64(1)
+    elif mistake == "src_dist":
+        d[row, src[row]] = 1
+    elif mistake == "src_sigma":
+        s[row, src[row]] = np.uint64(2)
+    elif mistake == "unreached_sigma":
+        assert len(unreach) > 0
+        s[row, unreach[0]] = np.uint64(1)
+    elif mistake == "swap_rows":  # rows stored under the wrong source
+        d[[row, row + 1]] = d[[row + 1, row]]
+        s[[row, row + 1]] = s[[row + 1, row]]
+    elif mistake == "level_shift":  # every distance one too large
+        fin = d[row] != INT_MAX
+        d[row, fin] += 1
+        d[row, src[row]] = 0
+    viol, where = orc.check_search_rows(R, C, src, d, s)
+    assert viol > 0
+    assert where[0] in (row, row + 1)
+
+
+def test_teps_units_is_component_degree_sum(orc):
+    """or_teps_units (the GTEPS numerator) against an independent count: the
+    degree sum of the source's connected component from scipy."""
+    from scipy.sparse.csgraph import connected_components
+
+    for seed in (1, 2, 3):
+        R, C = random_graph(400, 0.004, seed)
+        n = len(R) - 1
+        A = sp.csr_matrix((np.ones(len(C)), C, R), shape=(n, n))
+        _, lab = connected_components(A, directed=False)
+        deg = np.diff(R).astype(np.int64)
+        comp_deg = np.bincount(lab, weights=deg).astype(np.int64)
+        for s in np.random.default_rng(seed).integers(0, n, 20):
+            assert orc.teps_units(R, orc.bfs(R, C, int(s))) == comp_deg[lab[s]]
