@@ -66,6 +66,7 @@ EXPORTS = (
 
 QUERY_BFS, QUERY_BC, QUERY_DIAMETER = 0, 1, 2
 QF_SIGMA, QF_DELTA = 1, 2
+QR_SIGMA_WRAPPED = 1  # gu_query_result.flags: a path count wrapped mod 2^64
 ERR_CAPACITY, ERR_TIMEOUT = -9, -10
 
 
@@ -143,7 +144,7 @@ class Query(ctypes.Structure):
 class QueryResult(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int32), ("kind", ctypes.c_int32), ("nsrc", ctypes.c_int32),
                 ("diameter", ctypes.c_int32), ("launches", ctypes.c_int32),
-                ("reserved", ctypes.c_int32), ("dist_dev", ctypes.c_void_p),
+                ("flags", ctypes.c_int32), ("dist_dev", ctypes.c_void_p),
                 ("sigma_dev", ctypes.c_void_p), ("delta_dev", ctypes.c_void_p),
                 ("bc_dev", ctypes.c_void_p), ("submit_ms", ctypes.c_double),
                 ("admit_ms", ctypes.c_double), ("first_launch_ms", ctypes.c_double),
@@ -705,7 +706,7 @@ class Server:
     def fetch(self, res, graph):
         """Host copies of a finished query's outputs: dict of arrays."""
         n, k = self.n[graph], res.nsrc
-        out = {"status": res.status, "diameter": res.diameter}
+        out = {"status": res.status, "diameter": res.diameter, "flags": res.flags}
 
         def get(addr, dtype, count):
             a = np.empty(count, dtype=dtype)

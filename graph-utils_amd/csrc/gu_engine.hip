@@ -17,6 +17,7 @@
 #include "../../include/graph_utils_amd.h"
 #include "gu_kernels.hpp"
 #include "gu_persource.hpp"
+#include "gu_internal.hpp"
 
 namespace gu {
 
@@ -1099,7 +1100,7 @@ struct gu_graph {
   int *ecc = nullptr;         // device [256] per-source eccentricity
   int *ecc_h = nullptr;       // pinned mirror
   int *misc_h = nullptr;      // pinned: [0] overflow flag, [64, 320) depths
-  u64 *part_h = nullptr;      // pinned [3 * 256] per-source sums
+  u64 *part_h = nullptr;      // pinned [4 * 256] per-source sums and wrap flags
   // Captured chunks of levels of the batched engine, keyed by the launch
   // parameters and the engine arguments (which hold every buffer address);
   // captured on a private stream, replayed on the caller's.
@@ -1118,6 +1119,17 @@ struct gu_graph {
 namespace {
 
 int pick_grid(const gu_graph *g, int per_cu) { return std::max(1, g->cus * per_cu); }
+
+// 64-bit finaliser of splitmix64 (Steele, Lea and Flood, "Fast splittable
+// pseudorandom number generators", 2014): a bijection that mixes every input
+// bit into every output bit.
+u64 mix64(u64 x) {
+  x ^= x >> 30;
+  x *= 0xbf58476d1ce4e5b9ull;
+  x ^= x >> 27;
+  x *= 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
 
 constexpr int kRowTab = 2048;
 
@@ -1248,7 +1260,10 @@ int ensure_ws(gu_graph *g, int NW, bool stack, size_t want_S, hipStream_t st) {
     int *q2;
     u64 *m2;
     GU_HIP(dalloc(&q2, cap));
-    GU_HIP(dalloc(&m2, cap * w.NW));
+    if (hipError_t e = dalloc(&m2, cap * w.NW)) {
+      (void)hipFree(q2);
+      return fail(e);
+    }
     GU_HIP(hipMemcpyAsync(q2, w.q, w.S_cap * 4, hipMemcpyDeviceToDevice, st));
     GU_HIP(hipMemcpyAsync(m2, w.smask, w.S_cap * w.NW * 8, hipMemcpyDeviceToDevice, st));
     GU_HIP(hipStreamSynchronize(st));
@@ -1517,7 +1532,7 @@ int ensure_pws(gu_graph *g, bool need_dist) {
     GU_HIP(dalloc(&p.queues, (size_t)kPsCap * (2 * n + 2)));
     GU_HIP(dalloc(&p.depth, kPsCap));
     GU_HIP(dalloc(&p.ovf, 1));
-    GU_HIP(dalloc(&p.part, 3 * kPsCap));
+    GU_HIP(dalloc(&p.part, 4 * kPsCap));
     GU_HIP(dalloc(&p.srcs, kPsCap));
     p.cap = kPsCap;
     p.bytes = (size_t)kPsCap * (2 * n + 2) * 4 + kPsCap * 32 + 4;
@@ -1539,6 +1554,9 @@ struct HostRows {
   float *const *delta = nullptr;
   double *const *bc = nullptr;
   int32_t *ecc = nullptr;
+  // per source (host): 1 if a path count wrapped modulo 2^64; exact per
+  // source on the per-source engine, per launch on the batched engine
+  int32_t *wrap = nullptr;
 };
 
 // Per-source engine over nsrc sources, kPsCap workgroups per launch.  On a
@@ -1614,17 +1632,18 @@ int run_persource(gu_graph *g, const int *sources, int nsrc, const HostRows &hr,
                                                 (u64 *const *)(g->rowtab + 1280), g->nof, g->n);
       GU_HIP(hipGetLastError());
     }
-    GU_HIP(hipMemcpyAsync(part, p.part, sizeof(u64) * 3 * nb, hipMemcpyDeviceToHost, st));
+    GU_HIP(hipMemcpyAsync(part, p.part, sizeof(u64) * 4 * nb, hipMemcpyDeviceToHost, st));
     GU_HIP(hipMemcpyAsync(depth, p.depth, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
     GU_HIP(hipMemcpyAsync(g->misc_h, p.ovf, sizeof(int), hipMemcpyDeviceToHost, st));
     GU_HIP(hipStreamSynchronize(st));
     ovf_any |= g->misc_h[0];
     for (int i = 0; i < nb; i++) {
-      s.teps_units += (int64_t)part[3 * i];
-      s.reached += (int64_t)part[3 * i + 1];
-      s.dag_edges += (int64_t)part[3 * i + 2];
+      s.teps_units += (int64_t)part[4 * i];
+      s.reached += (int64_t)part[4 * i + 1];
+      s.dag_edges += (int64_t)part[4 * i + 2];
       dmax = std::max(dmax, depth[i]);
       if (hr.ecc) hr.ecc[b0 + i] = depth[i];
+      if (hr.wrap) hr.wrap[b0 + i] = sigma && part[4 * i + 3] != 0;
     }
     s.batches++;
   }
@@ -1700,16 +1719,17 @@ int run_persource_bc(gu_graph *g, const int *sources, int nsrc, const HostRows &
                                                        (double *const *)(g->rowtab + 768), nb, g->n,
                                                        rl ? g->nof : nullptr);
     GU_HIP(hipGetLastError());
-    GU_HIP(hipMemcpyAsync(part, p.part, sizeof(u64) * 3 * nb, hipMemcpyDeviceToHost, st));
+    GU_HIP(hipMemcpyAsync(part, p.part, sizeof(u64) * 4 * nb, hipMemcpyDeviceToHost, st));
     GU_HIP(hipMemcpyAsync(depth, p.depth, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
     GU_HIP(hipMemcpyAsync(g->misc_h, p.ovf, sizeof(int), hipMemcpyDeviceToHost, st));
     GU_HIP(hipStreamSynchronize(st));
     ovf_any |= g->misc_h[0];
     for (int i = 0; i < nb; i++) {
-      s.teps_units += (int64_t)part[3 * i];
-      s.reached += (int64_t)part[3 * i + 1];
-      s.dag_edges += (int64_t)part[3 * i + 2];
+      s.teps_units += (int64_t)part[4 * i];
+      s.reached += (int64_t)part[4 * i + 1];
+      s.dag_edges += (int64_t)part[4 * i + 2];
       dmax = std::max(dmax, depth[i]);
+      if (hr.wrap) hr.wrap[b0 + i] = part[4 * i + 3] != 0;
     }
     s.batches++;
   }
@@ -1777,6 +1797,7 @@ int run_job(gu_graph *g, const int32_t *sources, int32_t nsrc, Job job, const Ho
   const int variant = resolve_variant(g, cfg);
   // Path counts are computed for Brandes and when the caller asks for them.
   const bool want_sigma = job == JOB_BC || (job == JOB_BFS && hr.sigma != nullptr);
+  if (hr.wrap) std::memset(hr.wrap, 0, sizeof(int32_t) * (size_t)nsrc);
   hipStream_t st = (hipStream_t)cfg.stream;
   GU_HIP(hipSetDevice(g->device));
   EventPair ev;
@@ -1817,6 +1838,8 @@ int run_job(gu_graph *g, const int32_t *sources, int32_t nsrc, Job job, const Ho
       rc = run_batch<4>(g, sources + b0, nb, job, want_sigma, cfg, variant, o, &info, st);
     if (rc) break;
     if (hr.ecc) std::memcpy(hr.ecc + b0, g->ecc_h, sizeof(int) * nb);
+    if (hr.wrap)
+      for (int i = 0; i < nb; i++) hr.wrap[b0 + i] = want_sigma && info.overflow;
     s.batches++;
     s.topdown_levels += info.td;
     s.bottomup_levels += info.bu;
@@ -2055,7 +2078,7 @@ int alloc_handle_bufs(gu_graph *g) {
       hipHostMalloc((void **)&g->rowtab_h, sizeof(void *) * kRowTab, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void **)&g->ecc_h, sizeof(int) * 256, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void **)&g->misc_h, sizeof(int) * 320, hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void **)&g->part_h, sizeof(u64) * 768, hipHostMallocDefault) != hipSuccess)
+      hipHostMalloc((void **)&g->part_h, sizeof(u64) * 4 * 256, hipHostMallocDefault) != hipSuccess)
     return GU_ERR_NOMEM;
   std::memset(g->ctrl_h, 0, sizeof(Ctrl));
   std::memset(g->rowtab_h, 0, sizeof(void *) * kRowTab);
@@ -2071,9 +2094,8 @@ int gu_graph_create(const int32_t *R, const int32_t *C, const int32_t *F, int32_
                     int32_t directed, gu_graph_t *out) {
   if (!out || !R || n < 0 || m < 0 || m >= INT32_MAX || (m > 0 && !C)) return GU_ERR_INVALID;
   *out = nullptr;
-  int count = 0;
-  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return GU_ERR_NODEVICE;
-  // host-side checks of the CSR shape (the reference trusts its parser)
+  // host-side checks of the CSR shape (the reference trusts its parser),
+  // before any device call
   if (R[0] != 0 || (int64_t)R[n] != m) return GU_ERR_INVALID;
   int maxd = 0;
   long long nonzero = 0;
@@ -2083,11 +2105,24 @@ int gu_graph_create(const int32_t *R, const int32_t *C, const int32_t *F, int32_
     nonzero += R[v + 1] > R[v];
   }
   bool strict = true;  // rows strictly increasing: no parallel edges
+  // An undirected graph's entries are a symmetric multiset: (v, w) occurs as
+  // often as (w, v).  Then the sum over entries of h(v, w) equals that of
+  // h(w, v) for any h; with a 64-bit mixing hash an asymmetric CSR fails
+  // the equality except with probability about 2^-64, and a symmetric one
+  // always passes.
+  u64 fwd = 0, bwd = 0;
   for (int32_t v = 0; v < n; v++)
     for (int64_t k = R[v]; k < R[v + 1]; k++) {
       if (C[k] < 0 || C[k] >= n) return GU_ERR_RANGE;
       strict &= k == R[v] || C[k] > C[k - 1];
+      if (!directed) {
+        fwd += mix64(((u64)(uint32_t)v << 32) | (uint32_t)C[k]);
+        bwd += mix64(((u64)(uint32_t)C[k] << 32) | (uint32_t)v);
+      }
     }
+  if (fwd != bwd) return GU_ERR_INVALID;  // declared undirected, not symmetric
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return GU_ERR_NODEVICE;
   gu_graph *g = new (std::nothrow) gu_graph;
   if (!g) return GU_ERR_NOMEM;
   g->n = n;
@@ -2310,6 +2345,35 @@ int gu_betweenness_rows(gu_graph_t g, const int32_t *sources, int32_t nsrc,
   return run_job(g, sources, nsrc, JOB_BC, hr, nullptr, cfg, stats);
 }
 
+}  // extern "C"
+
+// Row forms with per-source wrap flags, for the query server (C++ linkage,
+// declared in gu_internal.hpp).
+int gu_search_rows_wrap(gu_graph_t g, const int32_t *sources, int32_t nsrc, int32_t *const *dist_rows,
+                        uint64_t *const *sigma_rows, int32_t *ecc, int32_t *wrap, const gu_config *cfg,
+                        gu_stats *stats) {
+  if (!g || nsrc < 0) return GU_ERR_INVALID;
+  HostRows hr;
+  hr.dist = dist_rows;
+  hr.sigma = (u64 *const *)sigma_rows;
+  hr.ecc = ecc;
+  hr.wrap = wrap;
+  const Job job = (!dist_rows && !sigma_rows && ecc) ? JOB_DIAM : JOB_BFS;
+  return run_job(g, sources, nsrc, job, hr, nullptr, cfg, stats);
+}
+
+int gu_bc_rows_wrap(gu_graph_t g, const int32_t *sources, int32_t nsrc, float *const *delta_rows,
+                    double *const *bc_targets, int32_t *wrap, const gu_config *cfg, gu_stats *stats) {
+  if (!g || nsrc < 0) return GU_ERR_INVALID;
+  HostRows hr;
+  hr.delta = delta_rows;
+  hr.bc = bc_targets;
+  hr.wrap = wrap;
+  return run_job(g, sources, nsrc, JOB_BC, hr, nullptr, cfg, stats);
+}
+
+extern "C" {
+
 int gu_diameter(gu_graph_t g, const int32_t *sources, int32_t nsrc, int32_t *diameter,
                 const gu_config *cfg, gu_stats *stats) {
   if (!diameter) return GU_ERR_INVALID;
@@ -2509,7 +2573,8 @@ int gu_lbs(const int32_t *counts, int64_t S, int32_t *scan, int32_t *result, int
   GU_HIP(hipEventSynchronize(ev.b));
   float ms = 0;
   GU_HIP(hipEventElapsedTime(&ms, ev.a, ev.b));
-  if (W >= INT_MAX || (W > 0 && (!result || W > result_cap))) rc = GU_ERR_CAPACITY;
+  // result == NULL asks for the scan and W only
+  if (W >= INT_MAX || (result && W > result_cap)) rc = GU_ERR_CAPACITY;
   if (W_out) *W_out = W;
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));

@@ -29,7 +29,8 @@ struct PsArgs {
   u64 *const *sigma; // [nsrc] path-count rows, or null
   int *queues;       // [nsrc][2n+2]: frontier entries beyond the LDS capacity
   int *depth;        // [nsrc] deepest level reached
-  u64 *part;         // [nsrc][3] (TEPS units, reached vertices, DAG edges)
+  u64 *part;         // [nsrc][4] (TEPS units, reached vertices, DAG edges,
+                     //   1 if one of the source's path counts wrapped)
   int *ovf;          // set when a path count wrapped
 };
 
@@ -399,7 +400,7 @@ __device__ int ps_levels(const int *R, const int *C, int n, int s, int *d, u64 *
 }
 
 // Workgroup sums of the per-thread counters; thread 0 writes the source's
-// part / depth entries.
+// part / depth entries (part[3]: a path count of this source wrapped).
 __device__ __forceinline__ void ps_finish(u64 teps, u64 dag, u64 reached, int L, int ovf, int *s_ovf,
                                           u64 (*s_red)[2], u64 *part, int *depth, int *ovf_out) {
   const int tid = threadIdx.x, T = blockDim.x;
@@ -419,9 +420,10 @@ __device__ __forceinline__ void ps_finish(u64 teps, u64 dag, u64 reached, int L,
       t += s_red[w][0];
       g += s_red[w][1];
     }
-    part[3 * blockIdx.x] = t;
-    part[3 * blockIdx.x + 1] = reached;
-    part[3 * blockIdx.x + 2] = g;
+    part[4 * blockIdx.x] = t;
+    part[4 * blockIdx.x + 1] = reached;
+    part[4 * blockIdx.x + 2] = g;
+    part[4 * blockIdx.x + 3] = *s_ovf ? 1ull : 0ull;
     depth[blockIdx.x] = L;
     if (*s_ovf) atomicOr(ovf_out, 1);
   }

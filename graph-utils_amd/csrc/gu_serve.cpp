@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/graph_utils_amd.h"
+#include "gu_internal.hpp"
 #include "gu_serve.hpp"
 
 namespace {
@@ -31,14 +32,14 @@ struct GpuExec final : gus::Exec {
     return c;
   }
   int search(int slot, int graph, const int32_t *src, int k, int32_t *const *drows,
-             uint64_t *const *srows, int32_t *ecc) override {
+             uint64_t *const *srows, int32_t *ecc, int32_t *wrap) override {
     gu_config c = cfg(slot);
-    return gu_multi_search_rows(views[graph][slot], src, k, drows, srows, ecc, &c, nullptr);
+    return gu_search_rows_wrap(views[graph][slot], src, k, drows, srows, ecc, wrap, &c, nullptr);
   }
   int bc(int slot, int graph, const int32_t *src, int k, float *const *xrows,
-         double *const *targets) override {
+         double *const *targets, int32_t *wrap) override {
     gu_config c = cfg(slot);
-    return gu_betweenness_rows(views[graph][slot], src, k, xrows, targets, &c, nullptr);
+    return gu_bc_rows_wrap(views[graph][slot], src, k, xrows, targets, wrap, &c, nullptr);
   }
   int zero(int slot, void *p, size_t bytes) override {
     return hipMemsetAsync(p, 0, bytes, streams[(size_t)slot]) == hipSuccess ? GU_OK : GU_ERR_HIP;
@@ -149,6 +150,7 @@ static void to_c(const gus::Result &r, gu_query_result *o) {
   o->nsrc = r.nsrc;
   o->diameter = r.diameter;
   o->launches = r.launches;
+  o->flags = (r.flags & gus::kFlagWrapped) ? GU_QR_SIGMA_WRAPPED : 0;
   o->dist_dev = r.dist;
   o->sigma_dev = r.sigma;
   o->delta_dev = r.delta;

@@ -98,10 +98,12 @@ struct Exec {
   virtual ~Exec() {}
   virtual char *arena_base() = 0;
   virtual void thread_init(int slot) { (void)slot; }
+  // wrap[k] (host) receives per source 1 when one of its path counts
+  // wrapped modulo 2^64
   virtual int search(int slot, int graph, const int32_t *src, int k, int32_t *const *drows,
-                     uint64_t *const *srows, int32_t *ecc) = 0;
+                     uint64_t *const *srows, int32_t *ecc, int32_t *wrap) = 0;
   virtual int bc(int slot, int graph, const int32_t *src, int k, float *const *xrows,
-                 double *const *targets) = 0;
+                 double *const *targets, int32_t *wrap) = 0;
   virtual int zero(int slot, void *p, size_t bytes) = 0;
 };
 
@@ -113,8 +115,10 @@ struct Spec {
   bool want_delta = false;  // BC: keep the per-source dependency rows
 };
 
+constexpr int kFlagWrapped = 1;  // a path count of one of the query's sources wrapped
+
 struct Result {
-  int status = 0, cls = 0, nsrc = 0, diameter = 0, launches = 0;
+  int status = 0, cls = 0, nsrc = 0, diameter = 0, launches = 0, flags = 0;
   int32_t *dist = nullptr;
   uint64_t *sigma = nullptr;
   float *delta = nullptr;
@@ -207,18 +211,22 @@ class Server {
     return kOk;
   }
 
-  // Blocks until the query is done (timeout_ms < 0: no limit).
+  // Blocks until the query is done (timeout_ms < 0: no limit).  The ticket
+  // is looked up again at every wake-up: another thread may release the
+  // query meanwhile (then kErrInvalid).
   int wait(int64_t ticket, int timeout_ms, Result *r) {
     std::unique_lock<std::mutex> lk(mu_);
-    auto it = queries_.find(ticket);
-    if (it == queries_.end()) return kErrInvalid;
-    Q *q = it->second.get();
-    auto pred = [&] { return q->done || stop_; };
+    Q *q = nullptr;
+    auto pred = [&] {
+      auto it = queries_.find(ticket);
+      q = it == queries_.end() ? nullptr : it->second.get();
+      return !q || q->done || stop_;
+    };
     if (timeout_ms < 0)
       cv_done_.wait(lk, pred);
     else if (!cv_done_.wait_for(lk, std::chrono::milliseconds(timeout_ms), pred))
       return kErrTimeout;
-    if (!q->done) return kErrInvalid;
+    if (!q || !q->done) return kErrInvalid;
     fill(q, r);
     q->reported = true;
     return kOk;
@@ -296,7 +304,7 @@ class Server {
     double *bc = nullptr;
     bool bc_zeroed = false;
     int diameter = 0;
-    int next = 0, done_src = 0, inflight = 0, launches = 0, status = 0;
+    int next = 0, done_src = 0, inflight = 0, launches = 0, status = 0, flags = 0;
     uint64_t seq = 0;
     bool admitted = false, done = false, reported = false;
     double t_submit = 0, t_admit = -1, t_first = -1, t_done = -1;
@@ -316,6 +324,7 @@ class Server {
     r->nsrc = q->nsrc;
     r->diameter = q->diameter;
     r->launches = q->launches;
+    r->flags = q->flags;
     r->dist = q->dist;
     r->sigma = q->sigma;
     r->delta = q->delta;
@@ -406,7 +415,7 @@ class Server {
   void worker(int slot) {
     ex_->thread_init(slot);
     Pack p;
-    std::vector<int32_t> src, ecc;
+    std::vector<int32_t> src, ecc, wrap;
     std::vector<int32_t *> dr;
     std::vector<uint64_t *> sr;
     std::vector<float *> xr;
@@ -424,6 +433,7 @@ class Server {
       xr.clear();
       bt.clear();
       ecc.assign((size_t)p.nsrc, 0);
+      wrap.assign((size_t)p.nsrc, 0);
       int rc = 0;
       for (const Item &it : p.items) {
         Q *q = it.q;
@@ -444,15 +454,22 @@ class Server {
       }
       if (rc == 0) {
         switch (p.cls) {
-          case C_BFS_SIGMA: rc = ex_->search(slot, p.graph, src.data(), p.nsrc, dr.data(), sr.data(), nullptr); break;
-          case C_BFS: rc = ex_->search(slot, p.graph, src.data(), p.nsrc, dr.data(), nullptr, nullptr); break;
+          case C_BFS_SIGMA:
+            rc = ex_->search(slot, p.graph, src.data(), p.nsrc, dr.data(), sr.data(), nullptr, wrap.data());
+            break;
+          case C_BFS:
+            rc = ex_->search(slot, p.graph, src.data(), p.nsrc, dr.data(), nullptr, nullptr, wrap.data());
+            break;
           case C_BC: {
             bool any_delta = false;
             for (float *x : xr) any_delta |= x != nullptr;
-            rc = ex_->bc(slot, p.graph, src.data(), p.nsrc, any_delta ? xr.data() : nullptr, bt.data());
+            rc = ex_->bc(slot, p.graph, src.data(), p.nsrc, any_delta ? xr.data() : nullptr, bt.data(),
+                         wrap.data());
             break;
           }
-          default: rc = ex_->search(slot, p.graph, src.data(), p.nsrc, nullptr, nullptr, ecc.data()); break;
+          default:
+            rc = ex_->search(slot, p.graph, src.data(), p.nsrc, nullptr, nullptr, ecc.data(), wrap.data());
+            break;
         }
       }
       {
@@ -462,6 +479,8 @@ class Server {
           Q *q = it.q;
           if (p.cls == C_DIAM)
             for (int j = 0; j < it.count; j++) q->diameter = std::max(q->diameter, ecc[(size_t)(pos + j)]);
+          for (int j = 0; j < it.count; j++)
+            if (wrap[(size_t)(pos + j)]) q->flags |= kFlagWrapped;
           pos += it.count;
           q->inflight--;
           q->done_src += it.count;

@@ -147,7 +147,11 @@ const char *gu_strerror(int status);
  * directed = 0 declares the CSR symmetric (every entry v->w has its w->v,
  * as parse_metis and parse_market produce); the engines then pull path
  * counts and frontier membership over the same adjacency.  directed = 1
- * builds the in-edge CSR for the pulls. */
+ * builds the in-edge CSR for the pulls.  The arguments are checked on the
+ * host before any device call: GU_ERR_INVALID for a bad shape or a CSR
+ * declared undirected whose entries are not symmetric (for example a Matrix
+ * Market file with more entries than its header, whose surplus
+ * parse_market drops), GU_ERR_RANGE for a column id outside [0, n). */
 int gu_graph_create(const int32_t *R, const int32_t *C, const int32_t *F,
                     int32_t n, int64_t m, int32_t directed, gu_graph_t *out);
 int gu_graph_destroy(gu_graph_t g);
@@ -244,7 +248,8 @@ int gu_autotune(gu_graph_t g, const int32_t *sources, int32_t nsrc,
  * counts_dev; result_dev[k] = upper_bound(scan, k) - 1 for k < W, the last
  * segment whose offset is <= k.  *W_out receives the total W.
  * Fails with GU_ERR_CAPACITY when W > result_cap (then nothing is written
- * to result_dev).  Counts must be >= 0 and W < 2^31. */
+ * to result_dev).  result_dev may be NULL: then only the scan and W are
+ * computed (GU_OK).  Counts must be >= 0 and W < 2^31. */
 int gu_lbs(const int32_t *counts_dev, int64_t S, int32_t *scan_dev,
            int32_t *result_dev, int64_t result_cap, int64_t *W_out,
            const gu_config *cfg, gu_stats *stats);
@@ -266,6 +271,14 @@ typedef struct gu_server *gu_server_t;
 #define GU_QUERY_DIAMETER 2  /* max eccentricity over the sources */
 #define GU_QF_SIGMA 1        /* BFS: also path counts */
 #define GU_QF_DELTA 2        /* BC: also per-source dependency rows */
+/* Result flag: a shortest-path count of one of the query's sources wrapped
+ * modulo 2^64 (some reached vertex has at least 2^64 shortest paths, as on
+ * large grids).  Distances stay exact and path counts exact mod 2^64, as the
+ * reference's unsigned long long counts; dependencies and BC computed from
+ * wrapped counts are not meaningful (they may be inf or NaN).  Exact per
+ * source on the per-source engine; on the batched engine a launch in which
+ * some source wrapped flags every query it carried. */
+#define GU_QR_SIGMA_WRAPPED 1
 
 typedef struct {
   int64_t hbm_budget;  /* bytes for query outputs (0 = a quarter of free HBM) */
@@ -289,7 +302,7 @@ typedef struct {
   int32_t nsrc;
   int32_t diameter;         /* GU_QUERY_DIAMETER */
   int32_t launches;         /* launches that carried its sources */
-  int32_t reserved;
+  int32_t flags;            /* GU_QR_* */
   int32_t *dist_dev;        /* [nsrc x n] (BFS) */
   uint64_t *sigma_dev;      /* [nsrc x n] (BFS with GU_QF_SIGMA) */
   float *delta_dev;         /* [nsrc x n] (BC with GU_QF_DELTA) */

@@ -445,3 +445,42 @@ int64_t or_check_search_rows(const int32_t *RT, const int32_t *CT, int32_t n,
   free(args);
   return viol;
 }
+
+/* shortest_paths.cu:11-39 with the additions checked for a carry: returns 1
+ * when some reached vertex has at least 2^64 shortest paths from src (the
+ * first such vertex, in BFS order, has exact parent counts whose sum carries
+ * out of 64 bits), else 0. */
+int32_t or_sigma_wraps(const int32_t *R, const int32_t *C, int32_t n,
+                       int32_t src) {
+  size_t nn = (size_t)(n > 0 ? n : 1);
+  int32_t *d = (int32_t *)malloc(sizeof(int32_t) * nn);
+  uint64_t *s = (uint64_t *)malloc(sizeof(uint64_t) * nn);
+  int32_t *Q = (int32_t *)malloc(sizeof(int32_t) * nn);
+  for (int32_t i = 0; i < n; i++) {
+    d[i] = INT_MAX;
+    s[i] = 0;
+  }
+  d[src] = 0;
+  s[src] = 1;
+  int32_t head = 0, tail = 0, wrapped = 0;
+  Q[tail++] = src;
+  while (head < tail) {
+    int32_t v = Q[head++];
+    for (int32_t k = R[v]; k < R[v + 1]; k++) {
+      int32_t w = C[k];
+      if (d[w] == INT_MAX) {
+        d[w] = d[v] + 1;
+        Q[tail++] = w;
+      }
+      if (d[w] == d[v] + 1) {
+        uint64_t t = s[w] + s[v];
+        wrapped |= t < s[v];
+        s[w] = t;
+      }
+    }
+  }
+  free(d);
+  free(s);
+  free(Q);
+  return wrapped;
+}

@@ -118,6 +118,11 @@ int64_t or_check_search_rows(const int32_t *RT, const int32_t *CT, int32_t n,
                              const int32_t *dist, const uint64_t *sigma,
                              int32_t nthreads, int64_t *first_bad);
 
+/* 1 when some vertex reached from src has at least 2^64 shortest paths (a
+ * path count of or_sigma wraps), else 0. */
+int32_t or_sigma_wraps(const int32_t *R, const int32_t *C, int32_t n,
+                       int32_t src);
+
 #ifdef __cplusplus
 }
 #endif

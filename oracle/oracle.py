@@ -49,6 +49,8 @@ def lib():
         L.or_check_search_rows.argtypes = [_P, _P, i32, _P, i32, _P, _P, i32,
                                            ctypes.POINTER(ctypes.c_int64)]
         L.or_check_search_rows.restype = i64
+        L.or_sigma_wraps.argtypes = [_P, _P, i32, i32]
+        L.or_sigma_wraps.restype = i32
         _lib = L
     return _lib
 
@@ -213,6 +215,13 @@ def eccentricities_mt(R, C, sources, nthreads=None):
     lib().or_rows_mt(_p(R), _p(C), len(R) - 1, _p(src), len(src), int(nthreads or default_threads()),
                      None, None, _p(e))
     return e
+
+
+def sigma_wraps(R, C, src):
+    """True when some vertex reached from src has at least 2^64 shortest
+    paths (its path count wraps modulo 2^64)."""
+    R, C = _csr(R, C)
+    return bool(lib().or_sigma_wraps(_p(R), _p(C), len(R) - 1, int(src)))
 
 
 def check_search_rows(RT, CT, sources, dist, sigma=None, nthreads=None):

@@ -75,7 +75,7 @@ struct FakeExec final : gus::Exec {
     --concurrent;
   }
   int search(int, int graph, const int32_t *src, int k, int32_t *const *drows, uint64_t *const *srows,
-             int32_t *ecc) override {
+             int32_t *ecc, int32_t *wrap) override {
     const Graph &g = graphs[(size_t)graph];
     std::vector<int32_t> d((size_t)g.n);
     std::vector<uint64_t> s((size_t)g.n);
@@ -85,6 +85,7 @@ struct FakeExec final : gus::Exec {
       or_sigma(g.R.data(), g.C.data(), g.n, src[i], d.data(), s.data());
       if (drows) std::memcpy(drows[i], d.data(), sizeof(int32_t) * (size_t)g.n);
       if (srows) std::memcpy(srows[i], s.data(), sizeof(uint64_t) * (size_t)g.n);
+      if (wrap) wrap[i] = srows ? or_sigma_wraps(g.R.data(), g.C.data(), g.n, src[i]) : 0;
       if (ecc) {
         int e = 0;
         for (int v = 0; v < g.n; v++)
@@ -95,7 +96,8 @@ struct FakeExec final : gus::Exec {
     busy(k);
     return GU_OK;
   }
-  int bc(int, int graph, const int32_t *src, int k, float *const *xrows, double *const *targets) override {
+  int bc(int, int graph, const int32_t *src, int k, float *const *xrows, double *const *targets,
+         int32_t *wrap) override {
     const Graph &g = graphs[(size_t)graph];
     std::vector<double> delta((size_t)g.n), acc((size_t)g.n, 0.0);
     for (int i = 0; i < k; i++) {
@@ -103,6 +105,7 @@ struct FakeExec final : gus::Exec {
     }
     for (int i = 0; i < k; i++) {
       or_brandes_f64(g.R.data(), g.C.data(), g.n, src[i], nullptr, nullptr, delta.data());
+      if (wrap) wrap[i] = or_sigma_wraps(g.R.data(), g.C.data(), g.n, src[i]);
       if (xrows && xrows[i])
         for (int v = 0; v < g.n; v++) xrows[i][v] = (float)delta[(size_t)v];
       for (int v = 0; v < g.n; v++) acc[(size_t)v] += delta[(size_t)v];
@@ -128,7 +131,7 @@ struct QSpec {
 };
 
 struct QOut {
-  int status = 0, diameter = 0, launches = 0;
+  int status = 0, diameter = 0, launches = 0, flags = 0;
   std::vector<int32_t> dist;
   std::vector<uint64_t> sigma;
   std::vector<float> delta;
@@ -140,6 +143,7 @@ QOut copy_out(const gus::Result &r, int n) {
   o.status = r.status;
   o.diameter = r.diameter;
   o.launches = r.launches;
+  o.flags = r.flags;
   const size_t k = (size_t)r.nsrc, N = (size_t)n;
   if (r.dist) o.dist.assign(r.dist, r.dist + k * N);
   if (r.sigma) o.sigma.assign(r.sigma, r.sigma + k * N);
@@ -268,6 +272,7 @@ void compare(const std::vector<Graph> &gs, const std::vector<QSpec> &qs, const s
     CHECK(a[i].sigma == b[i].sigma, "%s q%zu path counts differ", label, i);
     CHECK(a[i].delta == b[i].delta, "%s q%zu dependency rows differ", label, i);
     CHECK(a[i].diameter == b[i].diameter, "%s q%zu diameter %d vs %d", label, i, a[i].diameter, b[i].diameter);
+    CHECK(a[i].flags == b[i].flags, "%s q%zu flags %d vs %d", label, i, a[i].flags, b[i].flags);
     CHECK(a[i].bc.size() == b[i].bc.size(), "%s q%zu bc size", label, i);
     double worst = 0;
     for (size_t v = 0; v < a[i].bc.size() && v < b[i].bc.size(); v++) {
@@ -381,6 +386,75 @@ int main() {
     s.sources = oob.data();
     s.nsrc = 1;
     CHECK(srv.submit(s, &tt) == gus::kErrInvalid, "out-of-range source refused");
+  }
+  // wrapped path counts are flagged per query: on the intact 40 x 40 grid
+  // the counts from a corner reach C(78, 39) > 2^64, those from the centre
+  // stay below 2^38
+  {
+    int32_t *R, *C, *F, n;
+    int64_t m;
+    if (gu_gen_grid(40, 0.0, 1, 0, &R, &C, &F, &n, &m) != GU_OK) return 2;
+    std::vector<Graph> wg = {take(R, C, F, n, m)};
+    FakeExec ex(64 << 20);
+    ex.graphs = wg;
+    gus::Server srv(&ex, 64 << 20, 2, 2);  // packs of 2: the queries share launches
+    int id;
+    srv.add_graph(wg[0].n, &id);
+    std::vector<int32_t> corner = {0}, centre = {20 * 40 + 20}, both = {20 * 40 + 20, 39};
+    const int cls[] = {gus::C_BFS_SIGMA, gus::C_BC, gus::C_BFS, gus::C_DIAM};
+    for (int c : cls) {
+      int64_t t[3];
+      const std::vector<int32_t> *in[3] = {&corner, &centre, &both};
+      for (int j = 0; j < 3; j++) {
+        gus::Spec s;
+        s.graph = 0;
+        s.cls = c;
+        s.nsrc = (int)in[j]->size();
+        s.sources = in[j]->data();
+        CHECK(srv.submit(s, &t[j]) == 0, "submit wrap query");
+      }
+      const bool counts = c == gus::C_BFS_SIGMA || c == gus::C_BC;
+      const int want[3] = {counts ? gus::kFlagWrapped : 0, 0, counts ? gus::kFlagWrapped : 0};
+      for (int j = 0; j < 3; j++) {
+        gus::Result r;
+        CHECK(srv.wait(t[j], 5000, &r) == 0 && r.status == 0, "wrap query done");
+        CHECK(r.flags == want[j], "class %d query %d: flags %d, want %d", c, j, r.flags, want[j]);
+        CHECK(srv.release(t[j]) == 0, "release wrap query");
+      }
+    }
+  }
+
+  // a client waiting on a ticket that another client releases: the waiter
+  // looks the ticket up again and gets kErrInvalid (no access to the freed
+  // query)
+  for (int rep = 0; rep < 50; rep++) {
+    FakeExec ex(1 << 20);
+    ex.graphs = gs;
+    gus::Server srv(&ex, 1 << 20, 1, 8);
+    int id;
+    srv.add_graph(gs[0].n, &id);
+    std::vector<int32_t> src = {1, 2, 3};
+    gus::Spec s;
+    s.graph = 0;
+    s.cls = gus::C_BFS_SIGMA;
+    s.nsrc = 3;
+    s.sources = src.data();
+    int64_t t;
+    CHECK(srv.submit(s, &t) == 0, "submit for the release race");
+    std::atomic<int> got{1};
+    std::thread waiter([&] {
+      gus::Result r;
+      got = srv.wait(t, 5000, &r);
+    });
+    int done = 0;
+    while (!done) {
+      srv.poll(t, &done);
+      std::this_thread::yield();
+    }
+    const int rr = srv.release(t);
+    waiter.join();
+    CHECK(rr == 0, "release while another thread waits");
+    CHECK(got == 0 || got == gus::kErrInvalid, "waiter returned %d", got.load());
   }
   std::printf("serve_test: %d checks, %d failed\n", g_checks, g_fail);
   return g_fail ? 1 : 0;

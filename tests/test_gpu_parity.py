@@ -435,6 +435,25 @@ def test_lbs_capacity_error(gu, gpu):
     assert np.all(rb.to_host(np.int32) == 0)  # nothing written
 
 
+@pytest.mark.parametrize("S", [5000, 300000])
+def test_lbs_scan_only(gu, orc, gpu, S):
+    """gu_lbs with no result buffer computes the scan and the total only and
+    returns GU_OK, on the one-launch path (up to 8192 segments) and the
+    three-phase scan."""
+    c = np.random.default_rng(S).integers(0, 50, S).astype(np.int32)
+    cb = gu.DeviceBuffer.from_array(c)
+    sb = gu.DeviceBuffer(4 * S)
+    announce("lbs scan only S=%d" % S)
+    try:
+        W, _ = gu.lbs_device(cb, S, sb, None, 0)
+        scan = sb.to_host(np.int32, S)
+    finally:
+        cb.close()
+        sb.close()
+    oscan, oW = orc.exclusive_scan(c)
+    assert W == oW and np.array_equal(scan, oscan)
+
+
 def test_source_out_of_range(gu, gpu):
     hg = _load(gu, "test1")
     announce("source out of range")

@@ -468,3 +468,49 @@ def test_missing_library_fails_loudly():
     r = subprocess.run(["python3", "-c", code], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "not built" in r.stdout
+
+
+def _create(gu, R, C, directed):
+    """gu_graph_create's status; a handle made on a GPU machine is destroyed."""
+    R = np.ascontiguousarray(R, dtype=np.int32)
+    C = np.ascontiguousarray(C, dtype=np.int32)
+    h = ctypes.c_void_p()
+    rc = gu.lib().gu_graph_create(R.ctypes.data_as(ctypes.c_void_p), C.ctypes.data_as(ctypes.c_void_p), None,
+                                  len(R) - 1, len(C), directed, ctypes.byref(h))
+    if rc == 0:
+        gu.lib().gu_graph_destroy(h)
+    return rc
+
+
+def test_graph_create_refuses_asymmetric_undirected(gu):
+    """directed = 0 promises a symmetric CSR (the pulls read the adjacency as
+    in-edges); one that is not is refused on the host before any device
+    call, so this holds with or without a GPU."""
+    R = np.array([0, 2, 3, 4], np.int32)   # 0->1, 0->2, 1->0, 2->1: 2->1 has no 1->2
+    C = np.array([1, 2, 0, 1], np.int32)
+    assert _create(gu, R, C, 0) == gu.ERR_INVALID
+    ok = (gu.OK, gu.ERR_NODEVICE)
+    assert _create(gu, R, C, 1) in ok                    # the same CSR as a directed graph
+    Rs = np.array([0, 2, 4, 6], np.int32)                # the symmetric triangle
+    Cs = np.array([1, 2, 0, 2, 0, 1], np.int32)
+    assert _create(gu, Rs, Cs, 0) in ok
+    # parallel edges must come in pairs too: 0-1 twice, 1-0 once
+    Rp = np.array([0, 2, 3], np.int32)
+    Cp = np.array([1, 1, 0], np.int32)
+    assert _create(gu, Rp, Cp, 0) == gu.ERR_INVALID
+    assert _create(gu, np.array([0, 2, 4], np.int32), np.array([1, 1, 0, 0], np.int32), 0) in ok
+
+
+def test_over_long_market_file_is_not_a_symmetric_graph(gu, tmp_path):
+    """parse_market keeps the first m sorted entries of a file with more
+    entries than its header (as parse.cpp does), which drops mirror entries;
+    the resulting CSR cannot be created as an undirected graph."""
+    p = tmp_path / "long.mtx"
+    p.write_text("3 3 1\n3 1\n2 1\n")  # sorted entries (0,1) (0,2) (1,0) (2,0); m = 2 keeps the first two
+    g = gu.HostGraph.parse_market(str(p))
+    assert g.m == 2 and list(g.C) == [1, 2]
+    assert _create(gu, g.R, g.C, 0) == gu.ERR_INVALID
+    # every fixture and generator output is symmetric
+    for hg in (gu.HostGraph.parse_metis(graph_path("jazz")), gu.HostGraph.rmat(10, 8, seed=3),
+               gu.HostGraph.grid(20, 0.1, seed=4)):
+        assert _create(gu, hg.R, hg.C, 0) in (gu.OK, gu.ERR_NODEVICE)

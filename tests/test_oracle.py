@@ -429,3 +429,22 @@ def test_teps_units_is_component_degree_sum(orc):
         comp_deg = np.bincount(lab, weights=deg).astype(np.int64)
         for s in np.random.default_rng(seed).integers(0, n, 20):
             assert orc.teps_units(R, orc.bfs(R, C, int(s))) == comp_deg[lab[s]]
+
+
+@pytest.mark.parametrize("W,wraps", [(34, False), (35, True)])
+def test_sigma_wraps_grid_threshold(orc, W, wraps):
+    """From a corner of the intact W x W grid the largest path count is
+    C(2W-2, W-1) at the far corner: it fits in 64 bits for W = 34
+    (7.2e18) and not for W = 35 (2.8e19)."""
+    assert (comb(2 * W - 2, W - 1) >= 1 << 64) == wraps
+    R, C = grid_graph(W)
+    assert orc.sigma_wraps(R, C, 0) == wraps
+    # from the centre every count is far below 2^64 at W = 34
+    if not wraps:
+        assert not orc.sigma_wraps(R, C, (W // 2) * W + W // 2)
+
+
+def test_sigma_wraps_fixtures_never(orc):
+    for name in FIXTURES:
+        R, C = load_ref_graph(orc, name)
+        assert not any(orc.sigma_wraps(R, C, s) for s in range(len(R) - 1))
