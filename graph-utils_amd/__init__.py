@@ -62,6 +62,7 @@ EXPORTS = (
     "gu_server_submit", "gu_server_wait", "gu_server_next_done", "gu_server_poll",
     "gu_server_release", "gu_server_now_ms", "gu_server_get_stats", "gu_server_destroy",
     "gu_autotune", "gu_parse_market", "gu_parse_snap", "gu_write_edgelist", "gu_reverse",
+    "gu_lbs_noalloc",
 )
 
 QUERY_BFS, QUERY_BC, QUERY_DIAMETER = 0, 1, 2
@@ -216,6 +217,8 @@ def lib():
                                        ctypes.POINTER(Config), ctypes.POINTER(Stats)]),
         "gu_lbs": (ctypes.c_int, [_P, ctypes.c_int64, _P, _P, ctypes.c_int64, _I64P,
                                   ctypes.POINTER(Config), ctypes.POINTER(Stats)]),
+        "gu_lbs_noalloc": (ctypes.c_int, [_P, ctypes.c_int64, _P, _P, ctypes.c_int64, _I64P,
+                                          ctypes.POINTER(Config), ctypes.POINTER(Stats)]),
         "gu_parse_metis": (ctypes.c_int, [ctypes.c_char_p, _I32PP, _I32PP, _I32PP, _I32P, _I64P]),
         "gu_write_metis": (ctypes.c_int, [ctypes.c_char_p, _P, _P, ctypes.c_int32, ctypes.c_int64]),
         "gu_parse_market": (ctypes.c_int, [ctypes.c_char_p, _I32PP, _I32PP, _I32PP, _I32P, _I64P]),
@@ -745,6 +748,24 @@ def lbs_device(counts_buf, S, scan_buf, result_buf, result_cap):
                       result_buf.ptr if result_buf is not None else None, result_cap,
                       ctypes.byref(W), ctypes.byref(cfg), ctypes.byref(st))
     _check(rc, "gu_lbs")
+    return W.value, st
+
+
+# Largest inputs of gu_lbs_noalloc (GU_LBS_NOALLOC_MAX_SEGMENTS / _ITEMS).
+LBS_NOALLOC_MAX_SEGMENTS = 1 << 27
+LBS_NOALLOC_MAX_ITEMS = 1 << 28
+
+
+def lbs_device_noalloc(counts_buf, S, scan_buf, result_buf, result_cap):
+    """gu_lbs_noalloc: lbs_device without any allocation (a fixed scratch
+    per GPU); returns (W, stats) or raises GuError."""
+    W = ctypes.c_int64()
+    st = Stats()
+    cfg = make_config()
+    rc = lib().gu_lbs_noalloc(counts_buf.ptr, S, scan_buf.ptr,
+                              result_buf.ptr if result_buf is not None else None, result_cap,
+                              ctypes.byref(W), ctypes.byref(cfg), ctypes.byref(st))
+    _check(rc, "gu_lbs_noalloc")
     return W.value, st
 
 

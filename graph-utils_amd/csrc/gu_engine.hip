@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -2285,6 +2286,79 @@ int gu_graph_device_bytes(gu_graph_t g, int64_t *bytes) {
 }  // extern "C"
 
 namespace {
+// Scan tiles of the three-phase counts scan (kernels of 256 threads).
+long long lbs_scan_tiles(long long S) {
+  const long long TS = 256LL * kScanIPT;
+  return (S + TS - 1) / TS;
+}
+
+// The launch sequence of a load-balanced search over counts[0, S): up to
+// kLbsSmall segments one k_lbs_small launch, else the three-phase scan into
+// tsum (lbs_scan_tiles(S) + 2 words, the total at tsum[nt + 1]) and, with a
+// result, the tile starts tlo (cap / kLbsXT + 2 ints) and the expansion.
+// Items beyond cap are not written (the kernels test W > cap).
+hipError_t lbs_launch(const int32_t *counts, long long S, int32_t *scan, int32_t *result, long long cap, int cus,
+                      long long *tsum, long long *total_d, int *tlo, hipStream_t st) {
+  const int T = 256;
+  const long long nt = lbs_scan_tiles(S);
+  const int grid = std::max(1, cus * 8);
+  if (S <= kLbsSmall) {
+    // without a result buffer only the scan and the total are wanted.  Two
+    // workgroups of 512 per CU: 2^26 items in 1805 Zipf segments took 50.1
+    // us, against 52.0 with four of 256 and 50.2 with one of 1024 (2 and 8
+    // workgroups of 256 per CU ran at 848 and 1141 against 1230 G items/s)
+    k_lbs_small<<<result ? std::max(1, cus * 2) : 1, kLbsSmallT, 0, st>>>(counts, (int)S, scan,
+                                                                    result ? cap : -1, result, total_d);
+  } else {
+    k_cscan_reduce<<<grid, T, 0, st>>>(counts, S, tsum);
+    k_cscan_tiles<<<1, T, 0, st>>>(tsum, nt, total_d);
+    k_cscan_apply<<<grid, T, 0, st>>>(counts, S, tsum, scan, total_d);
+    if (result) {
+      k_lbs_tiles<<<grid, 256, 0, st>>>(scan, S, total_d, cap, tlo);
+      k_lbs_expand<<<std::max(1, cus * 4), 256, 0, st>>>(scan, S, total_d, cap, tlo, result);
+    }
+  }
+  return hipGetLastError();
+}
+
+// Largest inputs of gu_lbs_noalloc, and the bytes of its scratch: the tile
+// sums and total of the three-phase scan (lbs_scan_tiles(S) + 2 words) and
+// the tile starts of the expansion (W / kLbsXT + 2 ints).
+constexpr long long kNaMaxS = GU_LBS_NOALLOC_MAX_SEGMENTS;
+constexpr long long kNaMaxW = GU_LBS_NOALLOC_MAX_ITEMS;
+constexpr size_t kNaTsumBytes = (size_t)((kNaMaxS + 256LL * kScanIPT - 1) / (256LL * kScanIPT) + 2) * 8;
+constexpr size_t kNaTloBytes = (size_t)(kNaMaxW / kLbsXT + 2) * 4;
+constexpr size_t kNaScratchBytes = (kNaTsumBytes + kNaTloBytes + 255) / 256 * 256;
+
+}  // namespace
+
+// Scratch of gu_lbs_noalloc: one copy per GPU, held until the process ends.
+// Host code must not pass this name to a kernel (on the host it is a
+// stand-in, not a device address); it uses the address that
+// hipGetSymbolAddress returns for the current device (scratch_address).
+static __device__ __align__(256) unsigned char scratch[kNaScratchBytes];
+
+namespace {
+
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mu;
+void *g_na_scratch[kNaMaxDevices] = {};
+
+// Device address of `scratch` on device dev (the current device), looked up
+// on the first call on that device and kept.
+int scratch_address(int dev, void **out) {
+  if (dev < 0 || dev >= kNaMaxDevices) return GU_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (!g_na_scratch[dev]) {
+    void *ptr = nullptr;
+    GU_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return GU_ERR_HIP;
+    g_na_scratch[dev] = ptr;
+  }
+  *out = g_na_scratch[dev];
+  return GU_OK;
+}
+
 // Row i of a contiguous [nsrc x n] buffer.
 template <class T>
 std::vector<T *> contiguous_rows(T *base, int32_t nsrc, int32_t n) {
@@ -2529,9 +2603,7 @@ int gu_lbs(const int32_t *counts, int64_t S, int32_t *scan, int32_t *result, int
   int dev = 0, cus = 0;
   GU_HIP(hipGetDevice(&dev));
   GU_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  const int T = 256;
-  const long long TS = (long long)T * kScanIPT;
-  const long long nt = (S + TS - 1) / TS;
+  const long long nt = lbs_scan_tiles(S);
   struct Tile {
     long long *p = nullptr;
     ~Tile() {
@@ -2547,24 +2619,7 @@ int gu_lbs(const int32_t *counts, int64_t S, int32_t *scan, int32_t *result, int
   GU_HIP(hipEventCreate(&ev.a));
   GU_HIP(hipEventCreate(&ev.b));
   GU_HIP(hipEventRecord(ev.a, st));
-  const int grid = std::max(1, cus * 8);
-  if (S <= kLbsSmall) {
-    // without a result buffer only the scan and the total are wanted.  Two
-    // workgroups of 512 per CU: 2^26 items in 1805 Zipf segments took 50.1
-    // us, against 52.0 with four of 256 and 50.2 with one of 1024 (2 and 8
-    // workgroups of 256 per CU ran at 848 and 1141 against 1230 G items/s)
-    k_lbs_small<<<result ? std::max(1, cus * 2) : 1, kLbsSmallT, 0, st>>>(counts, (int)S, scan,
-                                                                    result ? result_cap : -1, result, total_d);
-  } else {
-    k_cscan_reduce<<<grid, T, 0, st>>>(counts, S, tsum.p);
-    k_cscan_tiles<<<1, T, 0, st>>>(tsum.p, nt, total_d);
-    k_cscan_apply<<<grid, T, 0, st>>>(counts, S, tsum.p, scan, total_d);
-    if (result) {
-      k_lbs_tiles<<<grid, 256, 0, st>>>(scan, S, total_d, result_cap, tlo);
-      k_lbs_expand<<<std::max(1, cus * 4), 256, 0, st>>>(scan, S, total_d, result_cap, tlo, result);
-    }
-  }
-  GU_HIP(hipGetLastError());
+  GU_HIP(lbs_launch(counts, S, scan, result, result_cap, cus, tsum.p, total_d, tlo, st));
   GU_HIP(hipEventRecord(ev.b, st));
   long long W = 0;
   GU_HIP(hipMemcpyAsync(&W, total_d, sizeof(long long), hipMemcpyDeviceToHost, st));
@@ -2581,6 +2636,38 @@ int gu_lbs(const int32_t *counts, int64_t S, int32_t *scan, int32_t *result, int
     stats->device_ms = ms;
   }
   return rc;
+}
+
+int gu_lbs_noalloc(const int32_t *counts, int64_t S, int32_t *scan, int32_t *result, int64_t result_cap,
+                   int64_t *W_out, const gu_config *cfgp, gu_stats *stats) {
+  if (S < 0 || (S > 0 && (!counts || !scan)) || result_cap < 0) return GU_ERR_INVALID;
+  if (S > kNaMaxS) return GU_ERR_CAPACITY;  // more segments than the scratch holds
+  gu_config cfg{};
+  if (cfgp) cfg = *cfgp;
+  hipStream_t st = (hipStream_t)cfg.stream;
+  if (stats) std::memset(stats, 0, sizeof(*stats));  // no device time: no events are made
+  if (W_out) *W_out = 0;
+  if (S == 0) return GU_OK;
+  int dev = 0, cus = 0;
+  GU_HIP(hipGetDevice(&dev));
+  GU_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  void *ptr = nullptr;
+  const int rc0 = scratch_address(dev, &ptr);
+  if (rc0) return rc0;
+  // every scratch word a launch reads is written by the same call first, so
+  // nothing needs clearing between calls
+  long long *tsum = static_cast<long long *>(ptr);
+  long long *total_d = tsum + lbs_scan_tiles(S) + 1;
+  int *tlo = reinterpret_cast<int *>(static_cast<char *>(ptr) + kNaTsumBytes);
+  // the expansion writes at most kNaMaxW items (its tile-start table)
+  const long long cap = std::min<long long>(result_cap, kNaMaxW);
+  GU_HIP(lbs_launch(counts, S, scan, result, cap, cus, tsum, total_d, tlo, st));
+  long long W = 0;
+  GU_HIP(hipMemcpyAsync(&W, total_d, sizeof(long long), hipMemcpyDeviceToHost, st));
+  GU_HIP(hipStreamSynchronize(st));
+  if (W_out) *W_out = W;
+  if (W >= INT_MAX || (result && (W > result_cap || W > kNaMaxW))) return GU_ERR_CAPACITY;
+  return GU_OK;
 }
 
 }  // extern "C"

@@ -254,6 +254,27 @@ int gu_lbs(const int32_t *counts_dev, int64_t S, int32_t *scan_dev,
            int32_t *result_dev, int64_t result_cap, int64_t *W_out,
            const gu_config *cfg, gu_stats *stats);
 
+/* gu_lbs without any allocation, for callers that cannot allocate device
+ * memory (code running under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  It has no counterpart in the
+ * reference.  Same arguments, outputs and errors as gu_lbs, except:
+ *  - its scratch (tile sums, the total, tile starts) is a fixed array of
+ *    about 2.5 MiB in device memory that each GPU the library runs on holds
+ *    until the process ends; the call makes no hipMalloc / hipFree and no
+ *    HIP events, so stats->device_ms is 0;
+ *  - S > GU_LBS_NOALLOC_MAX_SEGMENTS is refused with GU_ERR_CAPACITY before
+ *    any device work; with a result buffer, W > GU_LBS_NOALLOC_MAX_ITEMS is
+ *    refused with GU_ERR_CAPACITY after the scan (scan_dev and *W_out are
+ *    written, result_dev is not);
+ *  - calls on one GPU share its scratch: calls that overlap on one GPU
+ *    (several streams or host threads at once) race and must be
+ *    serialised by the caller. */
+#define GU_LBS_NOALLOC_MAX_SEGMENTS (1LL << 27)
+#define GU_LBS_NOALLOC_MAX_ITEMS (1LL << 28)
+int gu_lbs_noalloc(const int32_t *counts_dev, int64_t S, int32_t *scan_dev,
+                   int32_t *result_dev, int64_t result_cap, int64_t *W_out,
+                   const gu_config *cfg, gu_stats *stats);
+
 /* ---- query server (no counterpart in the reference) -------------------- */
 /* A continuous-batching server over resident graphs.  Clients submit BFS,
  * Brandes BC and diameter-sampling queries of any number of sources from
