@@ -113,15 +113,47 @@ def _line(metric, value, unit, args, ms, dtype, config, extra):
     print(json.dumps(out), flush=True)
 
 
-def bench_grid(args, gu, props):
+def _leg_result(metric, value, unit, steps, warmup, ms, dtype, config, extra):
+    """One workload's result: a line of its own (--workload) or a sub-object
+    of the default line."""
+    out = {"metric": metric, "value": value, "unit": unit, "steps": steps, "warmup": warmup,
+           "ms_per_step": round(ms, 4), "dtype": dtype, "config": config}
+    out.update(extra)
+    return out
+
+
+def _print_leg(args, r):
+    extra = {k: v for k, v in r.items()
+             if k not in ("metric", "value", "unit", "steps", "warmup", "ms_per_step", "dtype", "config")}
+    _line(r["metric"], r["value"], r["unit"], args, r["ms_per_step"], r["dtype"], r["config"], extra)
+
+
+def _roofline(alg_bytes, pass_ms, traffic, **more):
+    """HBM roofline of one pass: model (algorithmic) bytes / pass time against
+    the 8 TB/s peak, and the same for the bytes the profiler counted
+    (traffic, from the committed counter runs) when known."""
+    gbs = alg_bytes / (pass_ms / 1e3) / 1e9
+    r = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+         "frac_counter": (round(traffic / (pass_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None),
+         "traffic_over_model": round(traffic / alg_bytes, 3) if traffic else None,
+         "algorithmic_bytes_per_step": int(alg_bytes), "pass_ms": round(pass_ms, 4)}
+    r.update(more)
+    return r
+
+
+def bench_grid(args, gu, props, steps=None, warmup=None, cpu_sources=96):
     """2048x2048 grid minus 10% of edges, largest component: a step is one
     256-source BFS with path counts (one 256-lane batch); then diameter
-    sampling over 64 sources.  Thousands of levels: measures per-level cost."""
+    sampling over 64 sources.  Thousands of levels: measures per-level cost.
+    Returns the result dict."""
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
     t0 = time.time()
     hg = gu.HostGraph.grid(2048, 0.10, seed=30)
     gen_s = time.time() - t0
     n, k = hg.n, 256
-    batches = [hg.sample_sources(k, seed=3000 + i) for i in range(args.warmup + args.steps)]
+    batches = [hg.sample_sources(k, seed=3000 + i) for i in range(warmup + steps)]
     dist = gu.DeviceBuffer(4 * k * n)
     sig = gu.DeviceBuffer(8 * k * n)
     cfg = gu.make_config(args.variant, direction=args.direction, batch_width=args.width or 256,
@@ -130,12 +162,12 @@ def bench_grid(args, gu, props):
         tuned = None
         if not args.no_tune and args.variant == "auto" and args.engine == 0:
             tuned = dg.autotune(hg.sample_sources(k, seed=3998), gu.QUERY_BFS)
-        for i in range(args.warmup):
+        for i in range(warmup):
             dg.multi_search_device(batches[i], dist, sig, cfg)
         gu.synchronize()
         stats = []
         t1 = time.perf_counter()
-        for i in range(args.warmup, args.warmup + args.steps):
+        for i in range(warmup, warmup + steps):
             stats.append(dg.multi_search_device(batches[i], dist, sig, cfg))
         gu.synchronize()
         wall = time.perf_counter() - t1
@@ -146,6 +178,8 @@ def bench_grid(args, gu, props):
         t2 = time.perf_counter()
         diam, dst = dg.diameter(dsrc, dcfg)
         dwall = time.perf_counter() - t2
+    dist.close()
+    sig.close()
     units = sum(s.teps_units for s in stats)
     dev_ms = sum(s.device_ms for s in stats)
     levels = stats[-1].max_level + 1
@@ -154,33 +188,35 @@ def bench_grid(args, gu, props):
     if not args.no_cpu:
         from oracle import oracle as orc
 
-        sample = batches[args.warmup][:96]
+        sample = batches[warmup][:cpu_sources]
         u, secs = orc.sigma_batch_timed(hg.R, hg.C, sample, 1)
         extra["cpu_baseline"] = {
             "value": round(u / secs / 1e9, 5), "unit": "GTEPS", "cores": 1, "kind": "port",
-            "sample": "the first 96 sources of the first timed batch, sequential BFS + path counts "
-                      "(oracle restating shortest_paths.cu:11-39), %.1f s" % secs}
-    _line("GTEPS 256-source BFS + path counts, 2048^2 grid -10% LCC", round(units / wall / 1e9, 3),
-          "GTEPS", args, wall / args.steps * 1e3, "int32+u64",
-          {"workload": "2048x2048 4-neighbour grid, 10% edges deleted (seeded), largest component; "
-                       "256 sources per step", "n": n, "m_directed": hg.m,
-           "batch_width": args.width or 256, "variant": args.variant, "direction": args.direction,
-           "engine": stats[-1].engine_used,
-           "generator_seconds": round(gen_s, 2), "device": props["name"], "arch": props["arch"]},
-          {"levels_per_step": levels, "us_per_level": round(dev_ms / args.steps / levels * 1e3, 2),
-           "autotuned": tuned, "algorithmic_bytes_per_step": int(alg / args.steps),
-           "roofline": {"bound": "hbm", "achieved": round(alg / (dev_ms / 1e3) / 1e9, 1),
-                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(alg / (dev_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
-                        "traffic": profile_traffic("grid2048_256src_traffic.txt")},
-           "diameter_sampling": {"sources": 64, "diameter": diam, "ms": round(dwall * 1e3, 3),
-                                 "levels": dst.max_level + 1}, **extra})
+            "sample": "the first %d sources of the first timed batch, sequential BFS + path counts "
+                      "(oracle restating shortest_paths.cu:11-39), %.1f s" % (len(sample), secs)}
+    return _leg_result(
+        "GTEPS 256-source BFS + path counts, 2048^2 grid -10% LCC", round(units / wall / 1e9, 3),
+        "GTEPS", steps, warmup, wall / steps * 1e3, "int32+u64",
+        {"workload": "2048x2048 4-neighbour grid, 10% edges deleted (seeded), largest component; "
+                     "256 sources per step", "n": n, "m_directed": hg.m,
+         "batch_width": args.width or 256, "variant": args.variant, "direction": args.direction,
+         "engine": stats[-1].engine_used,
+         "generator_seconds": round(gen_s, 2), "device": props["name"], "arch": props["arch"]},
+        {"levels_per_step": levels, "us_per_level": round(dev_ms / steps / levels * 1e3, 2),
+         "autotuned": tuned,
+         "roofline": _roofline(alg / steps, dev_ms / steps, profile_traffic("grid2048_256src_traffic.txt"),
+                               scope="one 256-source per-source-engine pass (traversal and the "
+                                     "row write-back kernels)"),
+         "diameter_sampling": {"sources": 64, "diameter": diam, "ms": round(dwall * 1e3, 3),
+                               "levels": dst.max_level + 1}, **extra})
 
 
-def bench_bc(args, gu, props):
+def bench_bc(args, gu, props, steps=None, cpu_sources=8):
     """Sampled Brandes BC on R-MAT scale 22 (ef16): 1024 sources per step,
     dependencies in FP32 with compensated sums, BC accumulated in FP64; the
-    batch width (64/128/256) that runs fastest is reported."""
+    batch width (64/128/256) that runs fastest is reported.  Returns the
+    result dict."""
+    steps = max(args.steps if steps is None else steps, 1)
     t0 = time.time()
     hg = gu.HostGraph.rmat(22, 16, 0.57, 0.19, 0.19, seed=22)
     gen_s = time.time() - t0
@@ -199,13 +235,13 @@ def bench_bc(args, gu, props):
             gu.synchronize()
             t1 = time.perf_counter()
             units = 0
-            for _ in range(max(args.steps, 1)):
+            for _ in range(steps):
                 bc.zero()
                 st = dg.betweenness_device(src, None, bc, cfg)
                 units += st.teps_units
             gu.synchronize()
-            wall = (time.perf_counter() - t1) / max(args.steps, 1)
-            res[width] = (wall, units / max(args.steps, 1), st)
+            wall = (time.perf_counter() - t1) / steps
+            res[width] = (wall, units / steps, st)
         best = min(res, key=lambda w: res[w][0])
         check = {}
         if args.bc_check:
@@ -223,37 +259,38 @@ def bench_bc(args, gu, props):
             check = {"sources": int(len(ks)), "max_rel_err": float(rel.max(initial=0.0)),
                      "zero_where_oracle_zero": bool(np.all(got[~nz] == 0)),
                      "oracle_seconds": round(time.time() - t2, 1)}
+    bc.close()
     wall, units, st = res[best]
     alg = model_bytes_bc(n, k, (k + best - 1) // best, st)
-    extra = {"roofline": {"bound": "hbm", "achieved": round(alg / (st.device_ms / 1e3) / 1e9, 1),
-                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": round(alg / (st.device_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
-                          "traffic": profile_traffic("bc_rmat22_1024src_traffic.txt"),
-                          "scope": "one 1024-source Brandes call at the best width (forward, "
-                                   "backward and FP64 reduce kernels); traffic measured at width 256",
-                          "algorithmic_bytes_per_step": alg, "pass_ms": round(st.device_ms, 3)}}
+    extra = {"roofline": _roofline(alg, st.device_ms, profile_traffic("bc_rmat22_1024src_traffic.txt"),
+                                   scope="one 1024-source Brandes call at the best width (forward, "
+                                         "backward and FP64 reduce kernels); traffic measured at width 256")}
     if not args.no_cpu:
         from oracle import oracle as orc
 
-        cs = src[:8]
+        cs = src[:cpu_sources]
         u, secs = orc.brandes_timed(hg.R, hg.C, cs)
         extra["cpu_baseline"] = {
             "value": round(u / secs / 1e9, 5), "unit": "GTEPS", "cores": 1, "kind": "port",
             "sample": "the first %d of the 1024 sources, sequential FP32 Brandes (oracle restating "
                       "bc.cu:11-57), %.1f s; the rate applies linearly to all 1024" % (len(cs), secs)}
-    _line("GTEPS sampled BC (1024 sources), R-MAT s22", round(units / wall / 1e9, 3), "GTEPS", args,
-          wall * 1e3, "f32 delta / f64 BC / u64 sigma",
-          {"workload": "R-MAT s22 ef16, 1024 sampled non-isolated sources per step", "n": n,
-           "m_directed": hg.m, "generator_seconds": round(gen_s, 2), "device": props["name"],
-           "arch": props["arch"]},
-          {"batch_width": best, "autotuned": tuned,
-           "ms_by_width": {str(w): round(res[w][0] * 1e3, 3) for w in res},
-           "levels_last_batch": st.max_level + 1, "bc_check": check, **extra})
+    return _leg_result(
+        "GTEPS sampled BC (1024 sources), R-MAT s22", round(units / wall / 1e9, 3), "GTEPS", steps, 1,
+        wall * 1e3, "f32 delta / f64 BC / u64 sigma",
+        {"workload": "R-MAT s22 ef16, 1024 sampled non-isolated sources per step", "n": n,
+         "m_directed": hg.m, "generator_seconds": round(gen_s, 2), "device": props["name"],
+         "arch": props["arch"]},
+        {"batch_width": best, "autotuned": tuned,
+         "ms_by_width": {str(w): round(res[w][0] * 1e3, 3) for w in res},
+         "levels_last_batch": st.max_level + 1, "bc_check": check, **extra})
 
 
-def bench_lbs(args, gu, props):
+def bench_lbs(args, gu, props, steps=None, warmup=None):
     """Device-wide load-balanced search: 2^26 work items, Zipf(1.1) segment
-    counts capped at 2^20 (int32); a step is scan + search."""
+    counts capped at 2^20 (int32); a step is scan + search.  Returns the
+    result dict."""
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
     W = 1 << 26
     if args.lbs_segments:
         # a fixed segment count (the reference's lbs.cu example uses 100,000):
@@ -270,22 +307,23 @@ def bench_lbs(args, gu, props):
     cb = gu.DeviceBuffer.from_array(c)
     sb = gu.DeviceBuffer(4 * S)
     rb = gu.DeviceBuffer(4 * W)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         gu.lbs_device(cb, S, sb, rb, W)
     gu.synchronize()
     dev = 0.0
     t1 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         _, st = gu.lbs_device(cb, S, sb, rb, W)
         dev += st.device_ms
     gu.synchronize()
     wall = time.perf_counter() - t1
+    for b in (cb, sb, rb):
+        b.close()
     alg = 8 * S + 4 * W
-    extra = {"roofline": {"bound": "hbm", "achieved": round(alg / (dev / args.steps / 1e3) / 1e9, 1),
-                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": round(alg / (dev / args.steps / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
-                          "traffic": None if args.lbs_segments else profile_traffic("lbs_zipf_2p26_traffic.txt")},
-             "device_ms_per_step": round(dev / args.steps, 4), "segments": S}
+    extra = {"roofline": _roofline(alg, dev / steps,
+                                   None if args.lbs_segments else profile_traffic("lbs_zipf_2p26_traffic.txt"),
+                                   scope="one scan + search call"),
+             "device_ms_per_step": round(dev / steps, 4), "segments": S}
     if not args.no_cpu:
         from oracle import oracle as orc
 
@@ -293,11 +331,12 @@ def bench_lbs(args, gu, props):
         extra["cpu_baseline"] = {"value": round(W / secs / 1e9, 4), "unit": "G items/s", "cores": 1,
                                  "kind": "port", "sample": "the full 2^26-item input, "
                                  "scan + merge (lbs.cu:18-31), %.2f s" % secs}
-    _line("G items/s load-balanced search, 2^26 items, Zipf(1.1)", round(W / (dev / args.steps / 1e3) / 1e9, 3),
-          "G items/s", args, wall / args.steps * 1e3, "int32",
-          {"workload": ("2^26 work items in %d Zipf(1.1)-weighted segments" % S if args.lbs_segments
-                        else "2^26 work items in %d Zipf(1.1) segments (cap 2^20)" % S),
-           "device": props["name"], "arch": props["arch"]}, extra)
+    return _leg_result(
+        "G items/s load-balanced search, 2^26 items, Zipf(1.1)", round(W / (dev / steps / 1e3) / 1e9, 3),
+        "G items/s", steps, warmup, wall / steps * 1e3, "int32",
+        {"workload": ("2^26 work items in %d Zipf(1.1)-weighted segments" % S if args.lbs_segments
+                      else "2^26 work items in %d Zipf(1.1) segments (cap 2^20)" % S),
+         "device": props["name"], "arch": props["arch"]}, extra)
 
 
 def _serve_queries(gu, hgs, nq, seed):
@@ -320,13 +359,18 @@ def _pct(x, p):
     return round(float(np.percentile(np.asarray(x), p)), 2) if len(x) else None
 
 
-def _served(gu, dgs, qs, arrivals_ms, budget, slots, max_pack):
+def _served(gu, dgs, qs, arrivals_ms, budget, slots, max_pack, host_budget=0):
     """Runs the stream through the server; returns (latencies ms, makespan ms,
-    stats).  A collector thread takes finished queries as they come and
-    releases them; latency = done - scheduled arrival on the server clock."""
+    stats).  With host_budget > 0 every query asks for its results in host
+    memory (GU_QF_HOST: the server copies them to its pinned host arena
+    before reporting the query done), else they stay in HBM.  A collector
+    thread takes finished queries as they come and releases them; latency =
+    done (results where they were asked for) - scheduled arrival, on the
+    server clock."""
     import threading
 
-    with gu.Server(hbm_budget=budget, slots=slots, max_pack=max_pack) as srv:
+    host = host_budget > 0
+    with gu.Server(hbm_budget=budget, slots=slots, max_pack=max_pack, host_budget=host_budget) as srv:
         ids = [srv.add_graph(dg) for dg in dgs]
         # warm-up: every slot sees every kind at full width on every graph
         warm = []
@@ -334,7 +378,7 @@ def _served(gu, dgs, qs, arrivals_ms, budget, slots, max_pack):
             src = np.arange(256, dtype=np.int32) % dg.n
             for kind, fl in ((gu.QUERY_BFS, gu.QF_SIGMA), (gu.QUERY_BC, 0), (gu.QUERY_DIAMETER, 0)):
                 for _ in range(slots):
-                    warm.append(srv.submit(ids[g], kind, src, fl))
+                    warm.append(srv.submit(ids[g], kind, src, fl | (gu.QF_HOST if host else 0)))
         for t in warm:
             srv.wait(t)
             srv.release(t)
@@ -355,7 +399,7 @@ def _served(gu, dgs, qs, arrivals_ms, budget, slots, max_pack):
             wait = start + arrivals_ms[i] - srv.now_ms()
             if wait > 0.2:
                 time.sleep(wait / 1e3)
-            tick[i] = srv.submit(ids[g], kind, src, fl)
+            tick[i] = srv.submit(ids[g], kind, src, fl | (gu.QF_HOST if host else 0))
         th.join()
         st = srv.stats()
     lat = [done[tick[i]][0] - (start + arrivals_ms[i]) for i in range(nq)]
@@ -400,18 +444,43 @@ def _plain(gu, dgs, qs, arrivals_ms, to_host):
     return lat, makespan
 
 
+def _output_bytes(gu, hgs, qs):
+    """Bytes of results the stream asks for: distance + path-count rows of
+    BFS queries, BC vectors, diameters."""
+    b = 0
+    for g, kind, src, fl in qs:
+        n = hgs[g].n
+        b += len(src) * n * 12 if kind == gu.QUERY_BFS else (8 * n if kind == gu.QUERY_BC else 4)
+    return b
+
+
+def _leg(lat, mk, nq):
+    return {"qps": round(nq / (mk / 1e3), 2), "p50_ms": _pct(lat, 50), "p99_ms": _pct(lat, 99)}
+
+
 def serve_stream(args, gu, hgs, tune, plain_host=True):
-    """Runs the seeded serving stream over the resident graphs `hgs`:
-    saturated throughput (every query queued at t=0), then Poisson arrivals
-    at 70% and 90% of that rate through the server, and the 90% arrivals
-    through the plain path (one query per call, serial).  Returns the result
-    dict and the 90% run's makespan in ms."""
+    """Runs the seeded serving stream over the resident graphs `hgs`, for two
+    result destinations:
+      host: results in host memory (the survey's definition of a served
+        query: arrival -> host-visible result).  Served = the server with
+        GU_QF_HOST; plain = one query per call through the drop-in pattern
+        (allocate outputs, launch, copy to pageable host memory, free),
+        serial, in arrival order.
+      device: results left in HBM on both sides.
+    For each destination: the server's saturated rate (every query queued at
+    t=0), then Poisson arrivals at 90% of that rate through the server and,
+    identical, through the plain path; the device side also at 70%.
+    Returns the result dict and the host-results served run's makespan."""
     nq = args.queries
     qs = _serve_queries(gu, hgs, nq, seed=4242)
     budget = args.budget_gb << 30
+    host_budget = args.host_budget_gb << 30
     names = ("rmat18", "rmat20", "grid1024")
     dgs = [gu.DeviceGraph(h) for h in hgs]
     unit, tuned = {}, {}
+    unit_gaps = np.random.default_rng(99).exponential(1.0, nq)
+    arr_of = lambda rate: (np.cumsum(unit_gaps) - unit_gaps[0]) * (1e3 / rate)
+    zero = np.zeros(nq)
     try:
         if tune:
             for name, h, dg in zip(names, hgs, dgs):
@@ -433,49 +502,50 @@ def serve_stream(args, gu, hgs, tune, plain_host=True):
                 b.close()
             unit[name] = row
         print("serve: unit costs %s" % json.dumps(unit), file=sys.stderr, flush=True)
-        zero = np.zeros(nq)
-        lat_s, mk_s, st_s = _served(gu, dgs, qs, zero, budget, args.slots, 256)
-        sat_qps = nq / (mk_s / 1e3)
-        # the same seeded stream at two fixed offered loads: 70% of the
-        # saturated rate (latency) and 90% (the comparison with the plain
-        # path, which runs the identical arrivals)
-        qps = args.qps if args.qps > 0 else 0.9 * sat_qps
-        unit_gaps = np.random.default_rng(99).exponential(1.0, nq)
-        arr_of = lambda rate: (np.cumsum(unit_gaps) - unit_gaps[0]) * (1e3 / rate)
-        arr, arr70 = arr_of(qps), arr_of(0.7 * sat_qps)
-        print("serve: saturated %.1f q/s; offered %.1f q/s" % (sat_qps, qps), file=sys.stderr, flush=True)
-        lat_70, mk_70, st_70 = _served(gu, dgs, qs, arr70, budget, args.slots, 256)
-        lat_l, mk_l, st_l = _served(gu, dgs, qs, arr, budget, args.slots, 256)
-        print("serve: served runs done", file=sys.stderr, flush=True)
-        lat_pd, mk_pd = _plain(gu, dgs, qs, arr, False)
-        print("serve: plain (device results) done", file=sys.stderr, flush=True)
+        # results to the host
+        lat_hs, mk_hs, st_hs = _served(gu, dgs, qs, zero, budget, args.slots, 256, host_budget)
+        sat_h = nq / (mk_hs / 1e3)
+        qps_h = args.qps if args.qps > 0 else 0.9 * sat_h
+        arr_h = arr_of(qps_h)
+        lat_h, mk_h, st_h = _served(gu, dgs, qs, arr_h, budget, args.slots, 256, host_budget)
+        print("serve: host results, saturated %.1f q/s, served at %.1f q/s offered" % (sat_h, qps_h),
+              file=sys.stderr, flush=True)
         if plain_host:
-            lat_ph, mk_ph = _plain(gu, dgs, qs, arr, True)
+            lat_ph, mk_ph = _plain(gu, dgs, qs, arr_h, True)
             print("serve: plain (host results) done", file=sys.stderr, flush=True)
+        # results left in HBM
+        lat_s, mk_s, st_s = _served(gu, dgs, qs, zero, budget, args.slots, 256)
+        sat_d = nq / (mk_s / 1e3)
+        qps_d = args.qps if args.qps > 0 else 0.9 * sat_d
+        arr_d, arr70 = arr_of(qps_d), arr_of(0.7 * sat_d)
+        lat_70, mk_70, st_70 = _served(gu, dgs, qs, arr70, budget, args.slots, 256)
+        lat_d, mk_d, st_d = _served(gu, dgs, qs, arr_d, budget, args.slots, 256)
+        lat_pd, mk_pd = _plain(gu, dgs, qs, arr_d, False)
+        print("serve: device results, saturated %.1f q/s; plain done" % sat_d, file=sys.stderr, flush=True)
     finally:
         for dg in dgs:
             dg.close()
-    served_qps = nq / (mk_l / 1e3)
-    res = {
-        "queries": nq,
-        "autotuned": tuned,
-        "launch_ms": unit,
-        "saturated": {"served_qps": round(sat_qps, 2), "p50_ms": _pct(lat_s, 50), "p99_ms": _pct(lat_s, 99),
-                      "server": st_s},
-        "served_at_70pct": {"offered_qps": round(0.7 * sat_qps, 2), "qps": round(nq / (mk_70 / 1e3), 2),
-                            "p50_ms": _pct(lat_70, 50), "p99_ms": _pct(lat_70, 99), "server": st_70},
-        "offered_qps": round(qps, 2),
-        "served": {"qps": round(served_qps, 2), "p50_ms": _pct(lat_l, 50), "p99_ms": _pct(lat_l, 99),
-                   "server": st_l},
-        "plain_device_results": {"qps": round(nq / (mk_pd / 1e3), 2), "p50_ms": _pct(lat_pd, 50),
-                                 "p99_ms": _pct(lat_pd, 99)},
-    }
-    res["qps_ratio_vs_plain_device"] = round(served_qps / res["plain_device_results"]["qps"], 2)
+    out_b = _output_bytes(gu, hgs, qs)
+    hostr = {"saturated_qps": round(sat_h, 2), "offered_qps": round(qps_h, 2),
+             "served": dict(_leg(lat_h, mk_h, nq), server=st_h),
+             "results_gb": round(out_b / 1e9, 2),
+             "served_results_gbs": round(out_b / (mk_h / 1e3) / 1e9, 2)}
     if plain_host:
-        res["plain_host_results"] = {"qps": round(nq / (mk_ph / 1e3), 2), "p50_ms": _pct(lat_ph, 50),
-                                     "p99_ms": _pct(lat_ph, 99)}
-        res["qps_ratio_vs_plain_host"] = round(served_qps / res["plain_host_results"]["qps"], 2)
-    return res, mk_l
+        hostr["plain"] = _leg(lat_ph, mk_ph, nq)
+        hostr["like_for_like_ratio"] = round(hostr["served"]["qps"] / hostr["plain"]["qps"], 2)
+        hostr["p99_ratio"] = round(hostr["plain"]["p99_ms"] / hostr["served"]["p99_ms"], 2)
+    devr = {"saturated_qps": round(sat_d, 2), "offered_qps": round(qps_d, 2),
+            "served": dict(_leg(lat_d, mk_d, nq), server=st_d),
+            "served_at_70pct": dict(_leg(lat_70, mk_70, nq), offered_qps=round(0.7 * sat_d, 2)),
+            "plain": _leg(lat_pd, mk_pd, nq)}
+    devr["ratio"] = round(devr["served"]["qps"] / devr["plain"]["qps"], 2)
+    devr["p99_ratio"] = round(devr["plain"]["p99_ms"] / devr["served"]["p99_ms"], 2)
+    res = {"queries": nq, "autotuned": tuned, "launch_ms": unit,
+           "qps": hostr["served"]["qps"], "p50_ms": hostr["served"]["p50_ms"],
+           "p99_ms": hostr["served"]["p99_ms"],
+           "like_for_like_ratio": hostr.get("like_for_like_ratio"),
+           "host_results": hostr, "device_results": devr}
+    return res, mk_h
 
 
 def serve_graphs(gu, rmat20=None):
@@ -502,7 +572,7 @@ def bench_serve(args, gu, props):
     gen_s = time.time() - t0
     res, mk_l = serve_stream(args, gu, hgs, not args.no_tune)
     nq = args.queries
-    _line("served queries/s, mixed BFS/BC/diameter stream over 3 resident graphs", res["served"]["qps"],
+    _line("served queries/s, mixed BFS/BC/diameter stream over 3 resident graphs", res["qps"],
           "queries/s", args, mk_l / nq, "int32+u64 / f32+f64 BC",
           {"workload": SERVE_WORKLOAD % nq,
            "hbm_budget_gb": args.budget_gb, "slots": args.slots, "max_pack": 256,
@@ -519,6 +589,8 @@ def main():
                     "comparison run (0 = 90%% of the measured saturated rate)")
     ap.add_argument("--slots", type=int, default=4, help="serve workload: launches in flight")
     ap.add_argument("--budget-gb", type=int, default=64, help="serve workload: HBM budget for outputs")
+    ap.add_argument("--host-budget-gb", type=int, default=64,
+                    help="serve workload: pinned host memory for results delivered to the host")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -538,6 +610,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true",
                     help="bfs workload: skip the serving stream reported beside the GTEPS figure")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (degree-profile defaults)")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="bfs workload: skip the grid, BC and LBS legs reported beside the headline")
     ap.add_argument("--level-graphs", type=int, default=0,
                     help="batched engine: 1 replays captured chunks of levels as HIP graphs, 0 launches "
                          "the kernels directly")
@@ -562,9 +636,11 @@ def main():
         gu.synchronize()
         a.close()
         b.close()
+    if args.workload == "serve":
+        bench_serve(args, gu, props)
+        return
     if args.workload != "bfs":
-        {"grid": bench_grid, "bc": bench_bc, "lbs": bench_lbs,
-         "serve": bench_serve}[args.workload](args, gu, props)
+        _print_leg(args, {"grid": bench_grid, "bc": bench_bc, "lbs": bench_lbs}[args.workload](args, gu, props))
         return
     t0 = time.time()
     hg = gu.HostGraph.rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=20)
@@ -608,7 +684,7 @@ def main():
             e2e_units += st.teps_units
         e2e_gteps = e2e_units / (time.perf_counter() - t2) / 1e9
 
-        variants = {}
+        variants, variants_td, td_levels = {}, {}, {}
         if not args.no_variants:
             for name in ["shuffle", "warp", "cta_warp", "scan", "edge_parallel", "linear"]:
                 vc = gu.make_config(name)
@@ -619,6 +695,18 @@ def main():
                     u += st.teps_units
                     ms += st.device_ms
                 variants[name] = round(u / (ms / 1e3) / 1e9, 2)
+            # every level top-down (the reference kernels' direction), so
+            # the expanders themselves are compared; the first 5 timed batches
+            for name in ["shuffle", "warp", "cta_warp", "scan", "edge_parallel", "linear"]:
+                vc = gu.make_config(name, direction=gu.DIR_TOPDOWN)
+                dg.multi_search_device(batches[0], dist, sig, vc)
+                u, ms = 0, 0.0
+                for i in range(args.warmup, min(nb, args.warmup + 5)):
+                    st = dg.multi_search_device(batches[i], dist, sig, vc)
+                    u += st.teps_units
+                    ms += st.device_ms
+                variants_td[name] = round(u / (ms / 1e3) / 1e9, 2)
+                td_levels[name] = st.topdown_levels
 
     gteps = units / wall / 1e9
     achieved = alg_bytes / (dev_ms / 1e3) / 1e9
@@ -647,22 +735,25 @@ def main():
         "bottomup_levels_last": stats[-1].bottomup_levels,
         "variant_used": stats[-1].variant_used,
         "autotuned": tuned,
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": profile_traffic("bfs_rmat20_64src_traffic.txt"),
-            "scope": "one batched search pass (all kernels of the launch sequence); algorithmic "
-                     "bytes = per-source model summed over the batch",
-            "algorithmic_bytes_per_step": int(alg_bytes / args.steps),
-            "pass_ms": round(dev_ms / args.steps, 4),
-            "measured_copy_gbs": copy_gbs(gu),
-        },
+        "roofline": _roofline(alg_bytes / args.steps, dev_ms / args.steps,
+                              profile_traffic("bfs_rmat20_64src_traffic.txt"),
+                              scope="one batched search pass (all kernels of the launch sequence); "
+                                    "algorithmic bytes = per-source model summed over the batch",
+                              measured_copy_gbs=copy_gbs(gu)),
         "variants_gteps": variants,
+        "variants_topdown_gteps": variants_td,
         "pcie_inclusive_gteps": round(e2e_gteps, 2),
     }
+    dist.close()
+    sig.close()
+    if not args.no_legs:
+        # the other benchmark configurations, each with its roofline and CPU
+        # baseline, on fewer steps than their own --workload lines
+        t0 = time.time()
+        out["grid"] = bench_grid(args, gu, props, steps=3, warmup=1, cpu_sources=32)
+        out["bc"] = bench_bc(args, gu, props, steps=1, cpu_sources=4)
+        out["lbs"] = bench_lbs(args, gu, props, steps=20, warmup=3)
+        out["legs_seconds"] = round(time.time() - t0, 1)
     if not args.no_serve:
         # the metric's second half: the mixed query stream over three resident
         # graphs (the R-MAT s20 above is one of them), served and plain

@@ -66,7 +66,7 @@ EXPORTS = (
 )
 
 QUERY_BFS, QUERY_BC, QUERY_DIAMETER = 0, 1, 2
-QF_SIGMA, QF_DELTA = 1, 2
+QF_SIGMA, QF_DELTA, QF_HOST = 1, 2, 4
 QR_SIGMA_WRAPPED = 1  # gu_query_result.flags: a path count wrapped mod 2^64
 ERR_CAPACITY, ERR_TIMEOUT = -9, -10
 
@@ -134,7 +134,7 @@ _P = ctypes.c_void_p
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_int64), ("slots", ctypes.c_int32),
                 ("max_pack", ctypes.c_int32), ("variant", ctypes.c_int32),
-                ("engine", ctypes.c_int32)]
+                ("engine", ctypes.c_int32), ("host_budget", ctypes.c_int64)]
 
 
 class Query(ctypes.Structure):
@@ -149,7 +149,9 @@ class QueryResult(ctypes.Structure):
                 ("sigma_dev", ctypes.c_void_p), ("delta_dev", ctypes.c_void_p),
                 ("bc_dev", ctypes.c_void_p), ("submit_ms", ctypes.c_double),
                 ("admit_ms", ctypes.c_double), ("first_launch_ms", ctypes.c_double),
-                ("done_ms", ctypes.c_double), ("hbm_bytes", ctypes.c_int64)]
+                ("done_ms", ctypes.c_double), ("hbm_bytes", ctypes.c_int64),
+                ("dist_host", ctypes.c_void_p), ("sigma_host", ctypes.c_void_p),
+                ("delta_host", ctypes.c_void_p), ("bc_host", ctypes.c_void_p)]
 
 
 class ServerStats(ctypes.Structure):
@@ -157,7 +159,9 @@ class ServerStats(ctypes.Structure):
                 ("rejected", ctypes.c_int64), ("launches", ctypes.c_int64),
                 ("sources", ctypes.c_int64), ("hbm_in_use", ctypes.c_int64),
                 ("hbm_peak", ctypes.c_int64), ("hbm_budget", ctypes.c_int64),
-                ("waiting", ctypes.c_int32), ("active", ctypes.c_int32)]
+                ("waiting", ctypes.c_int32), ("active", ctypes.c_int32),
+                ("host_in_use", ctypes.c_int64), ("host_peak", ctypes.c_int64),
+                ("host_budget", ctypes.c_int64)]
 
     def as_dict(self):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}
@@ -660,9 +664,9 @@ class Server:
     """Continuous-batching query server (gu_server_*): resident graphs,
     HBM-budget admission, shared multi-source launches on `slots` streams."""
 
-    def __init__(self, hbm_budget=0, slots=0, max_pack=0, variant="auto", engine=ENGINE_AUTO):
+    def __init__(self, hbm_budget=0, slots=0, max_pack=0, variant="auto", engine=ENGINE_AUTO, host_budget=0):
         c = ServerConfig(hbm_budget, slots, max_pack,
-                         VARIANTS[variant] if isinstance(variant, str) else int(variant), engine)
+                         VARIANTS[variant] if isinstance(variant, str) else int(variant), engine, host_budget)
         h = ctypes.c_void_p()
         _check(lib().gu_server_create(ctypes.byref(c), ctypes.byref(h)), "gu_server_create")
         self.h = h
@@ -707,7 +711,8 @@ class Server:
         return s.as_dict()
 
     def fetch(self, res, graph):
-        """Host copies of a finished query's outputs: dict of arrays."""
+        """Copies of a finished query's outputs: dict of arrays (from the
+        device block, or from the host arena for GU_QF_HOST queries)."""
         n, k = self.n[graph], res.nsrc
         out = {"status": res.status, "diameter": res.diameter, "flags": res.flags}
 
@@ -717,14 +722,19 @@ class Server:
                 _check(lib().gu_memcpy_d2h(_ptr(a), ctypes.c_void_p(addr), a.nbytes), "d2h")
             return a
 
-        if res.dist_dev:
-            out["dist"] = get(res.dist_dev, np.int32, k * n).reshape(k, n)
-        if res.sigma_dev:
-            out["sigma"] = get(res.sigma_dev, np.uint64, k * n).reshape(k, n)
-        if res.delta_dev:
-            out["delta"] = get(res.delta_dev, np.float32, k * n).reshape(k, n)
-        if res.bc_dev:
-            out["bc"] = get(res.bc_dev, np.float64, n)
+        def host(addr, dtype, count):
+            buf = (ctypes.c_char * (count * np.dtype(dtype).itemsize)).from_address(addr)
+            return np.frombuffer(buf, dtype=dtype, count=count).copy()
+
+        for key, dt, cnt in (("dist", np.int32, k * n), ("sigma", np.uint64, k * n),
+                             ("delta", np.float32, k * n), ("bc", np.float64, n)):
+            h, d = getattr(res, key + "_host"), getattr(res, key + "_dev")
+            if h:
+                out[key] = host(h, dt, cnt)
+            elif d:
+                out[key] = get(d, dt, cnt)
+            if key in out and key != "bc":
+                out[key] = out[key].reshape(k, n)
         return out
 
     def close(self):

@@ -18,6 +18,7 @@ namespace {
 struct GpuExec final : gus::Exec {
   int device = 0;
   char *arena = nullptr;
+  char *host = nullptr;  // pinned host arena (results delivered to the host)
   int slots = 0;
   gu_config base{};
   std::vector<hipStream_t> streams;
@@ -44,6 +45,20 @@ struct GpuExec final : gus::Exec {
   int zero(int slot, void *p, size_t bytes) override {
     return hipMemsetAsync(p, 0, bytes, streams[(size_t)slot]) == hipSuccess ? GU_OK : GU_ERR_HIP;
   }
+  char *host_base() override { return host; }
+  int to_host(int slot, void *dst, const void *src, size_t bytes) override {
+    hipStream_t st = streams[(size_t)slot];
+    if (hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st) != hipSuccess) return GU_ERR_HIP;
+    return hipStreamSynchronize(st) == hipSuccess ? GU_OK : GU_ERR_HIP;
+  }
+  void free_all() {
+    for (hipStream_t x : streams) (void)hipStreamDestroy(x);
+    streams.clear();
+    if (arena) (void)hipFree(arena);
+    if (host) (void)hipHostFree(host);
+    arena = nullptr;
+    host = nullptr;
+  }
 };
 
 }  // namespace
@@ -60,7 +75,8 @@ int gu_server_create(const gu_server_config *cfgp, gu_server_t *out) {
   *out = nullptr;
   gu_server_config c{};
   if (cfgp) c = *cfgp;
-  if (c.slots < 0 || c.slots > 64 || c.max_pack < 0 || c.max_pack > 256 || c.hbm_budget < 0)
+  if (c.slots < 0 || c.slots > 64 || c.max_pack < 0 || c.max_pack > 256 || c.hbm_budget < 0 ||
+      c.host_budget < 0)
     return GU_ERR_INVALID;
   const int slots = c.slots ? c.slots : 4;
   const int max_pack = c.max_pack ? c.max_pack : 256;
@@ -83,23 +99,27 @@ int gu_server_create(const gu_server_config *cfgp, gu_server_t *out) {
     delete s;
     return GU_ERR_NOMEM;
   }
+  if (c.host_budget > 0 &&
+      hipHostMalloc((void **)&s->ex.host, (size_t)c.host_budget, hipHostMallocDefault) != hipSuccess) {
+    s->ex.free_all();
+    delete s;
+    return GU_ERR_NOMEM;
+  }
   s->ex.slots = slots;
   s->ex.base.variant = c.variant ? c.variant : GU_VARIANT_AUTO;
   s->ex.base.engine = c.engine;
   for (int i = 0; i < slots; i++) {
     hipStream_t st;
     if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
-      for (hipStream_t x : s->ex.streams) (void)hipStreamDestroy(x);
-      (void)hipFree(s->ex.arena);
+      s->ex.free_all();
       delete s;
       return GU_ERR_HIP;
     }
     s->ex.streams.push_back(st);
   }
-  s->srv = new (std::nothrow) gus::Server(&s->ex, budget, slots, max_pack);
+  s->srv = new (std::nothrow) gus::Server(&s->ex, budget, slots, max_pack, (size_t)c.host_budget);
   if (!s->srv) {
-    for (hipStream_t x : s->ex.streams) (void)hipStreamDestroy(x);
-    (void)hipFree(s->ex.arena);
+    s->ex.free_all();
     delete s;
     return GU_ERR_NOMEM;
   }
@@ -135,6 +155,8 @@ int gu_server_submit(gu_server_t s, const gu_query *q, int64_t *ticket) {
   sp.graph = q->graph;
   sp.nsrc = q->nsrc;
   sp.sources = q->sources;
+  sp.to_host = (q->flags & GU_QF_HOST) != 0;
+  if (sp.to_host && !s->ex.host) return GU_ERR_INVALID;  // the server has no host arena
   switch (q->kind) {
     case GU_QUERY_BFS: sp.cls = (q->flags & GU_QF_SIGMA) ? gus::C_BFS_SIGMA : gus::C_BFS; break;
     case GU_QUERY_BC: sp.cls = gus::C_BC; sp.want_delta = (q->flags & GU_QF_DELTA) != 0; break;
@@ -155,6 +177,10 @@ static void to_c(const gus::Result &r, gu_query_result *o) {
   o->sigma_dev = r.sigma;
   o->delta_dev = r.delta;
   o->bc_dev = r.bc;
+  o->dist_host = r.h_dist;
+  o->sigma_host = r.h_sigma;
+  o->delta_host = r.h_delta;
+  o->bc_host = r.h_bc;
   o->submit_ms = r.submit_ms;
   o->admit_ms = r.admit_ms;
   o->first_launch_ms = r.first_launch_ms;
@@ -212,6 +238,9 @@ int gu_server_get_stats(gu_server_t s, gu_server_stats *out) {
   out->hbm_budget = st.hbm_budget;
   out->waiting = st.waiting;
   out->active = st.active;
+  out->host_in_use = st.host_in_use;
+  out->host_peak = st.host_peak;
+  out->host_budget = st.host_budget;
   return GU_OK;
 }
 
@@ -221,8 +250,7 @@ int gu_server_destroy(gu_server_t s) {
   (void)hipSetDevice(s->ex.device);
   for (int g = 0; g < s->ex.ngraphs; g++)
     for (int i = 0; i < s->ex.slots; i++) gu_graph_destroy(s->ex.views[g][i]);
-  for (hipStream_t x : s->ex.streams) (void)hipStreamDestroy(x);
-  (void)hipFree(s->ex.arena);
+  s->ex.free_all();
   delete s;
   return GU_OK;
 }

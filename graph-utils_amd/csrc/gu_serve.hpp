@@ -18,6 +18,13 @@
 // source; BC sums a query's sources in source order within a launch and
 // launch by launch (FP64), so only the grouping of that FP64 sum can differ.
 //
+// A query may ask for its results on the host: it then also holds a block
+// of a second arena in pinned host memory; after each launch the worker
+// copies that launch's rows of such queries to their host blocks (and a BC
+// vector after the query's last launch) while the other slots keep
+// computing, and the query is reported done only when its results are in
+// host memory.  Its device block is released as soon as it is done.
+//
 // This header holds the scheduling logic only; the executor that runs a
 // launch is an interface, so the same logic runs over the HIP engine in the
 // library and over a CPU stand-in in the scheduler's unit test.
@@ -105,6 +112,10 @@ struct Exec {
   virtual int bc(int slot, int graph, const int32_t *src, int k, float *const *xrows,
                  double *const *targets, int32_t *wrap) = 0;
   virtual int zero(int slot, void *p, size_t bytes) = 0;
+  // Host arena (pinned) and a device-to-host copy on the slot's stream that
+  // returns once the bytes are in host memory.
+  virtual char *host_base() { return nullptr; }
+  virtual int to_host(int slot, void *dst, const void *src, size_t bytes) = 0;
 };
 
 struct Spec {
@@ -113,6 +124,7 @@ struct Spec {
   int nsrc = 0;
   const int32_t *sources = nullptr;
   bool want_delta = false;  // BC: keep the per-source dependency rows
+  bool to_host = false;     // deliver the results to the host arena
 };
 
 constexpr int kFlagWrapped = 1;  // a path count of one of the query's sources wrapped
@@ -123,6 +135,12 @@ struct Result {
   uint64_t *sigma = nullptr;
   float *delta = nullptr;
   double *bc = nullptr;
+  // host copies of a query submitted with to_host (the device pointers
+  // above are then null: the device block is already released)
+  int32_t *h_dist = nullptr;
+  uint64_t *h_sigma = nullptr;
+  float *h_delta = nullptr;
+  double *h_bc = nullptr;
   double submit_ms = 0, admit_ms = 0, first_launch_ms = 0, done_ms = 0;
   size_t bytes = 0;
 };
@@ -131,6 +149,7 @@ struct Stats {
   int64_t submitted = 0, completed = 0, rejected = 0, launches = 0, sources = 0;
   int64_t hbm_in_use = 0, hbm_peak = 0, hbm_budget = 0;
   int32_t waiting = 0, active = 0;
+  int64_t host_in_use = 0, host_peak = 0, host_budget = 0;
 };
 
 // Arena bytes a query holds: its output rows and BC vector.
@@ -145,9 +164,9 @@ inline size_t query_bytes(int cls, int64_t k, int64_t n, bool want_delta) {
 
 class Server {
  public:
-  Server(Exec *ex, size_t budget, int slots, int max_pack)
-      : ex_(ex), arena_(budget), slots_(std::max(1, slots)), max_pack_(std::max(1, max_pack)),
-        t0_(std::chrono::steady_clock::now()) {
+  Server(Exec *ex, size_t budget, int slots, int max_pack, size_t host_budget = 0)
+      : ex_(ex), arena_(budget), host_arena_(host_budget), slots_(std::max(1, slots)),
+        max_pack_(std::max(1, max_pack)), t0_(std::chrono::steady_clock::now()) {
     for (int i = 0; i < slots_; i++) workers_.emplace_back([this, i] { worker(i); });
   }
   ~Server() { stop(); }
@@ -188,6 +207,7 @@ class Server {
       q->cls = s.cls;
       q->nsrc = s.nsrc;
       q->want_delta = s.want_delta;
+      q->to_host = s.to_host;
       q->src.assign(s.sources, s.sources + s.nsrc);
       q->n = n;
       q->bytes = query_bytes(s.cls, s.nsrc, n, s.want_delta);
@@ -197,7 +217,7 @@ class Server {
       stats_.submitted++;
       Q *raw = q.get();
       queries_[q->ticket] = std::move(q);
-      if (raw->bytes > arena_.cap()) {  // can never fit: reject now
+      if (raw->bytes > arena_.cap() || (raw->to_host && raw->bytes > host_arena_.cap())) {  // never fits
         raw->status = kErrCapacity;
         finish(raw);
         stats_.rejected++;
@@ -268,7 +288,7 @@ class Server {
       auto it = queries_.find(ticket);
       if (it == queries_.end() || !it->second->done) return kErrInvalid;
       Q *q = it->second.get();
-      if (q->admitted) arena_.release(q->off, q->bytes);
+      release_blocks(q);
       queries_.erase(it);
       admit();
     }
@@ -284,6 +304,9 @@ class Server {
     s.hbm_peak = (int64_t)arena_.peak();
     s.hbm_budget = (int64_t)arena_.cap();
     s.waiting = (int32_t)waiting_.size();
+    s.host_in_use = (int64_t)host_arena_.used();
+    s.host_peak = (int64_t)host_arena_.peak();
+    s.host_budget = (int64_t)host_arena_.cap();
     int a = 0;
     for (auto &kv : queries_) a += kv.second->admitted && !kv.second->done;
     s.active = a;
@@ -295,13 +318,18 @@ class Server {
     int64_t ticket = 0;
     int graph = 0, cls = 0, nsrc = 0;
     int64_t n = 0;
-    bool want_delta = false;
+    bool want_delta = false, to_host = false;
     std::vector<int32_t> src;
-    size_t bytes = 0, off = 0;
+    size_t bytes = 0, off = 0, host_off = 0;
+    bool dev_held = false, host_held = false;
     int32_t *dist = nullptr;
     uint64_t *sigma = nullptr;
     float *delta = nullptr;
     double *bc = nullptr;
+    int32_t *h_dist = nullptr;
+    uint64_t *h_sigma = nullptr;
+    float *h_delta = nullptr;
+    double *h_bc = nullptr;
     bool bc_zeroed = false;
     int diameter = 0;
     int next = 0, done_src = 0, inflight = 0, launches = 0, status = 0, flags = 0;
@@ -329,6 +357,10 @@ class Server {
     r->sigma = q->sigma;
     r->delta = q->delta;
     r->bc = q->bc;
+    r->h_dist = q->h_dist;
+    r->h_sigma = q->h_sigma;
+    r->h_delta = q->h_delta;
+    r->h_bc = q->h_bc;
     r->submit_ms = q->t_submit;
     r->admit_ms = q->t_admit;
     r->first_launch_ms = q->t_first;
@@ -337,35 +369,72 @@ class Server {
   }
 
   // Caller holds mu_.
+  void release_blocks(Q *q) {
+    if (q->dev_held) arena_.release(q->off, q->bytes);
+    if (q->host_held) host_arena_.release(q->host_off, q->bytes);
+    q->dev_held = q->host_held = false;
+  }
+
+  // Caller holds mu_.  A query delivered to the host gives its device block
+  // back at once (its device pointers become null).
   void finish(Q *q) {
     q->done = true;
     q->t_done = now_ms();
     done_fifo_.push_back(q->ticket);
     if (q->status == 0) stats_.completed++;
+    if (q->to_host && q->dev_held) {
+      arena_.release(q->off, q->bytes);
+      q->dev_held = false;
+      q->dist = nullptr;
+      q->sigma = nullptr;
+      q->delta = nullptr;
+      q->bc = nullptr;
+      admit();
+    }
   }
 
   // FIFO admission under the budget (caller holds mu_).
   void admit() {
     while (!waiting_.empty()) {
       Q *q = waiting_.front();
-      size_t off = 0;
+      size_t off = 0, hoff = 0;
       if (!arena_.alloc(q->bytes, &off)) break;
+      if (q->to_host && !host_arena_.alloc(q->bytes, &hoff)) {
+        arena_.release(off, q->bytes);
+        break;
+      }
       waiting_.pop_front();
       q->off = off;
+      q->host_off = hoff;
+      q->dev_held = true;
+      q->host_held = q->to_host;
       q->admitted = true;
       q->t_admit = now_ms();
       q->seq = seq_++;
       char *b = ex_->arena_base() + off;
+      char *hb = q->to_host ? ex_->host_base() + hoff : nullptr;
       const size_t n = (size_t)q->n, k = (size_t)q->nsrc;
+      // the host block has the device block's layout
       switch (q->cls) {
         case C_BFS_SIGMA:
           q->dist = (int32_t *)b;
           q->sigma = (uint64_t *)(b + Arena::round(k * n * 4));
+          if (hb) {
+            q->h_dist = (int32_t *)hb;
+            q->h_sigma = (uint64_t *)(hb + Arena::round(k * n * 4));
+          }
           break;
-        case C_BFS: q->dist = (int32_t *)b; break;
+        case C_BFS:
+          q->dist = (int32_t *)b;
+          if (hb) q->h_dist = (int32_t *)hb;
+          break;
         case C_BC:
           q->bc = (double *)b;
           if (q->want_delta) q->delta = (float *)(b + Arena::round(n * 8));
+          if (hb) {
+            q->h_bc = (double *)hb;
+            if (q->want_delta) q->h_delta = (float *)(hb + Arena::round(n * 8));
+          }
           break;
         default: break;
       }
@@ -472,6 +541,18 @@ class Server {
             break;
         }
       }
+      // results wanted on the host: this launch's rows of those queries,
+      // and a BC vector once its query's last launch is done (a BC query is
+      // in one launch at a time and its sources go out in order)
+      for (const Item &it : p.items) {
+        Q *q = it.q;
+        if (rc || !q->to_host) continue;
+        const size_t j0 = (size_t)it.first, c = (size_t)it.count;
+        if (q->dist) rc = ex_->to_host(slot, q->h_dist + j0 * n, q->dist + j0 * n, c * n * 4);
+        if (!rc && q->sigma) rc = ex_->to_host(slot, q->h_sigma + j0 * n, q->sigma + j0 * n, c * n * 8);
+        if (!rc && q->delta) rc = ex_->to_host(slot, q->h_delta + j0 * n, q->delta + j0 * n, c * n * 4);
+        if (!rc && q->bc && it.first + it.count == q->nsrc) rc = ex_->to_host(slot, q->h_bc, q->bc, n * 8);
+      }
       {
         std::lock_guard<std::mutex> lk(mu_);
         int pos = 0;
@@ -501,7 +582,7 @@ class Server {
   }
 
   Exec *ex_;
-  Arena arena_;
+  Arena arena_, host_arena_;
   const int slots_, max_pack_;
   const std::chrono::steady_clock::time_point t0_;
   std::mutex mu_;

@@ -292,6 +292,9 @@ typedef struct gu_server *gu_server_t;
 #define GU_QUERY_DIAMETER 2  /* max eccentricity over the sources */
 #define GU_QF_SIGMA 1        /* BFS: also path counts */
 #define GU_QF_DELTA 2        /* BC: also per-source dependency rows */
+#define GU_QF_HOST 4         /* deliver the outputs to host memory: the query
+                                is done once they are in the server's pinned
+                                host arena (gu_server_config.host_budget > 0) */
 /* Result flag: a shortest-path count of one of the query's sources wrapped
  * modulo 2^64 (some reached vertex has at least 2^64 shortest paths, as on
  * large grids).  Distances stay exact and path counts exact mod 2^64, as the
@@ -307,6 +310,8 @@ typedef struct {
   int32_t max_pack;    /* sources per launch, <= 256 (0 = 256) */
   int32_t variant;     /* GU_VARIANT_* for every launch (0 = AUTO) */
   int32_t engine;      /* GU_ENGINE_* for every launch (0 = AUTO) */
+  int64_t host_budget; /* bytes of pinned host memory for GU_QF_HOST query
+                          outputs (0 = none: such queries are refused) */
 } gu_server_config;
 
 typedef struct {
@@ -331,8 +336,15 @@ typedef struct {
   double submit_ms;         /* server clock (ms since creation) */
   double admit_ms;
   double first_launch_ms;
-  double done_ms;
+  double done_ms;           /* with GU_QF_HOST: outputs in host memory */
   int64_t hbm_bytes;        /* arena bytes held until release */
+  /* GU_QF_HOST: the outputs in the server's pinned host arena, valid until
+   * gu_server_release (the *_dev pointers are then NULL: the query's device
+   * block is given back as soon as the copy is done) */
+  int32_t *dist_host;
+  uint64_t *sigma_host;
+  float *delta_host;
+  double *bc_host;
 } gu_query_result;
 
 typedef struct {
@@ -341,6 +353,7 @@ typedef struct {
   int64_t hbm_in_use, hbm_peak, hbm_budget;
   int32_t waiting;          /* submitted, not yet admitted */
   int32_t active;           /* admitted, not finished */
+  int64_t host_in_use, host_peak, host_budget;  /* pinned host arena */
 } gu_server_stats;
 
 /* Creates the server on the current device: the output arena, one stream

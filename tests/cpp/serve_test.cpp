@@ -60,12 +60,19 @@ Graph take(int32_t *R, int32_t *C, int32_t *F, int32_t n, int64_t m) {
 }
 
 struct FakeExec final : gus::Exec {
-  std::vector<char> arena;
+  std::vector<char> arena, host;
   std::vector<Graph> graphs;
   int poison = -1;               // a launch holding this source fails
   std::atomic<int> concurrent{0}, max_concurrent{0};
-  explicit FakeExec(size_t budget) : arena(budget) {}
+  explicit FakeExec(size_t budget, size_t host_budget = 0) : arena(budget), host(host_budget) {}
   char *arena_base() override { return arena.data(); }
+  char *host_base() override { return host.data(); }
+  std::atomic<int64_t> copied{0};
+  int to_host(int, void *dst, const void *src, size_t bytes) override {
+    std::memcpy(dst, src, bytes);
+    copied += (int64_t)bytes;
+    return GU_OK;
+  }
   void busy(int k) {
     int c = ++concurrent;
     int m = max_concurrent.load();
@@ -145,10 +152,15 @@ QOut copy_out(const gus::Result &r, int n) {
   o.launches = r.launches;
   o.flags = r.flags;
   const size_t k = (size_t)r.nsrc, N = (size_t)n;
-  if (r.dist) o.dist.assign(r.dist, r.dist + k * N);
-  if (r.sigma) o.sigma.assign(r.sigma, r.sigma + k * N);
-  if (r.delta) o.delta.assign(r.delta, r.delta + k * N);
-  if (r.bc) o.bc.assign(r.bc, r.bc + N);
+  // host-delivered queries: results in the host arena, device pointers null
+  const int32_t *d = r.h_dist ? r.h_dist : r.dist;
+  const uint64_t *s = r.h_sigma ? r.h_sigma : r.sigma;
+  const float *x = r.h_delta ? r.h_delta : r.delta;
+  const double *b = r.h_bc ? r.h_bc : r.bc;
+  if (d) o.dist.assign(d, d + k * N);
+  if (s) o.sigma.assign(s, s + k * N);
+  if (x) o.delta.assign(x, x + k * N);
+  if (b) o.bc.assign(b, b + N);
   return o;
 }
 
@@ -188,8 +200,11 @@ std::vector<QSpec> make_queries(const std::vector<Graph> &gs, int count, uint64_
   return qs;
 }
 
+bool g_to_host = false;  // submit every query with results delivered to the host
+
 gus::Spec spec_of(const QSpec &q) {
   gus::Spec s;
+  s.to_host = g_to_host;
   s.graph = q.graph;
   s.cls = q.cls;
   s.nsrc = q.nsrc;
@@ -220,10 +235,10 @@ std::vector<QOut> run_alone(const std::vector<Graph> &gs, const std::vector<QSpe
 }
 
 std::vector<QOut> run_mix(const std::vector<Graph> &gs, const std::vector<QSpec> &qs, size_t budget,
-                          int slots, int max_pack, int *max_conc, gus::Stats *st) {
-  FakeExec ex(budget);
+                          int slots, int max_pack, int *max_conc, gus::Stats *st, size_t host_budget = 0) {
+  FakeExec ex(budget, host_budget);
   ex.graphs = gs;
-  gus::Server srv(&ex, budget, slots, max_pack);
+  gus::Server srv(&ex, budget, slots, max_pack, host_budget);
   for (const Graph &g : gs) {
     int id;
     srv.add_graph(g.n, &id);
@@ -387,6 +402,49 @@ int main() {
     s.nsrc = 1;
     CHECK(srv.submit(s, &tt) == gus::kErrInvalid, "out-of-range source refused");
   }
+  // results delivered to the host: the same answers, the device block given
+  // back once a query is done, admission throttled by the host budget too
+  {
+    g_to_host = true;
+    gus::Stats hst;
+    int hconc = 0;
+    const std::vector<QOut> hmix = run_mix(gs, qs, largest * 3, 4, 16, &hconc, &hst, largest * 4);
+    g_to_host = false;
+    compare(gs, qs, alone, hmix, "host mix");
+    CHECK(hst.completed == (int64_t)qs.size(), "host mix completed %lld", (long long)hst.completed);
+    CHECK(hst.hbm_in_use == 0 && hst.host_in_use == 0, "host mix: everything released");
+    CHECK(hst.host_peak > 0 && hst.host_peak <= (int64_t)(largest * 4), "host peak %lld", (long long)hst.host_peak);
+
+    FakeExec ex(1 << 20, 1 << 20);
+    ex.graphs = gs;
+    gus::Server srv(&ex, 1 << 20, 2, 8, 1 << 20);
+    int id;
+    srv.add_graph(gs[0].n, &id);
+    std::vector<int32_t> src = {1, 2, 3};
+    gus::Spec s;
+    s.graph = 0;
+    s.cls = gus::C_BFS_SIGMA;
+    s.nsrc = 3;
+    s.sources = src.data();
+    s.to_host = true;
+    int64_t t;
+    CHECK(srv.submit(s, &t) == 0, "submit host query");
+    gus::Result r;
+    CHECK(srv.wait(t, 5000, &r) == 0 && r.status == 0, "host query done");
+    gus::Stats a = srv.stats();
+    CHECK(!r.dist && !r.sigma && r.h_dist && r.h_sigma, "host query: host pointers only");
+    CHECK(a.hbm_in_use == 0 && a.host_in_use > 0, "device block back at completion (%lld, %lld)",
+          (long long)a.hbm_in_use, (long long)a.host_in_use);
+    CHECK(ex.copied.load() == (int64_t)(3 * gs[0].n * 12), "copied %lld bytes", (long long)ex.copied.load());
+    CHECK(srv.release(t) == 0 && srv.stats().host_in_use == 0, "host block released");
+    // a server without a host arena cannot hold host results
+    gus::Server none(&ex, 1 << 20, 1, 8);
+    none.add_graph(gs[0].n, &id);
+    CHECK(none.submit(s, &t) == 0 && none.wait(t, 5000, &r) == 0 && r.status == gus::kErrCapacity,
+          "host query without a host arena is refused (%d)", r.status);
+    none.release(t);
+  }
+
   // wrapped path counts are flagged per query: on the intact 40 x 40 grid
   // the counts from a corner reach C(78, 39) > 2^64, those from the centre
   // stay below 2^38

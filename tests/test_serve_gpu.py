@@ -207,6 +207,60 @@ def test_server_alone_vs_mix(gu, orc, gpu):
                 assert np.max(np.abs(b["bc"][pos] - ref[pos]) / ref[pos]) <= 1e-5
 
 
+def test_server_host_delivery(gu, orc, gpu):
+    """GU_QF_HOST: outputs copied to the server's pinned host arena before a
+    query is reported done; the same answers as device-resident results,
+    device pointers NULL and the device block given back at completion."""
+    announce("server host delivery")
+    graphs = [gu.HostGraph.rmat(12, 16, seed=51), gu.HostGraph.grid(48, 0.1, seed=52),
+              gu.HostGraph.parse_metis(graph_path("jazz"))]
+    dgs = [gu.DeviceGraph(h) for h in graphs]
+    try:
+        qs = _make_queries(gu, graphs, 40, seed=9, max_src=80)
+        big = max(len(q[2]) for q in qs) * max(h.n for h in graphs) * 12
+        dev, _ = _run(gu, dgs, qs, 3 * big, 4, 64, False)
+        hq = [(g, k, src, fl | gu.QF_HOST) for (g, k, src, fl) in qs]
+        out = [None] * len(hq)
+        with gu.Server(hbm_budget=3 * big, slots=4, max_pack=64, host_budget=4 * big) as srv:
+            ids = [srv.add_graph(dg) for dg in dgs]
+            tick = {srv.submit(ids[g], k, src, fl): i for i, (g, k, src, fl) in enumerate(hq)}
+            for _ in hq:
+                t, r = srv.next_done(60000)
+                i = tick[t]
+                assert r.status == 0
+                assert not (r.dist_dev or r.sigma_dev or r.delta_dev or r.bc_dev)
+                out[i] = srv.fetch(r, ids[hq[i][0]])
+                srv.release(t)
+            st = srv.stats()
+        assert st["completed"] == len(hq) and st["hbm_in_use"] == 0 and st["host_in_use"] == 0
+        assert 0 < st["host_peak"] <= 4 * big
+        with gu.Server(hbm_budget=big, slots=1) as srv:  # no host arena: refused
+            gid = srv.add_graph(dgs[2])
+            with pytest.raises(gu.GuError) as e:
+                srv.submit(gid, gu.QUERY_BFS, np.array([1], np.int32), gu.QF_HOST)
+            assert e.value.status == gu.ERR_INVALID
+    finally:
+        for dg in dgs:
+            dg.close()
+    for i, (g, kind, src, flags) in enumerate(qs):
+        a, b = dev[i], out[i]
+        for key in ("dist", "sigma", "delta"):
+            assert (key in a) == (key in b), (i, key)
+            if key in a:
+                assert np.array_equal(a[key], b[key]), (i, key)
+        assert a["diameter"] == b["diameter"] and a["flags"] == b["flags"]
+        if kind == gu.QUERY_BC:
+            nz = a["bc"] != 0
+            assert np.all(b["bc"][~nz] == 0)
+            if nz.any():
+                assert np.max(np.abs(a["bc"][nz] - b["bc"][nz]) / np.abs(a["bc"][nz])) <= 1e-12
+        if kind == gu.QUERY_BFS:
+            d_o, s_o = _oracle_rows(orc, graphs[g], src[:3])
+            assert np.array_equal(b["dist"][:3], d_o)
+            if "sigma" in b:
+                assert np.array_equal(b["sigma"][:3], s_o)
+
+
 def test_server_rejects_oversized_query(gu, gpu):
     announce("server oversized query")
     hg = gu.HostGraph.rmat(10, 8, seed=2)
