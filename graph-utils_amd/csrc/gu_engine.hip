@@ -1607,36 +1607,65 @@ int run_persource(gu_graph *g, const int *sources, int nsrc, const HostRows &hr,
     const int4 *LE = rl ? g->LE : nullptr;
     PsArgs a{rl ? g->LR : g->R, rl ? g->LC : g->C, LE, g->n, p.srcs, (int *const *)g->rowtab,
              sigma ? (u64 *const *)(g->rowtab + 256) : nullptr, p.queues, p.depth, p.part, p.ovf};
-    // undirected graphs pull path counts from the previous level
-    if (LE) {
-      if (sigma && !g->directed)
-        k_persource<true, true, true><<<nb, 1024, 0, st>>>(a);
-      else if (sigma)
-        k_persource<true, false, true><<<nb, 1024, 0, st>>>(a);
+    auto fetch = [&]() -> int {
+      GU_HIP(hipMemcpyAsync(part, p.part, sizeof(u64) * 4 * nb, hipMemcpyDeviceToHost, st));
+      GU_HIP(hipMemcpyAsync(depth, p.depth, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
+      GU_HIP(hipMemcpyAsync(g->misc_h, p.ovf, sizeof(int), hipMemcpyDeviceToHost, st));
+      GU_HIP(hipStreamSynchronize(st));
+      return GU_OK;
+    };
+    // undirected degree <= 4 graphs: the LDS level-set search (rows, when
+    // wanted, through the distance row); a launch in which a level set
+    // overflowed runs again on the distance-reading kernel
+    bool lds = LE && !g->directed;
+    const bool rows = hr.dist != nullptr || sigma;
+    if (lds) {
+      PsLdsArgs c{LE, g->n, p.srcs, rows ? (int *const *)g->rowtab : nullptr,
+                  sigma ? (u64 *const *)(g->rowtab + 256) : nullptr, p.queues, p.depth, p.part, p.ovf};
+      if (sigma)
+        k_ps_lds<true, true><<<nb, 1024, 0, st>>>(c);
+      else if (rows)
+        k_ps_lds<false, true><<<nb, 1024, 0, st>>>(c);
       else
-        k_persource<false, false, true><<<nb, 1024, 0, st>>>(a);
-    } else if (sigma && !g->directed) {
-      k_persource<true, true><<<nb, 1024, 0, st>>>(a);
-    } else if (sigma) {
-      k_persource<true, false><<<nb, 1024, 0, st>>>(a);
-    } else {
-      k_persource<false, false><<<nb, 1024, 0, st>>>(a);
+        k_ps_lds<false, false><<<nb, 1024, 0, st>>>(c);
+      GU_HIP(hipGetLastError());
+      if ((rc = fetch())) return rc;
+      for (int i = 0; i < nb; i++) lds &= (part[4 * i + 3] & 2) == 0;
+      if (!lds) k_fill_i32<<<1, 64, 0, st>>>(p.ovf, 1, 0);
     }
-    GU_HIP(hipGetLastError());
+    if (!lds) {
+      // undirected graphs pull path counts from the previous level
+      if (LE) {
+        if (sigma && !g->directed)
+          k_persource<true, true, true><<<nb, 1024, 0, st>>>(a);
+        else if (sigma)
+          k_persource<true, false, true><<<nb, 1024, 0, st>>>(a);
+        else
+          k_persource<false, false, true><<<nb, 1024, 0, st>>>(a);
+      } else if (sigma && !g->directed) {
+        k_persource<true, true><<<nb, 1024, 0, st>>>(a);
+      } else if (sigma) {
+        k_persource<true, false><<<nb, 1024, 0, st>>>(a);
+      } else {
+        k_persource<false, false><<<nb, 1024, 0, st>>>(a);
+      }
+      GU_HIP(hipGetLastError());
+    }
     if (rl) {
       const dim3 ug((unsigned)std::max(1, std::min(pick_grid(g, 1), (g->n + 255) / 256)), (unsigned)nb);
       if (hr.dist)
         k_ps_unpermute<int><<<ug, 256, 0, st>>>((int *const *)g->rowtab, (int *const *)(g->rowtab + 1024),
                                                 g->nof, g->n);
-      if (sigma)
+      if (sigma && lds)  // path counts written for reached vertices only
+        k_ps_unpermute_reach<u64><<<ug, 256, 0, st>>>((u64 *const *)(g->rowtab + 256),
+                                                      (u64 *const *)(g->rowtab + 1280), g->nof,
+                                                      (int *const *)g->rowtab, g->n, 0ull);
+      else if (sigma)
         k_ps_unpermute<u64><<<ug, 256, 0, st>>>((u64 *const *)(g->rowtab + 256),
                                                 (u64 *const *)(g->rowtab + 1280), g->nof, g->n);
       GU_HIP(hipGetLastError());
     }
-    GU_HIP(hipMemcpyAsync(part, p.part, sizeof(u64) * 4 * nb, hipMemcpyDeviceToHost, st));
-    GU_HIP(hipMemcpyAsync(depth, p.depth, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
-    GU_HIP(hipMemcpyAsync(g->misc_h, p.ovf, sizeof(int), hipMemcpyDeviceToHost, st));
-    GU_HIP(hipStreamSynchronize(st));
+    if (!lds && (rc = fetch())) return rc;
     ovf_any |= g->misc_h[0];
     for (int i = 0; i < nb; i++) {
       s.teps_units += (int64_t)part[4 * i];
@@ -1644,7 +1673,7 @@ int run_persource(gu_graph *g, const int *sources, int nsrc, const HostRows &hr,
       s.dag_edges += (int64_t)part[4 * i + 2];
       dmax = std::max(dmax, depth[i]);
       if (hr.ecc) hr.ecc[b0 + i] = depth[i];
-      if (hr.wrap) hr.wrap[b0 + i] = sigma && part[4 * i + 3] != 0;
+      if (hr.wrap) hr.wrap[b0 + i] = sigma && (part[4 * i + 3] & 1) != 0;
     }
     s.batches++;
   }
@@ -1701,15 +1730,28 @@ int run_persource_bc(gu_graph *g, const int *sources, int nsrc, const HostRows &
     PsBcArgs a{rl ? g->LR : g->R, rl ? g->LC : g->C, LE, g->n, p.srcs, (int *const *)g->rowtab,
                (u64 *const *)(g->rowtab + 256), (float *const *)(g->rowtab + 512),
                (float *const *)(g->rowtab + 1280), p.queues, p.depth, p.part, p.ovf};
-    if (LE && g->directed)
-      k_ps_bc<false, true><<<nb, 1024, 0, st>>>(a);
-    else if (LE)
-      k_ps_bc<true, true><<<nb, 1024, 0, st>>>(a);
-    else if (g->directed)
-      k_ps_bc<false><<<nb, 1024, 0, st>>>(a);
-    else
-      k_ps_bc<true><<<nb, 1024, 0, st>>>(a);
-    GU_HIP(hipGetLastError());
+    // undirected degree <= 4 graphs: the LDS level-set forward; a launch in
+    // which a level set overflowed runs again on k_ps_bc
+    bool lds = LE && !g->directed;
+    if (lds) {
+      k_ps_bc_lds<<<nb, 1024, 0, st>>>(a);
+      GU_HIP(hipGetLastError());
+      GU_HIP(hipMemcpyAsync(part, p.part, sizeof(u64) * 4 * nb, hipMemcpyDeviceToHost, st));
+      GU_HIP(hipStreamSynchronize(st));
+      for (int i = 0; i < nb; i++) lds &= (part[4 * i + 3] & 2) == 0;
+      if (!lds) k_fill_i32<<<1, 64, 0, st>>>(p.ovf, 1, 0);
+    }
+    if (!lds) {
+      if (LE && g->directed)
+        k_ps_bc<false, true><<<nb, 1024, 0, st>>>(a);
+      else if (LE)
+        k_ps_bc<true, true><<<nb, 1024, 0, st>>>(a);
+      else if (g->directed)
+        k_ps_bc<false><<<nb, 1024, 0, st>>>(a);
+      else
+        k_ps_bc<true><<<nb, 1024, 0, st>>>(a);
+      GU_HIP(hipGetLastError());
+    }
     if (rl && hr.delta) {
       const dim3 ug((unsigned)std::max(1, std::min(pick_grid(g, 1), (g->n + 255) / 256)), (unsigned)nb);
       k_ps_unpermute<float><<<ug, 256, 0, st>>>((float *const *)(g->rowtab + 512),
@@ -1730,7 +1772,7 @@ int run_persource_bc(gu_graph *g, const int *sources, int nsrc, const HostRows &
       s.reached += (int64_t)part[4 * i + 1];
       s.dag_edges += (int64_t)part[4 * i + 2];
       dmax = std::max(dmax, depth[i]);
-      if (hr.wrap) hr.wrap[b0 + i] = part[4 * i + 3] != 0;
+      if (hr.wrap) hr.wrap[b0 + i] = (part[4 * i + 3] & 1) != 0;
     }
     s.batches++;
   }

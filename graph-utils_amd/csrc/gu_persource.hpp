@@ -621,4 +621,305 @@ __global__ __launch_bounds__(256) void k_ps_unpermute(T *const *src, T *const *d
   for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < (size_t)n; v += stride) d[v] = s[nof[v]];
 }
 
+// ---- undirected degree <= 4 graphs: the last levels as LDS hash sets ----
+//
+// On an undirected graph every neighbour of a vertex at BFS level L lies at
+// level L-1, L or L+1 (or is not reached yet), so a source's search needs
+// only the vertex sets of the three levels around the current one, not its
+// distance row.  Those sets live in the workgroup's LDS as four rotating
+// open-addressing hash sets (4-slot buckets probed linearly, vertex ids, -1
+// empty): level L
+// reads the sets of L-1 (parents) and L (same level), inserts claims into
+// the set of L+1 with an LDS compare-and-swap (each vertex claimed once),
+// and clears the set that held L-2, which takes the claims of L+2 after
+// this level's barrier.  Per frontier vertex the only global reads are its
+// int4 of neighbours and its parents' path counts (written one level
+// before); distances and counts are stored once, when a vertex is
+// expanded.  A level set that cannot place a claim within kLdsProbe buckets
+// stops the source (flag 2 in part[3]) and the host reruns the launch on the
+// distance-reading kernels, so results never depend on LDS capacity.
+constexpr int kLdsBucketBits = 11;
+constexpr int kLdsBuckets = 1 << kLdsBucketBits;  // 4-slot buckets per level set (8192 slots)
+constexpr int kLdsProbe = 16;                     // buckets probed before giving up
+constexpr int kLdsFront = 3072;                   // LDS frontier entries per buffer
+struct PsLds {
+  int4 tab[4][kLdsBuckets];  // 128 KB
+  int fv[2][kLdsFront];      // 24 KB; entries beyond go to HBM
+};
+
+__device__ __forceinline__ unsigned lds_bucket(int w) {
+  return ((unsigned)w * 0x9E3779B1u) >> (32 - kLdsBucketBits);
+}
+__device__ __forceinline__ bool q4_has(const int4 q, int w) { return q.x == w || q.y == w || q.z == w || q.w == w; }
+__device__ __forceinline__ bool q4_open(const int4 q) { return q.x < 0 || q.y < 0 || q.z < 0 || q.w < 0; }
+
+// Buckets are probed linearly, one 16-byte LDS read each, and fill from
+// their first slot: a key moves on to the next bucket only when its bucket
+// is full, and nothing leaves a set during a level, so a bucket with a free
+// slot ends the search.  At most kLdsProbe buckets (an insert gives up
+// beyond).  With levels of up to 4096 vertices (load <= 1/2) almost every
+// lookup reads one bucket, so the lanes of a wave stay in step.
+__device__ __forceinline__ bool lds_has(const int4 *t, int w) {
+  unsigned b = lds_bucket(w);
+  for (int p = 0; p < kLdsProbe; p++) {
+    const int4 q = t[b];
+    if (q4_has(q, w)) return true;
+    if (q4_open(q)) return false;
+    b = (b + 1) & (kLdsBuckets - 1);
+  }
+  return false;
+}
+
+// Inserts w: 1 when this call placed it, 0 when it was there, -1 when no
+// slot within kLdsProbe buckets was free.  A lost race on a slot re-reads
+// the bucket.
+__device__ __forceinline__ int lds_insert(int4 *t, int w) {
+  unsigned b = lds_bucket(w);
+  for (int p = 0, tries = 0; p < kLdsProbe && tries < 4 * kLdsProbe; tries++) {
+    const int4 q = t[b];
+    if (q4_has(q, w)) return 0;
+    int k = q.x < 0 ? 0 : (q.y < 0 ? 1 : (q.z < 0 ? 2 : (q.w < 0 ? 3 : 4)));
+    if (k == 4) {
+      b = (b + 1) & (kLdsBuckets - 1);
+      p++;
+      continue;
+    }
+    const int x = atomicCAS(reinterpret_cast<int *>(&t[b]) + k, -1, w);
+    if (x < 0) return 1;
+    if (x == w) return 0;
+  }
+  return -1;
+}
+
+// Expansion of frontier vertex v at level L (-1 for an idle thread).
+template <bool SIGMA, bool WANT_D>
+__device__ __forceinline__ void ps_visit_lds(const int4 *LE, int *d, u64 *sg, int v, int L, const int4 *tp,
+                                             const int4 *tl, int4 *tn, int &ovf, int &bad, u64 &dag, u64 &teps,
+                                             unsigned *cnt, PsLds &f, int *spill, int *S, int nbuf) {
+  const int4 q = v >= 0 ? LE[v] : make_int4(-1, -1, -1, -1);
+  const int w[4] = {q.x, q.y, q.z, q.w};
+  bool par[4], got[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    par[j] = got[j] = false;
+    if (w[j] < 0) continue;
+    teps++;
+    if (L >= 1 && lds_has(tp, w[j])) {
+      par[j] = true;
+    } else if (!lds_has(tl, w[j])) {
+      const int r = lds_insert(tn, w[j]);
+      got[j] = r > 0;
+      bad |= r < 0;
+    }
+  }
+  u64 x[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    if (SIGMA) {
+      const u64 t = ld_wg(sg + (par[j] ? w[j] : 0));
+      x[j] = par[j] ? t : 0ull;
+    }
+  }
+  unsigned claims = 0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) claims += got[j];
+  unsigned pos = claims ? atomicAdd(cnt, claims) : 0u;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    if (!got[j]) continue;
+    if (S) S[pos] = w[j];
+    if (pos < (unsigned)kLdsFront)
+      f.fv[nbuf][pos] = w[j];
+    else
+      spill[pos - kLdsFront] = w[j];
+    pos++;
+  }
+  if (SIGMA) {
+    u64 acc = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (par[j]) {
+        acc += x[j];
+        ovf |= acc < x[j];
+        dag++;
+      }
+    }
+    if (L >= 1 && v >= 0) sg[v] = acc;
+  }
+  if (WANT_D && v >= 0) d[v] = L;
+}
+
+// The level loop over LDS level sets (one workgroup barrier a level).  With
+// BC the levels are kept in S (E[l] = start of level l) and a frontier's
+// entries beyond the LDS capacity are read back from S; otherwise they go to
+// the two `spill` halves.  Returns the deepest level, or -1 when a level set
+// overflowed.
+template <bool SIGMA, bool BC, bool WANT_D>
+__device__ int ps_levels_lds(const int4 *LE, int n, int s, int *d, u64 *sg, int *spill0, int *S, int *E, PsLds &f,
+                             unsigned *s_cnt, int *s_bad, int &ovf, u64 &teps, u64 &reached, u64 &dag) {
+  const int tid = threadIdx.x, T = blockDim.x;
+  for (int i = tid; i < 4 * kLdsBuckets; i += T) (&f.tab[0][0])[i] = make_int4(-1, -1, -1, -1);
+  if (tid < 3) s_cnt[tid] = 0;
+  if (tid == 0) *s_bad = 0;
+  __syncthreads();
+  if (tid == 0) {
+    f.tab[0][lds_bucket(s)].x = s;
+    if (SIGMA) sg[s] = 1;
+    f.fv[0][0] = s;
+    if (BC) {
+      S[0] = s;
+      E[0] = 0;
+      E[1] = 1;
+    }
+  }
+  __syncthreads();
+  unsigned qn = 1, qb = 0, qs = 1;  // frontier size; its start and end in S
+  int L = 0, b = 0, bad = 0;
+  reached = 1;
+  while (true) {
+    unsigned *cnt = &s_cnt[L % 3];
+    if (tid == 0) s_cnt[(L + 1) % 3] = 0;
+    const int4 *tp = f.tab[(L + 3) & 3], *tl = f.tab[L & 3];
+    int4 *tn = f.tab[(L + 1) & 3], *tc = f.tab[(L + 2) & 3];
+    for (int i = tid; i < kLdsBuckets; i += T) tc[i] = make_int4(-1, -1, -1, -1);  // held level L-2
+    int *Snext = BC ? S + qs : nullptr;
+    const int *sp_cur = BC ? S + qb + kLdsFront : spill0 + (size_t)b * n;
+    int *sp_next = BC ? S + qs + kLdsFront : spill0 + (size_t)(b ^ 1) * n;
+    for (unsigned base = 0; base < qn; base += T) {
+      const unsigned k = base + tid;
+      const int v = k >= qn ? -1 : (k < (unsigned)kLdsFront ? f.fv[b][k] : sp_cur[k - kLdsFront]);
+      ps_visit_lds<SIGMA, WANT_D || BC>(LE, d, sg, v, L, tp, tl, tn, ovf, bad, dag, teps, cnt, f, sp_next, Snext,
+                                        b ^ 1);
+    }
+    if (bad) *s_bad = 1;
+    __syncthreads();
+    const unsigned q2n = *cnt;
+    if (*s_bad) return -1;
+    if (q2n == 0) break;
+    reached += q2n;
+    qn = q2n;
+    qb = qs;
+    qs += q2n;
+    b ^= 1;
+    L++;
+    if (BC && tid == 0) E[L + 1] = (int)qs;
+  }
+  return L;
+}
+
+struct PsLdsArgs {
+  const int4 *LE;
+  int n;
+  const int *srcs;    // [nsrc] (cluster-order ids)
+  int *const *dist;   // [nsrc] rows, or null (no rows wanted: diameter)
+  u64 *const *sigma;  // [nsrc] rows, or null
+  int *queues;        // [nsrc][2n+2] frontier spill
+  int *depth;
+  u64 *part;          // [nsrc][4], as PsArgs; part[3] bit 1: level set overflow
+  int *ovf;
+};
+
+// The search: with rows (WANT_D) the distance row is first set to INT_MAX,
+// the only row initialised: it tells reached vertices apart when the rows
+// are written back (k_ps_unpermute_reach); path counts are written for
+// reached vertices only.
+template <bool SIGMA, bool WANT_D>
+__global__ __launch_bounds__(1024) void k_ps_lds(PsLdsArgs a) {
+  __shared__ PsLds f;
+  __shared__ unsigned s_cnt[3];
+  __shared__ u64 s_red[16][2];
+  __shared__ int s_ovf, s_bad;
+  const int tid = threadIdx.x, T = blockDim.x;
+  const int s = a.srcs[blockIdx.x];
+  int *d = WANT_D ? a.dist[blockIdx.x] : nullptr;
+  u64 *sg = SIGMA ? a.sigma[blockIdx.x] : nullptr;
+  int *spill = a.queues + (size_t)blockIdx.x * (2 * (size_t)a.n + 2);
+  if (WANT_D)
+    for (int i = tid; i < a.n; i += T) d[i] = INT_MAX;
+  if (tid == 0) s_ovf = 0;
+  __syncthreads();
+  int ovf = 0;
+  u64 teps = 0, reached = 1, dag = 0;
+  const int L = ps_levels_lds<SIGMA, false, WANT_D>(a.LE, a.n, s, d, sg, spill, nullptr, nullptr, f, s_cnt, &s_bad,
+                                                    ovf, teps, reached, dag);
+  ps_finish(teps, dag, reached, L < 0 ? 0 : L, ovf, &s_ovf, s_red, a.part, a.depth, a.ovf);
+  if (tid == 0 && L < 0) a.part[4 * blockIdx.x + 3] |= 2ull;
+}
+
+// Brandes over LDS level sets: the forward above with the BFS order in S
+// and every reached vertex's distance written, then the backward of k_ps_bc
+// (successors d = l+1; on an undirected graph every neighbour of a reached
+// vertex is reached, so only written distances are read).  The dependency
+// row is set to 0 first (the BC reduce and write-back read every vertex).
+__global__ __launch_bounds__(1024) void k_ps_bc_lds(PsBcArgs a) {
+  __shared__ PsLds f;
+  __shared__ unsigned s_cnt[3];
+  __shared__ u64 s_red[16][2];
+  __shared__ int s_ovf, s_bad;
+  const int tid = threadIdx.x, T = blockDim.x;
+  const int s = a.srcs[blockIdx.x];
+  int *d = a.dist[blockIdx.x];
+  u64 *sg = a.sigma[blockIdx.x];
+  float *dl = a.delta[blockIdx.x];
+  float *ql = a.quot[blockIdx.x];
+  int *S = a.stack + (size_t)blockIdx.x * (2 * (size_t)a.n + 2);
+  int *E = S + a.n;
+  for (int i = tid; i < a.n; i += T) dl[i] = 0.0f;
+  if (tid == 0) s_ovf = 0;
+  __syncthreads();
+  int ovf = 0;
+  u64 teps = 0, reached = 1, dag = 0;
+  const int L = ps_levels_lds<true, true, true>(a.LE, a.n, s, d, sg, nullptr, S, E, f, s_cnt, &s_bad, ovf, teps,
+                                                reached, dag);
+  __syncthreads();
+  if (L < 0) {  // rerun by the host on k_ps_bc
+    ps_finish(teps, dag, reached, 0, ovf, &s_ovf, s_red, a.part, a.depth, a.ovf);
+    if (tid == 0) a.part[4 * blockIdx.x + 3] |= 2ull;
+    return;
+  }
+  for (int i = E[L] + tid; L >= 1 && i < E[L + 1]; i += T) {
+    const int v = S[i];
+    ql[v] = 1.0f / (float)ld_wg(&sg[v]);
+  }
+  __syncthreads();
+  for (int l = L - 1; l >= 1; l--) {
+    for (int i = E[l] + tid; i < E[l + 1]; i += T) {
+      const int v = S[i];
+      const float sv = (float)ld_wg(&sg[v]);
+      const int4 q4 = a.LE[v];
+      const int w[4] = {q4.x, q4.y, q4.z, q4.w};
+      bool on[4];
+      float qw[4];
+      ps_successors<4>(d, ql, w, l, on, qw);
+      float sum = 0.0f, comp = 0.0f;
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        if (on[j]) ps_neumaier(sum, comp, sv * qw[j]);
+      const float dv = sum + comp;
+      dl[v] = dv;
+      ql[v] = (1.0f + dv) / sv;
+    }
+    __syncthreads();
+  }
+  ps_finish(teps, dag, reached, L, ovf, &s_ovf, s_red, a.part, a.depth, a.ovf);
+}
+
+// Rows back to the caller's order where only reached vertices hold values:
+// dst[b][v] = src[b][nof[v]] when the distance row dr[b] shows that vertex
+// reached, else fill.  blockIdx.y selects the row.
+template <class T>
+__global__ __launch_bounds__(256) void k_ps_unpermute_reach(T *const *src, T *const *dst, const int *nof,
+                                                            int *const *dr, int n, T fill) {
+  T *dd = dst[blockIdx.y];
+  if (!dd) return;
+  const T *ss = src[blockIdx.y];
+  const int *dv = dr[blockIdx.y];
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < (size_t)n; v += stride) {
+    const int x = nof[v];
+    const T val = ss[x];
+    dd[v] = dv[x] != INT_MAX ? val : fill;
+  }
+}
+
 }  // namespace gu
