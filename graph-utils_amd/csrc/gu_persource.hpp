@@ -833,7 +833,7 @@ struct PsLdsArgs {
 
 // The search: with rows (WANT_D) the distance row is first set to INT_MAX,
 // the only row initialised: it tells reached vertices apart when the rows
-// are written back (k_ps_unpermute_reach); path counts are written for
+// are written back (k_ps_unpermute_ds); path counts are written for
 // reached vertices only.
 template <bool SIGMA, bool WANT_D>
 __global__ __launch_bounds__(1024) void k_ps_lds(PsLdsArgs a) {
@@ -916,21 +916,23 @@ __global__ __launch_bounds__(1024) void k_ps_bc_lds(PsBcArgs a) {
   ps_finish(teps, dag, reached, L, ovf, &s_ovf, s_red, a.part, a.depth, a.ovf);
 }
 
-// Rows back to the caller's order where only reached vertices hold values:
-// dst[b][v] = src[b][nof[v]] when the distance row dr[b] shows that vertex
-// reached, else fill.  blockIdx.y selects the row.
-template <class T>
-__global__ __launch_bounds__(256) void k_ps_unpermute_reach(T *const *src, T *const *dst, const int *nof,
-                                                            int *const *dr, int n, T fill) {
-  T *dd = dst[blockIdx.y];
-  if (!dd) return;
-  const T *ss = src[blockIdx.y];
-  const int *dv = dr[blockIdx.y];
+// Both rows of an LDS level-set search back to the caller's order in one
+// pass (nof and the distance read once): dd[b][v] = dist, sd[b][v] = path
+// count where reached (distance below INT_MAX), else 0; either destination
+// may be null.  blockIdx.y selects the row.
+__global__ __launch_bounds__(256) void k_ps_unpermute_ds(int *const *dsrc, u64 *const *ssrc, int *const *ddst,
+                                                         u64 *const *sdst, const int *nof, int n) {
+  int *dd = ddst ? ddst[blockIdx.y] : nullptr;
+  u64 *sd = sdst[blockIdx.y];
+  const int *ds = dsrc[blockIdx.y];
+  const u64 *ss = ssrc[blockIdx.y];
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < (size_t)n; v += stride) {
     const int x = nof[v];
-    const T val = ss[x];
-    dd[v] = dv[x] != INT_MAX ? val : fill;
+    const int dv = ds[x];
+    const u64 sv = ss[x];
+    if (dd) dd[v] = dv;
+    if (sd) sd[v] = dv != INT_MAX ? sv : 0ull;
   }
 }
 
