@@ -1080,6 +1080,7 @@ struct gu_graph {
   // vertex (-1 padded), so a per-source visit is one load and a frontier
   // entry needs no row offsets.
   int4 *LE = nullptr;
+  int lds_flags = 0;          // kLds* facts of an undirected LE graph
   std::vector<int32_t> nof_h;
   Units pu;                   // work units over in-edges (pulls)
   Units ou;                   // work units over out-edges (Brandes backward)
@@ -1622,8 +1623,9 @@ int run_persource(gu_graph *g, const int *sources, int nsrc, const HostRows &hr,
     bool lds = LE && !g->directed && sigma;
     const bool rows = hr.dist != nullptr || sigma;
     if (lds) {
-      PsLdsArgs c{LE, g->n, p.srcs, rows ? (int *const *)g->rowtab : nullptr,
-                  sigma ? (u64 *const *)(g->rowtab + 256) : nullptr, p.queues, p.depth, p.part, p.ovf};
+      PsLdsArgs c{LE,      g->n,   p.srcs, rows ? (int *const *)g->rowtab : nullptr,
+                  sigma ? (u64 *const *)(g->rowtab + 256) : nullptr,
+                  p.queues, p.depth, p.part, p.ovf, g->lds_flags};
       if (sigma)
         k_ps_lds<true, true><<<nb, 1024, 0, st>>>(c);
       else if (rows)
@@ -1656,9 +1658,9 @@ int run_persource(gu_graph *g, const int *sources, int nsrc, const HostRows &hr,
     if (rl) {
       const dim3 ug((unsigned)std::max(1, std::min(pick_grid(g, 1), (g->n + 255) / 256)), (unsigned)nb);
       if (sigma && lds)  // path counts written for reached vertices only; both rows in one pass
-        k_ps_unpermute_ds<<<ug, 256, 0, st>>>((int *const *)g->rowtab, (u64 *const *)(g->rowtab + 256),
-                                              hr.dist ? (int *const *)(g->rowtab + 1024) : nullptr,
-                                              (u64 *const *)(g->rowtab + 1280), g->nof, g->n);
+        k_ps_unpermute_ds<<<dim3(ug.x, (unsigned)((nb + kUnpermRows - 1) / kUnpermRows)), 256, 0, st>>>(
+            (int *const *)g->rowtab, (u64 *const *)(g->rowtab + 256),
+            hr.dist ? (int *const *)(g->rowtab + 1024) : nullptr, (u64 *const *)(g->rowtab + 1280), g->nof, g->n, nb);
       if (hr.dist && !(sigma && lds))
         k_ps_unpermute<int><<<ug, 256, 0, st>>>((int *const *)g->rowtab, (int *const *)(g->rowtab + 1024),
                                                 g->nof, g->n);
@@ -1731,7 +1733,7 @@ int run_persource_bc(gu_graph *g, const int *sources, int nsrc, const HostRows &
     const int4 *LE = rl ? g->LE : nullptr;
     PsBcArgs a{rl ? g->LR : g->R, rl ? g->LC : g->C, LE, g->n, p.srcs, (int *const *)g->rowtab,
                (u64 *const *)(g->rowtab + 256), (float *const *)(g->rowtab + 512),
-               (float *const *)(g->rowtab + 1280), p.queues, p.depth, p.part, p.ovf};
+               (float *const *)(g->rowtab + 1280), p.queues, p.depth, p.part, p.ovf, g->lds_flags};
     // undirected degree <= 4 graphs: the LDS level-set forward; a launch in
     // which a level set overflowed runs again on k_ps_bc
     bool lds = LE && !g->directed;
@@ -2084,6 +2086,35 @@ void cluster_order(int n, const int32_t *R, const int32_t *C, std::vector<int32_
   }
 }
 
+// Bipartite and connected (kLds* bits) for an undirected graph, by one BFS
+// 2-colouring over every component: O(n + m) on the host.
+int lds_graph_facts(int n, const int32_t *R, const int32_t *C) {
+  std::vector<int8_t> col((size_t)n, -1);
+  std::vector<int32_t> q((size_t)n);
+  bool bip = true;
+  int comps = 0;
+  for (int s = 0; s < n; s++) {
+    if (col[(size_t)s] >= 0) continue;
+    comps++;
+    size_t h = 0, t = 0;
+    q[t++] = s;
+    col[(size_t)s] = 0;
+    while (h < t) {
+      const int v = q[h++];
+      for (int k = R[v]; k < R[v + 1]; k++) {
+        const int w = C[k];
+        if (col[(size_t)w] < 0) {
+          col[(size_t)w] = (int8_t)(col[(size_t)v] ^ 1);
+          q[t++] = w;
+        } else if (col[(size_t)w] == col[(size_t)v]) {
+          bip = false;
+        }
+      }
+    }
+  }
+  return (bip ? gu::kLdsBipartite : 0) | (comps == 1 ? gu::kLdsConnected : 0);
+}
+
 int build_cluster_copy(gu_graph *g, const int32_t *R, const int32_t *C) {
   const int n = g->n;
   std::vector<int32_t> oon, LR((size_t)n + 1), LC((size_t)std::max<long long>(g->m, 1));
@@ -2099,6 +2130,7 @@ int build_cluster_copy(gu_graph *g, const int32_t *R, const int32_t *C) {
       dalloc(&g->nof, (size_t)std::max(n, 1)) != hipSuccess)
     return GU_ERR_NOMEM;
   if (g->max_deg <= 4 && n > 0) {
+    if (!g->directed) g->lds_flags = lds_graph_facts(n, R, C);
     std::vector<int4> LE((size_t)n);
     for (int u = 0; u < n; u++) {
       int q[4] = {-1, -1, -1, -1};
@@ -2285,6 +2317,7 @@ int gu_graph_share(gu_graph_t g, gu_graph_t *out) {
   v->CT = root->CT;
   v->LR = root->LR;
   v->LE = root->LE;
+  v->lds_flags = root->lds_flags;
   v->LC = root->LC;
   v->nof = root->nof;
   v->pu = root->pu;

@@ -476,6 +476,7 @@ struct PsBcArgs {
   int *depth;
   u64 *part;
   int *ovf;
+  int lds_flags;        // kLds* graph facts for the level-set kernels
 };
 
 __device__ __forceinline__ void ps_neumaier(float &sum, float &comp, float x) {
@@ -642,6 +643,12 @@ constexpr int kLdsBucketBits = 11;
 constexpr int kLdsBuckets = 1 << kLdsBucketBits;  // 4-slot buckets per level set (8192 slots)
 constexpr int kLdsProbe = 16;                     // buckets probed before giving up
 constexpr int kLdsFront = 3072;                   // LDS frontier entries per buffer
+// Facts about the graph that the host measured when it built the handle.
+// Bipartite: BFS levels are independent sets, so a neighbour of a level-L
+// vertex is never at level L and the same-level lookup is skipped.
+// Connected: every vertex is reached, so the distance row needs no fill.
+constexpr int kLdsBipartite = 1;
+constexpr int kLdsConnected = 2;
 struct PsLds {
   int4 tab[4][kLdsBuckets];  // 128 KB
   int fv[2][kLdsFront];      // 24 KB; entries beyond go to HBM
@@ -671,11 +678,24 @@ __device__ __forceinline__ bool lds_has(const int4 *t, int w) {
 }
 
 // Inserts w: 1 when this call placed it, 0 when it was there, -1 when no
-// slot within kLdsProbe buckets was free.  A lost race on a slot re-reads
-// the bucket.
+// slot within kLdsProbe buckets was free.  The home bucket's slots are
+// claimed in order by compare-and-swap with no read first (at load <= 1/2
+// the first swap usually settles it); its swaps keep the fill order, since
+// slot k is tried only after slots 0..k-1 were seen taken.  Further buckets
+// are read first; a lost race on a slot re-reads the bucket.
 __device__ __forceinline__ int lds_insert(int4 *t, int w) {
   unsigned b = lds_bucket(w);
-  for (int p = 0, tries = 0; p < kLdsProbe && tries < 4 * kLdsProbe; tries++) {
+  {
+    int *sl = reinterpret_cast<int *>(&t[b]);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int x = atomicCAS(sl + k, -1, w);
+      if (x < 0) return 1;
+      if (x == w) return 0;
+    }
+    b = (b + 1) & (kLdsBuckets - 1);
+  }
+  for (int p = 1, tries = 0; p < kLdsProbe && tries < 4 * kLdsProbe; tries++) {
     const int4 q = t[b];
     if (q4_has(q, w)) return 0;
     int k = q.x < 0 ? 0 : (q.y < 0 ? 1 : (q.z < 0 ? 2 : (q.w < 0 ? 3 : 4)));
@@ -698,7 +718,7 @@ __device__ __forceinline__ int lds_insert(int4 *t, int w) {
 template <bool SIGMA, bool WANT_D>
 __device__ __forceinline__ void ps_visit_lds(const int4 q, int *d, u64 *sg, int v, int L, const int4 *tp,
                                              const int4 *tl, int4 *tn, int &ovf, int &bad, u64 &dag, u64 &teps,
-                                             unsigned *cnt, PsLds &f, int *spill, int *S, int nbuf) {
+                                             unsigned *cnt, PsLds &f, int *spill, int *S, int nbuf, bool bip) {
   const int w[4] = {q.x, q.y, q.z, q.w};
   bool par[4], got[4];
 #pragma unroll
@@ -708,7 +728,7 @@ __device__ __forceinline__ void ps_visit_lds(const int4 q, int *d, u64 *sg, int 
     teps++;
     if (L >= 1 && lds_has(tp, w[j])) {
       par[j] = true;
-    } else if (!lds_has(tl, w[j])) {
+    } else if (bip || !lds_has(tl, w[j])) {
       const int r = lds_insert(tn, w[j]);
       got[j] = r > 0;
       bad |= r < 0;
@@ -758,7 +778,7 @@ __device__ __forceinline__ void ps_visit_lds(const int4 q, int *d, u64 *sg, int 
 // overflowed.
 template <bool SIGMA, bool BC, bool WANT_D>
 __device__ int ps_levels_lds(const int4 *LE, int n, int s, int *d, u64 *sg, int *spill0, int *S, int *E, PsLds &f,
-                             unsigned *s_cnt, int *s_bad, int &ovf, u64 &teps, u64 &reached, u64 &dag) {
+                             unsigned *s_cnt, int *s_bad, int &ovf, u64 &teps, u64 &reached, u64 &dag, bool bip) {
   const int tid = threadIdx.x, T = blockDim.x;
   for (int i = tid; i < 4 * kLdsBuckets; i += T) (&f.tab[0][0])[i] = make_int4(-1, -1, -1, -1);
   if (tid < 3) s_cnt[tid] = 0;
@@ -799,7 +819,7 @@ __device__ int ps_levels_lds(const int4 *LE, int n, int s, int *d, u64 *sg, int 
       const int vn = entry(base + T + tid);
       const int4 qn4 = vn >= 0 ? LE[vn] : make_int4(-1, -1, -1, -1);
       ps_visit_lds<SIGMA, WANT_D || BC>(q, d, sg, v, L, tp, tl, tn, ovf, bad, dag, teps, cnt, f, sp_next, Snext,
-                                        b ^ 1);
+                                        b ^ 1, bip);
       v = vn;
       q = qn4;
     }
@@ -829,12 +849,14 @@ struct PsLdsArgs {
   int *depth;
   u64 *part;          // [nsrc][4], as PsArgs; part[3] bit 1: level set overflow
   int *ovf;
+  int flags;          // kLds* graph facts
 };
 
 // The search: with rows (WANT_D) the distance row is first set to INT_MAX,
 // the only row initialised: it tells reached vertices apart when the rows
 // are written back (k_ps_unpermute_ds); path counts are written for
-// reached vertices only.
+// reached vertices only.  On a connected graph every vertex's distance is
+// written by the search, so the fill is skipped.
 template <bool SIGMA, bool WANT_D>
 __global__ __launch_bounds__(1024) void k_ps_lds(PsLdsArgs a) {
   __shared__ PsLds f;
@@ -846,14 +868,14 @@ __global__ __launch_bounds__(1024) void k_ps_lds(PsLdsArgs a) {
   int *d = WANT_D ? a.dist[blockIdx.x] : nullptr;
   u64 *sg = SIGMA ? a.sigma[blockIdx.x] : nullptr;
   int *spill = a.queues + (size_t)blockIdx.x * (2 * (size_t)a.n + 2);
-  if (WANT_D)
+  if (WANT_D && !(a.flags & kLdsConnected))
     for (int i = tid; i < a.n; i += T) d[i] = INT_MAX;
   if (tid == 0) s_ovf = 0;
   __syncthreads();
   int ovf = 0;
   u64 teps = 0, reached = 1, dag = 0;
   const int L = ps_levels_lds<SIGMA, false, WANT_D>(a.LE, a.n, s, d, sg, spill, nullptr, nullptr, f, s_cnt, &s_bad,
-                                                    ovf, teps, reached, dag);
+                                                    ovf, teps, reached, dag, (a.flags & kLdsBipartite) != 0);
   ps_finish(teps, dag, reached, L < 0 ? 0 : L, ovf, &s_ovf, s_red, a.part, a.depth, a.ovf);
   if (tid == 0 && L < 0) a.part[4 * blockIdx.x + 3] |= 2ull;
 }
@@ -882,7 +904,7 @@ __global__ __launch_bounds__(1024) void k_ps_bc_lds(PsBcArgs a) {
   int ovf = 0;
   u64 teps = 0, reached = 1, dag = 0;
   const int L = ps_levels_lds<true, true, true>(a.LE, a.n, s, d, sg, nullptr, S, E, f, s_cnt, &s_bad, ovf, teps,
-                                                reached, dag);
+                                                reached, dag, (a.lds_flags & kLdsBipartite) != 0);
   __syncthreads();
   if (L < 0) {  // rerun by the host on k_ps_bc
     ps_finish(teps, dag, reached, 0, ovf, &s_ovf, s_red, a.part, a.depth, a.ovf);
@@ -917,22 +939,35 @@ __global__ __launch_bounds__(1024) void k_ps_bc_lds(PsBcArgs a) {
 }
 
 // Both rows of an LDS level-set search back to the caller's order in one
-// pass (nof and the distance read once): dd[b][v] = dist, sd[b][v] = path
-// count where reached (distance below INT_MAX), else 0; either destination
-// may be null.  blockIdx.y selects the row.
+// pass: dd[b][v] = dist, sd[b][v] = path count where reached (distance
+// below INT_MAX), else 0; the distance destination may be null.  Each
+// thread serves kUnpermRows rows of one vertex (blockIdx.y selects the
+// group of rows, nrows of them in all), so nof is read once per group and
+// 2 kUnpermRows gathers are in flight.
+constexpr int kUnpermRows = 4;
 __global__ __launch_bounds__(256) void k_ps_unpermute_ds(int *const *dsrc, u64 *const *ssrc, int *const *ddst,
-                                                         u64 *const *sdst, const int *nof, int n) {
-  int *dd = ddst ? ddst[blockIdx.y] : nullptr;
-  u64 *sd = sdst[blockIdx.y];
-  const int *ds = dsrc[blockIdx.y];
-  const u64 *ss = ssrc[blockIdx.y];
+                                                         u64 *const *sdst, const int *nof, int n, int nrows) {
+  const int r0 = blockIdx.y * kUnpermRows;
+  const int nr = min(kUnpermRows, nrows - r0);
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < (size_t)n; v += stride) {
     const int x = nof[v];
-    const int dv = ds[x];
-    const u64 sv = ss[x];
-    if (dd) dd[v] = dv;
-    if (sd) sd[v] = dv != INT_MAX ? sv : 0ull;
+    int dv[kUnpermRows];
+    u64 sv[kUnpermRows];
+#pragma unroll
+    for (int r = 0; r < kUnpermRows; r++) {
+      if (r < nr) {
+        dv[r] = dsrc[r0 + r][x];
+        sv[r] = ssrc[r0 + r][x];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kUnpermRows; r++) {
+      if (r < nr) {
+        if (ddst) ddst[r0 + r][v] = dv[r];
+        sdst[r0 + r][v] = dv[r] != INT_MAX ? sv[r] : 0ull;
+      }
+    }
   }
 }
 

@@ -577,6 +577,86 @@ def test_persource_tree_frontier_spill(gu, orc, gpu, k, depth):
     _check_bc(gu, orc, hg, src, 64, "tree%d-%d" % (k, depth), engine=PS)
 
 
+def _undirected_csr(gu, n, edges):
+    """Undirected HostGraph from an (E, 2) edge array: both directions, rows
+    sorted, duplicates and self-loops dropped."""
+    e = np.asarray(edges, dtype=np.int64).reshape(-1, 2)
+    e = e[e[:, 0] != e[:, 1]]
+    both = np.unique(np.concatenate([e, e[:, ::-1]]), axis=0)
+    R = np.zeros(n + 1, dtype=np.int32)
+    np.add.at(R, both[:, 0] + 1, 1)
+    R = np.cumsum(R).astype(np.int32)
+    return gu.HostGraph(R, both[:, 1].astype(np.int32))
+
+
+def _facts(hg):
+    """(bipartite, connected) by a 2-colouring BFS over every component."""
+    R, C, n = hg.R, hg.C, hg.n
+    col = np.full(n, -1, dtype=np.int8)
+    bip, comps = True, 0
+    for s in range(n):
+        if col[s] >= 0:
+            continue
+        comps += 1
+        col[s] = 0
+        q = [s]
+        for v in q:
+            for w in C[R[v]:R[v + 1]]:
+                if col[w] < 0:
+                    col[w] = col[v] ^ 1
+                    q.append(int(w))
+                elif col[w] == col[v]:
+                    bip = False
+    return bip, comps == 1
+
+
+def _lds_kind_graph(gu, kind):
+    rng = np.random.default_rng(91)
+    if kind == "odd_cycle":
+        n = 2001
+        return _undirected_csr(gu, n, np.stack([np.arange(n), (np.arange(n) + 1) % n], 1))
+    if kind == "grid_components":
+        return gu.HostGraph.grid(64, 0.30, seed=92, lcc_only=False)
+    if kind == "random_deg4":
+        n = 20000
+        stubs = rng.permutation(np.repeat(np.arange(n), rng.integers(0, 5, n)))  # degrees 0-4
+        return _undirected_csr(gu, n, stubs[: len(stubs) // 2 * 2].reshape(-1, 2))
+    if kind == "grid_diagonals":
+        W = 60
+        r, c = np.divmod(np.arange(W * W), W)
+        v = np.arange(W * W)
+        e = [np.stack([v[c < W - 1], v[c < W - 1] + 1], 1), np.stack([v[r < W - 1], v[r < W - 1] + W], 1)]
+        # a diagonal across every 6th interior cell joins two vertices of one
+        # colour (odd cycles); the cell's top and bottom edges go, so no
+        # vertex exceeds degree 4
+        keep = set(map(tuple, np.concatenate(e).tolist()))
+        for a in range(1, W - 2, 3):
+            for b in range(1, W - 2, 2):
+                x = a * W + b
+                keep.discard((x, x + 1))
+                keep.discard((x + W, x + W + 1))
+                keep.add((x, x + W + 1))
+        return _undirected_csr(gu, W * W, np.array(sorted(keep)))
+    raise ValueError(kind)
+
+
+@pytest.mark.parametrize("kind,facts", [("odd_cycle", (False, True)), ("grid_components", (True, False)),
+                                        ("random_deg4", (False, False)), ("grid_diagonals", (False, True))])
+def test_persource_lds_graph_kinds(gu, orc, gpu, kind, facts):
+    """Undirected degree <= 4 graphs of each kind the level-set search
+    treats apart (bipartite or not: a same-level lookup or none; connected
+    or not: a distance-row fill or none), against the oracle: rows, path
+    counts and dependencies; batches of 45 and 7 sources leave partial
+    groups of rows for the write-back."""
+    hg = _lds_kind_graph(gu, kind)
+    assert hg.degrees().max() <= 4
+    assert _facts(hg) == facts
+    for k, seed in ((45, 93), (7, 94)):
+        _ps_compare(gu, orc, hg, hg.sample_sources(k, seed=seed), kind)
+    _ps_compare(gu, orc, hg, hg.sample_sources(9, seed=95), kind + "-d", sigma=False)
+    _check_bc(gu, orc, hg, hg.sample_sources(45, seed=96), 64, kind, engine=PS)
+
+
 def test_persource_distances_only(gu, orc, gpu):
     hg = gu.HostGraph.grid(32, 0.10, seed=9)
     _ps_compare(gu, orc, hg, hg.sample_sources(40, seed=10), "grid32-d", sigma=False)
