@@ -1657,10 +1657,13 @@ int run_persource(gu_graph *g, const int *sources, int nsrc, const HostRows &hr,
     }
     if (rl) {
       const dim3 ug((unsigned)std::max(1, std::min(pick_grid(g, 1), (g->n + 255) / 256)), (unsigned)nb);
-      if (sigma && lds)  // path counts written for reached vertices only; both rows in one pass
-        k_ps_unpermute_ds<<<dim3(ug.x, (unsigned)((nb + kUnpermRows - 1) / kUnpermRows)), 256, 0, st>>>(
+      if (sigma && lds) {  // path counts written for reached vertices only; both rows in one pass
+        // chunks of about 1024 vertices, a multiple of 8 per row
+        const unsigned wg = (unsigned)std::max<long long>(8, ((g->n + 1023) / 1024 + 7) / 8 * 8);
+        k_ps_unpermute_ds<<<dim3(wg, (unsigned)nb), 256, 0, st>>>(
             (int *const *)g->rowtab, (u64 *const *)(g->rowtab + 256),
-            hr.dist ? (int *const *)(g->rowtab + 1024) : nullptr, (u64 *const *)(g->rowtab + 1280), g->nof, g->n, nb);
+            hr.dist ? (int *const *)(g->rowtab + 1024) : nullptr, (u64 *const *)(g->rowtab + 1280), g->nof, g->n);
+      }
       if (hr.dist && !(sigma && lds))
         k_ps_unpermute<int><<<ug, 256, 0, st>>>((int *const *)g->rowtab, (int *const *)(g->rowtab + 1024),
                                                 g->nof, g->n);

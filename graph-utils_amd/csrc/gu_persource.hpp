@@ -940,34 +940,29 @@ __global__ __launch_bounds__(1024) void k_ps_bc_lds(PsBcArgs a) {
 
 // Both rows of an LDS level-set search back to the caller's order in one
 // pass: dd[b][v] = dist, sd[b][v] = path count where reached (distance
-// below INT_MAX), else 0; the distance destination may be null.  Each
-// thread serves kUnpermRows rows of one vertex (blockIdx.y selects the
-// group of rows, nrows of them in all), so nof is read once per group and
-// 2 kUnpermRows gathers are in flight.
-constexpr int kUnpermRows = 4;
+// below INT_MAX), else 0; the distance destination may be null.
+// blockIdx.y selects the row.  XCD-aware: blocks are dispatched to the 8
+// XCDs round-robin, so block b takes contiguous chunk (b % 8) * (G / 8) +
+// b / 8 of the row's G chunks (G = gridDim.x, a multiple of 8) and each
+// XCD gathers from one eighth of the vertex range; the clusters a chunk
+// gathers from are shared with the neighbouring chunks of the same XCD
+// (and L2) instead of being fetched by every XCD.
 __global__ __launch_bounds__(256) void k_ps_unpermute_ds(int *const *dsrc, u64 *const *ssrc, int *const *ddst,
-                                                         u64 *const *sdst, const int *nof, int n, int nrows) {
-  const int r0 = blockIdx.y * kUnpermRows;
-  const int nr = min(kUnpermRows, nrows - r0);
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < (size_t)n; v += stride) {
+                                                         u64 *const *sdst, const int *nof, int n) {
+  int *dd = ddst ? ddst[blockIdx.y] : nullptr;
+  u64 *sd = sdst[blockIdx.y];
+  const int *ds = dsrc[blockIdx.y];
+  const u64 *ss = ssrc[blockIdx.y];
+  const unsigned G = gridDim.x, b = blockIdx.x;
+  const unsigned c = (b % 8) * (G / 8) + b / 8;
+  const size_t chunk = ((size_t)n + G - 1) / G;
+  const size_t lo = (size_t)c * chunk, hi = min((size_t)n, lo + chunk);
+  for (size_t v = lo + threadIdx.x; v < hi; v += blockDim.x) {
     const int x = nof[v];
-    int dv[kUnpermRows];
-    u64 sv[kUnpermRows];
-#pragma unroll
-    for (int r = 0; r < kUnpermRows; r++) {
-      if (r < nr) {
-        dv[r] = dsrc[r0 + r][x];
-        sv[r] = ssrc[r0 + r][x];
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < kUnpermRows; r++) {
-      if (r < nr) {
-        if (ddst) ddst[r0 + r][v] = dv[r];
-        sdst[r0 + r][v] = dv[r] != INT_MAX ? sv[r] : 0ull;
-      }
-    }
+    const int dv = ds[x];
+    const u64 sv = ss[x];
+    if (dd) dd[v] = dv;
+    sd[v] = dv != INT_MAX ? sv : 0ull;
   }
 }
 
