@@ -1080,6 +1080,10 @@ struct gu_graph {
   // vertex (-1 padded), so a per-source visit is one load and a frontier
   // entry needs no row offsets.
   int4 *LE = nullptr;
+  // The same neighbours as four int16 deltas from the vertex's own id
+  // (kLe8None: none), when every delta fits: half the bytes per visit for
+  // the level-set search.
+  u64 *LE8 = nullptr;
   int lds_flags = 0;          // kLds* facts of an undirected LE graph
   std::vector<int32_t> nof_h;
   Units pu;                   // work units over in-edges (pulls)
@@ -1625,7 +1629,7 @@ int run_persource(gu_graph *g, const int *sources, int nsrc, const HostRows &hr,
     if (lds) {
       PsLdsArgs c{LE,      g->n,   p.srcs, rows ? (int *const *)g->rowtab : nullptr,
                   sigma ? (u64 *const *)(g->rowtab + 256) : nullptr,
-                  p.queues, p.depth, p.part, p.ovf, g->lds_flags};
+                  p.queues, p.depth, p.part, p.ovf, g->lds_flags, g->LE8};
       if (sigma)
         k_ps_lds<true, true><<<nb, 1024, 0, st>>>(c);
       else if (rows)
@@ -1736,7 +1740,7 @@ int run_persource_bc(gu_graph *g, const int *sources, int nsrc, const HostRows &
     const int4 *LE = rl ? g->LE : nullptr;
     PsBcArgs a{rl ? g->LR : g->R, rl ? g->LC : g->C, LE, g->n, p.srcs, (int *const *)g->rowtab,
                (u64 *const *)(g->rowtab + 256), (float *const *)(g->rowtab + 512),
-               (float *const *)(g->rowtab + 1280), p.queues, p.depth, p.part, p.ovf, g->lds_flags};
+               (float *const *)(g->rowtab + 1280), p.queues, p.depth, p.part, p.ovf, g->lds_flags, g->LE8};
     // undirected degree <= 4 graphs: the LDS level-set forward; a launch in
     // which a level set overflowed runs again on k_ps_bc
     bool lds = LE && !g->directed;
@@ -2143,6 +2147,23 @@ int build_cluster_copy(gu_graph *g, const int32_t *R, const int32_t *C) {
     if (hipMalloc((void **)&g->LE, sizeof(int4) * (size_t)n) != hipSuccess) return GU_ERR_NOMEM;
     if (hipMemcpy(g->LE, LE.data(), sizeof(int4) * (size_t)n, hipMemcpyHostToDevice) != hipSuccess)
       return GU_ERR_HIP;
+    std::vector<u64> LE8((size_t)n);
+    bool fits = true;
+    for (int u = 0; u < n && fits; u++) {
+      const int q[4] = {LE[(size_t)u].x, LE[(size_t)u].y, LE[(size_t)u].z, LE[(size_t)u].w};
+      u64 p = 0;
+      for (int j = 0; j < 4; j++) {
+        const long long dlt = q[j] < 0 ? (long long)gu::kLe8None : (long long)q[j] - u;
+        fits &= q[j] < 0 || (dlt > gu::kLe8None && dlt <= 32767);
+        p |= (u64)(uint16_t)(int16_t)dlt << (16 * j);
+      }
+      LE8[(size_t)u] = p;
+    }
+    if (fits) {
+      if (hipMalloc((void **)&g->LE8, sizeof(u64) * (size_t)n) != hipSuccess) return GU_ERR_NOMEM;
+      if (hipMemcpy(g->LE8, LE8.data(), sizeof(u64) * (size_t)n, hipMemcpyHostToDevice) != hipSuccess)
+        return GU_ERR_HIP;
+    }
   }
   if (hipMemcpy(g->LR, LR.data(), sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(g->LC, LC.data(), sizeof(int) * LC.size(), hipMemcpyHostToDevice) != hipSuccess ||
@@ -2284,6 +2305,7 @@ int gu_graph_destroy(gu_graph_t g) {
     if (g->CT) (void)hipFree(g->CT);
     if (g->LR) (void)hipFree(g->LR);
     if (g->LE) (void)hipFree(g->LE);
+    if (g->LE8) (void)hipFree(g->LE8);
     if (g->LC) (void)hipFree(g->LC);
     if (g->nof) (void)hipFree(g->nof);
   } else {
@@ -2320,6 +2342,7 @@ int gu_graph_share(gu_graph_t g, gu_graph_t *out) {
   v->CT = root->CT;
   v->LR = root->LR;
   v->LE = root->LE;
+  v->LE8 = root->LE8;
   v->lds_flags = root->lds_flags;
   v->LC = root->LC;
   v->nof = root->nof;
@@ -2360,6 +2383,8 @@ int gu_graph_device_bytes(gu_graph_t g, int64_t *bytes) {
   if (!g || !bytes) return GU_ERR_INVALID;
   *bytes = (int64_t)(((size_t)g->n + 1) * 4 + (size_t)g->m * 8 + g->ws.bytes + g->pws.bytes);
   if (g->LR) *bytes += (int64_t)(((size_t)g->n + 1) * 4 + (size_t)g->m * 4 + (size_t)g->n * 4);
+  if (g->LE) *bytes += (int64_t)((size_t)g->n * 16);
+  if (g->LE8) *bytes += (int64_t)((size_t)g->n * 8);
   return GU_OK;
 }
 

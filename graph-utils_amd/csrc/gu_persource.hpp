@@ -477,6 +477,7 @@ struct PsBcArgs {
   u64 *part;
   int *ovf;
   int lds_flags;        // kLds* graph facts for the level-set kernels
+  const u64 *LE8;       // or null: the neighbour table as int16 deltas
 };
 
 __device__ __forceinline__ void ps_neumaier(float &sum, float &comp, float x) {
@@ -647,6 +648,20 @@ constexpr int kLdsFront = 3072;                   // LDS frontier entries per bu
 // Bipartite: BFS levels are independent sets, so a neighbour of a level-L
 // vertex is never at level L and the same-level lookup is skipped.
 // Connected: every vertex is reached, so the distance row needs no fill.
+// Compact neighbour table: four int16 deltas w - v in one u64 (lowest 16
+// bits first), kLe8None for an empty slot.
+constexpr int kLe8None = -32768;
+__device__ __forceinline__ int le8_one(u64 p, int j, int v) {
+  const int dl = (int)(short)(unsigned short)(p >> (16 * j));
+  return dl == kLe8None ? -1 : v + dl;
+}
+__device__ __forceinline__ int4 le8_decode(u64 p, int v) {
+  return make_int4(le8_one(p, 0, v), le8_one(p, 1, v), le8_one(p, 2, v), le8_one(p, 3, v));
+}
+// The neighbours of x (x >= 0) from whichever table the graph has.
+__device__ __forceinline__ int4 le_load(const int4 *LE, const u64 *LE8, int x) {
+  return LE8 ? le8_decode(LE8[x], x) : LE[x];
+}
 constexpr int kLdsBipartite = 1;
 constexpr int kLdsConnected = 2;
 struct PsLds {
@@ -777,8 +792,9 @@ __device__ __forceinline__ void ps_visit_lds(const int4 q, int *d, u64 *sg, int 
 // the two `spill` halves.  Returns the deepest level, or -1 when a level set
 // overflowed.
 template <bool SIGMA, bool BC, bool WANT_D>
-__device__ int ps_levels_lds(const int4 *LE, int n, int s, int *d, u64 *sg, int *spill0, int *S, int *E, PsLds &f,
-                             unsigned *s_cnt, int *s_bad, int &ovf, u64 &teps, u64 &reached, u64 &dag, bool bip) {
+__device__ int ps_levels_lds(const int4 *LE, const u64 *LE8, int n, int s, int *d, u64 *sg, int *spill0, int *S,
+                             int *E, PsLds &f, unsigned *s_cnt, int *s_bad, int &ovf, u64 &teps, u64 &reached,
+                             u64 &dag, bool bip) {
   const int tid = threadIdx.x, T = blockDim.x;
   for (int i = tid; i < 4 * kLdsBuckets; i += T) (&f.tab[0][0])[i] = make_int4(-1, -1, -1, -1);
   if (tid < 3) s_cnt[tid] = 0;
@@ -814,10 +830,10 @@ __device__ int ps_levels_lds(const int4 *LE, int n, int s, int *d, u64 *sg, int 
       return k >= qn ? -1 : (k < (unsigned)kLdsFront ? f.fv[b][k] : sp_cur[k - kLdsFront]);
     };
     int v = entry((unsigned)tid);
-    int4 q = v >= 0 ? LE[v] : make_int4(-1, -1, -1, -1);
+    int4 q = v >= 0 ? le_load(LE, LE8, v) : make_int4(-1, -1, -1, -1);
     for (unsigned base = 0; base < qn; base += T) {
       const int vn = entry(base + T + tid);
-      const int4 qn4 = vn >= 0 ? LE[vn] : make_int4(-1, -1, -1, -1);
+      const int4 qn4 = vn >= 0 ? le_load(LE, LE8, vn) : make_int4(-1, -1, -1, -1);
       ps_visit_lds<SIGMA, WANT_D || BC>(q, d, sg, v, L, tp, tl, tn, ovf, bad, dag, teps, cnt, f, sp_next, Snext,
                                         b ^ 1, bip);
       v = vn;
@@ -850,6 +866,7 @@ struct PsLdsArgs {
   u64 *part;          // [nsrc][4], as PsArgs; part[3] bit 1: level set overflow
   int *ovf;
   int flags;          // kLds* graph facts
+  const u64 *LE8;     // or null: the neighbour table as int16 deltas
 };
 
 // The search: with rows (WANT_D) the distance row is first set to INT_MAX,
@@ -874,7 +891,7 @@ __global__ __launch_bounds__(1024) void k_ps_lds(PsLdsArgs a) {
   __syncthreads();
   int ovf = 0;
   u64 teps = 0, reached = 1, dag = 0;
-  const int L = ps_levels_lds<SIGMA, false, WANT_D>(a.LE, a.n, s, d, sg, spill, nullptr, nullptr, f, s_cnt, &s_bad,
+  const int L = ps_levels_lds<SIGMA, false, WANT_D>(a.LE, a.LE8, a.n, s, d, sg, spill, nullptr, nullptr, f, s_cnt, &s_bad,
                                                     ovf, teps, reached, dag, (a.flags & kLdsBipartite) != 0);
   ps_finish(teps, dag, reached, L < 0 ? 0 : L, ovf, &s_ovf, s_red, a.part, a.depth, a.ovf);
   if (tid == 0 && L < 0) a.part[4 * blockIdx.x + 3] |= 2ull;
@@ -903,7 +920,7 @@ __global__ __launch_bounds__(1024) void k_ps_bc_lds(PsBcArgs a) {
   __syncthreads();
   int ovf = 0;
   u64 teps = 0, reached = 1, dag = 0;
-  const int L = ps_levels_lds<true, true, true>(a.LE, a.n, s, d, sg, nullptr, S, E, f, s_cnt, &s_bad, ovf, teps,
+  const int L = ps_levels_lds<true, true, true>(a.LE, a.LE8, a.n, s, d, sg, nullptr, S, E, f, s_cnt, &s_bad, ovf, teps,
                                                 reached, dag, (a.lds_flags & kLdsBipartite) != 0);
   __syncthreads();
   if (L < 0) {  // rerun by the host on k_ps_bc
@@ -920,7 +937,7 @@ __global__ __launch_bounds__(1024) void k_ps_bc_lds(PsBcArgs a) {
     for (int i = E[l] + tid; i < E[l + 1]; i += T) {
       const int v = S[i];
       const float sv = (float)ld_wg(&sg[v]);
-      const int4 q4 = a.LE[v];
+      const int4 q4 = le_load(a.LE, a.LE8, v);
       const int w[4] = {q4.x, q4.y, q4.z, q4.w};
       bool on[4];
       float qw[4];

@@ -617,8 +617,8 @@ def _lds_kind_graph(gu, kind):
         return _undirected_csr(gu, n, np.stack([np.arange(n), (np.arange(n) + 1) % n], 1))
     if kind == "grid_components":
         return gu.HostGraph.grid(64, 0.30, seed=92, lcc_only=False)
-    if kind == "random_deg4":
-        n = 20000
+    if kind in ("random_deg4", "random_deg4_wide"):
+        n = 20000 if kind == "random_deg4" else 120000  # the wide one: neighbour deltas beyond int16
         stubs = rng.permutation(np.repeat(np.arange(n), rng.integers(0, 5, n)))  # degrees 0-4
         return _undirected_csr(gu, n, stubs[: len(stubs) // 2 * 2].reshape(-1, 2))
     if kind == "grid_diagonals":
@@ -641,13 +641,14 @@ def _lds_kind_graph(gu, kind):
 
 
 @pytest.mark.parametrize("kind,facts", [("odd_cycle", (False, True)), ("grid_components", (True, False)),
-                                        ("random_deg4", (False, False)), ("grid_diagonals", (False, True))])
+                                        ("random_deg4", (False, False)), ("random_deg4_wide", (False, False)),
+                                        ("grid_diagonals", (False, True))])
 def test_persource_lds_graph_kinds(gu, orc, gpu, kind, facts):
     """Undirected degree <= 4 graphs of each kind the level-set search
     treats apart (bipartite or not: a same-level lookup or none; connected
-    or not: a distance-row fill or none), against the oracle: rows, path
-    counts and dependencies; batches of 45 and 7 sources leave partial
-    groups of rows for the write-back."""
+    or not: a distance-row fill or none; neighbour ids within int16 of the
+    vertex's own or not: the compact neighbour table or the int4 one),
+    against the oracle: rows, path counts and dependencies."""
     hg = _lds_kind_graph(gu, kind)
     assert hg.degrees().max() <= 4
     assert _facts(hg) == facts
