@@ -1630,12 +1630,7 @@ int run_persource(gu_graph *g, const int *sources, int nsrc, const HostRows &hr,
       PsLdsArgs c{LE,      g->n,   p.srcs, rows ? (int *const *)g->rowtab : nullptr,
                   sigma ? (u64 *const *)(g->rowtab + 256) : nullptr,
                   p.queues, p.depth, p.part, p.ovf, g->lds_flags, g->LE8};
-      if (sigma)
-        k_ps_lds<true, true><<<nb, 1024, 0, st>>>(c);
-      else if (rows)
-        k_ps_lds<false, true><<<nb, 1024, 0, st>>>(c);
-      else
-        k_ps_lds<false, false><<<nb, 1024, 0, st>>>(c);
+      k_ps_lds<true, true><<<nb, 1024, 0, st>>>(c);
       GU_HIP(hipGetLastError());
       if ((rc = fetch())) return rc;
       for (int i = 0; i < nb; i++) lds &= (part[4 * i + 3] & 2) == 0;

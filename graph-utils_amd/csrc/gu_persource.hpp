@@ -648,6 +648,10 @@ constexpr int kLdsFront = 3072;                   // LDS frontier entries per bu
 // Bipartite: BFS levels are independent sets, so a neighbour of a level-L
 // vertex is never at level L and the same-level lookup is skipped.
 // Connected: every vertex is reached, so the distance row needs no fill.
+// 16-bit distances of the level-set search's scratch rows.
+constexpr unsigned short kD16None = 0xffff;
+constexpr int kD16Max = 0xfffe;
+
 // Compact neighbour table: four int16 deltas w - v in one u64 (lowest 16
 // bits first), kLe8None for an empty slot.
 constexpr int kLe8None = -32768;
@@ -730,8 +734,8 @@ __device__ __forceinline__ int lds_insert(int4 *t, int w) {
 // neighbours q (loaded by the caller).  (Issuing the parents' path-count
 // loads ahead of the claims, or expanding two vertices per thread, measured
 // slower: 61.4 and 80.4 against 56.4 ms per 2048^2 pass.)
-template <bool SIGMA, bool WANT_D>
-__device__ __forceinline__ void ps_visit_lds(const int4 q, int *d, u64 *sg, int v, int L, const int4 *tp,
+template <bool SIGMA, bool WANT_D, class DT>
+__device__ __forceinline__ void ps_visit_lds(const int4 q, DT *d, u64 *sg, int v, int L, const int4 *tp,
                                              const int4 *tl, int4 *tn, int &ovf, int &bad, u64 &dag, u64 &teps,
                                              unsigned *cnt, PsLds &f, int *spill, int *S, int nbuf, bool bip) {
   const int w[4] = {q.x, q.y, q.z, q.w};
@@ -783,16 +787,17 @@ __device__ __forceinline__ void ps_visit_lds(const int4 q, int *d, u64 *sg, int 
     }
     if (L >= 1 && v >= 0) sg[v] = acc;
   }
-  if (WANT_D && v >= 0) d[v] = L;
+  if (WANT_D && v >= 0) d[v] = (DT)L;
 }
 
 // The level loop over LDS level sets (one workgroup barrier a level).  With
 // BC the levels are kept in S (E[l] = start of level l) and a frontier's
 // entries beyond the LDS capacity are read back from S; otherwise they go to
 // the two `spill` halves.  Returns the deepest level, or -1 when a level set
-// overflowed.
-template <bool SIGMA, bool BC, bool WANT_D>
-__device__ int ps_levels_lds(const int4 *LE, const u64 *LE8, int n, int s, int *d, u64 *sg, int *spill0, int *S,
+// overflowed (or, with 16-bit distances, the search went deeper than
+// kD16Max levels).
+template <bool SIGMA, bool BC, bool WANT_D, class DT>
+__device__ int ps_levels_lds(const int4 *LE, const u64 *LE8, int n, int s, DT *d, u64 *sg, int *spill0, int *S,
                              int *E, PsLds &f, unsigned *s_cnt, int *s_bad, int &ovf, u64 &teps, u64 &reached,
                              u64 &dag, bool bip) {
   const int tid = threadIdx.x, T = blockDim.x;
@@ -850,6 +855,7 @@ __device__ int ps_levels_lds(const int4 *LE, const u64 *LE8, int n, int s, int *
     qs += q2n;
     b ^= 1;
     L++;
+    if (sizeof(DT) == 2 && L > kD16Max) return -1;
     if (BC && tid == 0) E[L + 1] = (int)qs;
   }
   return L;
@@ -859,7 +865,7 @@ struct PsLdsArgs {
   const int4 *LE;
   int n;
   const int *srcs;    // [nsrc] (cluster-order ids)
-  int *const *dist;   // [nsrc] rows, or null (no rows wanted: diameter)
+  int *const *dist;   // [nsrc] scratch rows (16-bit distances), or null (diameter)
   u64 *const *sigma;  // [nsrc] rows, or null
   int *queues;        // [nsrc][2n+2] frontier spill
   int *depth;
@@ -869,11 +875,13 @@ struct PsLdsArgs {
   const u64 *LE8;     // or null: the neighbour table as int16 deltas
 };
 
-// The search: with rows (WANT_D) the distance row is first set to INT_MAX,
-// the only row initialised: it tells reached vertices apart when the rows
-// are written back (k_ps_unpermute_ds); path counts are written for
-// reached vertices only.  On a connected graph every vertex's distance is
-// written by the search, so the fill is skipped.
+// The search: with rows (WANT_D) the distance row, in 16 bits (kD16None:
+// not reached), is first set to kD16None, the only row initialised: it
+// tells reached vertices apart when the rows are written back
+// (k_ps_unpermute_ds); path counts are written for reached vertices only.
+// On a connected graph every vertex's distance is written by the search, so
+// the fill is skipped.  A source deeper than kD16Max levels stops as a
+// level-set overflow does (the host reruns the launch).
 template <bool SIGMA, bool WANT_D>
 __global__ __launch_bounds__(1024) void k_ps_lds(PsLdsArgs a) {
   __shared__ PsLds f;
@@ -882,11 +890,11 @@ __global__ __launch_bounds__(1024) void k_ps_lds(PsLdsArgs a) {
   __shared__ int s_ovf, s_bad;
   const int tid = threadIdx.x, T = blockDim.x;
   const int s = a.srcs[blockIdx.x];
-  int *d = WANT_D ? a.dist[blockIdx.x] : nullptr;
+  unsigned short *d = WANT_D ? reinterpret_cast<unsigned short *>(a.dist[blockIdx.x]) : nullptr;
   u64 *sg = SIGMA ? a.sigma[blockIdx.x] : nullptr;
   int *spill = a.queues + (size_t)blockIdx.x * (2 * (size_t)a.n + 2);
   if (WANT_D && !(a.flags & kLdsConnected))
-    for (int i = tid; i < a.n; i += T) d[i] = INT_MAX;
+    for (int i = tid; i < a.n; i += T) d[i] = kD16None;
   if (tid == 0) s_ovf = 0;
   __syncthreads();
   int ovf = 0;
@@ -956,8 +964,9 @@ __global__ __launch_bounds__(1024) void k_ps_bc_lds(PsBcArgs a) {
 }
 
 // Both rows of an LDS level-set search back to the caller's order in one
-// pass: dd[b][v] = dist, sd[b][v] = path count where reached (distance
-// below INT_MAX), else 0; the distance destination may be null.
+// pass: dd[b][v] = dist (INT_MAX where the 16-bit row says not reached),
+// sd[b][v] = path count where reached, else 0; the distance destination
+// may be null.
 // blockIdx.y selects the row.  XCD-aware: blocks are dispatched to the 8
 // XCDs round-robin, so block b takes contiguous chunk (b % 8) * (G / 8) +
 // b / 8 of the row's G chunks (G = gridDim.x, a multiple of 8) and each
@@ -968,7 +977,7 @@ __global__ __launch_bounds__(256) void k_ps_unpermute_ds(int *const *dsrc, u64 *
                                                          u64 *const *sdst, const int *nof, int n) {
   int *dd = ddst ? ddst[blockIdx.y] : nullptr;
   u64 *sd = sdst[blockIdx.y];
-  const int *ds = dsrc[blockIdx.y];
+  const unsigned short *ds = reinterpret_cast<const unsigned short *>(dsrc[blockIdx.y]);
   const u64 *ss = ssrc[blockIdx.y];
   const unsigned G = gridDim.x, b = blockIdx.x;
   const unsigned c = (b % 8) * (G / 8) + b / 8;
@@ -976,10 +985,10 @@ __global__ __launch_bounds__(256) void k_ps_unpermute_ds(int *const *dsrc, u64 *
   const size_t lo = (size_t)c * chunk, hi = min((size_t)n, lo + chunk);
   for (size_t v = lo + threadIdx.x; v < hi; v += blockDim.x) {
     const int x = nof[v];
-    const int dv = ds[x];
+    const unsigned short dv = ds[x];
     const u64 sv = ss[x];
-    if (dd) dd[v] = dv;
-    sd[v] = dv != INT_MAX ? sv : 0ull;
+    if (dd) dd[v] = dv != kD16None ? (int)dv : INT_MAX;
+    sd[v] = dv != kD16None ? sv : 0ull;
   }
 }
 

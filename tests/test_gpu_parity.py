@@ -658,6 +658,16 @@ def test_persource_lds_graph_kinds(gu, orc, gpu, kind, facts):
     _check_bc(gu, orc, hg, hg.sample_sources(45, seed=96), 64, kind, engine=PS)
 
 
+def test_persource_path_deeper_than_16bit_levels(gu, orc, gpu):
+    """A path of 70000 vertices searched from an end: 69999 levels, past
+    what the level-set search's 16-bit distance rows hold (65534), so the
+    launch is rerun on the distance kernel; rows and counts still match."""
+    n = 70000
+    hg = _undirected_csr(gu, n, np.stack([np.arange(n - 1), np.arange(1, n)], 1))
+    st = _ps_compare(gu, orc, hg, np.array([0, n - 1, n // 2, 3], dtype=np.int32), "path70000")
+    assert st.max_level == n - 1
+
+
 def test_persource_distances_only(gu, orc, gpu):
     hg = gu.HostGraph.grid(32, 0.10, seed=9)
     _ps_compare(gu, orc, hg, hg.sample_sources(40, seed=10), "grid32-d", sigma=False)
