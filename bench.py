@@ -94,6 +94,35 @@ def copy_gbs(gu, nbytes=1 << 30, reps=5):
         b.close()
 
 
+def power_leg(gu, step, seconds=2.0, period_ms=10):
+    """Board power while the line's own passes repeat for about `seconds`
+    (power_measurer of util_device.cu:79-130, through AMD SMI): the mean of
+    the socket-power samples, the energy counter's average over the same
+    interval, and GTEPS per watt.  None where the host cannot read the
+    device's power."""
+    pm = gu.PowerMeasurer(period_ms)
+    if not pm.start():
+        return {"available": False, "reason": "AMD SMI reports no power for this device (status %d)"
+                % pm.status}
+    units, i, t0 = 0, 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        units += step(i)
+        i += 1
+        if i % 16 == 0:
+            gu.synchronize()
+    avg = pm.end()
+    wall = time.perf_counter() - t0
+    r = pm.result
+    gteps = units / wall / 1e9
+    watts = r.energy_avg_w if r.energy_avg_w > 0 else avg
+    return {"available": True, "avg_w": round(avg, 1), "max_w": round(r.max_w, 1),
+            "energy_avg_w": round(r.energy_avg_w, 1) if r.energy_avg_w > 0 else None,
+            "samples": r.samples, "seconds": round(r.seconds, 2), "passes": i,
+            "gteps": round(gteps, 2), "gteps_per_w": round(gteps / watts, 3) if watts > 0 else None,
+            "sampler": "AMD SMI socket power every %d ms on a host thread; energy counter read at both "
+                       "ends" % period_ms}
+
+
 def cpu_threads():
     try:
         c = len(os.sched_getaffinity(0))
@@ -684,6 +713,9 @@ def main():
             e2e_units += st.teps_units
         e2e_gteps = e2e_units / (time.perf_counter() - t2) / 1e9
 
+        power = power_leg(gu, lambda i: dg.multi_search_device(batches[args.warmup + i % args.steps],
+                                                               dist, sig, cfg).teps_units)
+
         variants, variants_td, td_levels = {}, {}, {}
         if not args.no_variants:
             for name in ["shuffle", "warp", "cta_warp", "scan", "edge_parallel", "linear"]:
@@ -743,6 +775,7 @@ def main():
         "variants_gteps": variants,
         "variants_topdown_gteps": variants_td,
         "pcie_inclusive_gteps": round(e2e_gteps, 2),
+        "power": power,
     }
     dist.close()
     sig.close()

@@ -62,13 +62,13 @@ EXPORTS = (
     "gu_server_submit", "gu_server_wait", "gu_server_next_done", "gu_server_poll",
     "gu_server_release", "gu_server_now_ms", "gu_server_get_stats", "gu_server_destroy",
     "gu_autotune", "gu_parse_market", "gu_parse_snap", "gu_write_edgelist", "gu_reverse",
-    "gu_lbs_noalloc",
+    "gu_lbs_noalloc", "gu_power_start", "gu_power_stop", "gu_trace_push", "gu_trace_pop",
 )
 
 QUERY_BFS, QUERY_BC, QUERY_DIAMETER = 0, 1, 2
 QF_SIGMA, QF_DELTA, QF_HOST = 1, 2, 4
 QR_SIGMA_WRAPPED = 1  # gu_query_result.flags: a path count wrapped mod 2^64
-ERR_CAPACITY, ERR_TIMEOUT = -9, -10
+ERR_CAPACITY, ERR_TIMEOUT, ERR_UNSUPPORTED = -9, -10, -11
 
 
 class GuError(RuntimeError):
@@ -200,6 +200,10 @@ def lib():
         "gu_choose_device": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(DeviceProps)]),
         "gu_timer_start": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
         "gu_timer_stop": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double)]),
+        "gu_power_start": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(_P)]),
+        "gu_power_stop": (ctypes.c_int, [_P, ctypes.POINTER(PowerResult)]),
+        "gu_trace_push": (ctypes.c_int, [ctypes.c_char_p]),
+        "gu_trace_pop": (ctypes.c_int, []),
         "gu_malloc": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_size_t]),
         "gu_free": (ctypes.c_int, [_P]),
         "gu_memcpy_h2d": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
@@ -374,6 +378,68 @@ class DeviceBuffer:
             self.close()
         except Exception:
             pass
+
+
+class PowerResult(ctypes.Structure):
+    _fields_ = [("avg_w", ctypes.c_double), ("max_w", ctypes.c_double),
+                ("energy_avg_w", ctypes.c_double), ("energy_j", ctypes.c_double),
+                ("seconds", ctypes.c_double), ("samples", ctypes.c_int32)]
+
+
+class PowerMeasurer:
+    """power_measurer (util_device.cuh:47-62, util_device.cu:79-130): the
+    device's socket power sampled through AMD SMI every `period_ms` between
+    start() and end().  end() returns the mean of the samples in W, or -1 when
+    the host cannot read the device's power (the reference's answer on
+    boards NVML cannot read); `result` then holds the energy-counter average
+    and the sample count, or None."""
+
+    def __init__(self, period_ms, device=0):
+        self.period_ms = int(period_ms)
+        self.device = int(device)
+        self._h = None
+        self.result = None
+        self.status = None
+
+    def start(self):
+        h = ctypes.c_void_p()
+        self.status = lib().gu_power_start(self.device, self.period_ms, ctypes.byref(h))
+        if self.status == ERR_UNSUPPORTED:
+            self._h = None
+            return False
+        _check(self.status, "gu_power_start")
+        self._h = h
+        return True
+
+    def end(self):
+        if self._h is None:
+            self.result = None
+            return -1.0
+        r = PowerResult()
+        h, self._h = self._h, None
+        _check(lib().gu_power_stop(h, ctypes.byref(r)), "gu_power_stop")
+        self.result = r
+        return r.avg_w
+
+    start_power_sample = start
+    end_power_sample = end
+
+
+class TraceRange:
+    """A roctx range (gu_trace_push / gu_trace_pop) around a with-block."""
+
+    def __init__(self, message):
+        self.message = message.encode()
+        self.on = False
+
+    def __enter__(self):
+        self.on = lib().gu_trace_push(self.message) == 0
+        return self
+
+    def __exit__(self, *exc):
+        if self.on:
+            lib().gu_trace_pop()
+        return False
 
 
 class Timer:

@@ -1414,6 +1414,7 @@ template <int NW>
 int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const gu_config &cfg,
               int variant, const BatchOut &out, RunInfo *info, hipStream_t st) {
   const bool stack = job == JOB_BC;
+  GuTraceRange tr_pass(stack ? "gu:batch brandes" : "gu:batch search");
   int rc = ensure_ws(g, NW, stack, 0, st);
   if (rc) return rc;
   Workspace &w = g->ws;
@@ -1460,6 +1461,7 @@ int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const g
       e = make_eng<NW>(g, nb, stack, sigma);
       set_grids(e);
     }
+    GuTraceRange tr_chunk("gu:level chunk");
     if (cfg.level_graphs > 0) {
       hipGraphExec_t ex = nullptr;
       rc = level_graph<NW>(g, e, variant, block, grid, K, sigma, &ex);
@@ -1488,6 +1490,7 @@ int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const g
   const int last = g->ctrl_h->last_level;
   if (job == JOB_BFS && (out.dist || out.sigma))
   {
+    GuTraceRange tr("gu:rows out");
     const long long tiles = ((long long)g->n + 63) / 64 * ((nb + kWL - 1) / kWL);
     const int og = (int)std::max(1LL, std::min<long long>(tiles, pick_grid(g, kTransposeWgPerCu)));
     if (out.dist) k_transpose_out<NW, int><<<og, 256, 0, st>>>(e, w.dvm, out.dist, INT_MAX);
@@ -1508,6 +1511,7 @@ int run_batch(gu_graph *g, const int *srcs, int nb, Job job, bool sigma, const g
     eb.nhubs = g->ou.nhubs;
     Bw b{w.dl, w.ql, w.btag, w.blm, w.bwork, w.part, g->ou.seg0};
     const int cgrid = std::max(1, std::min(pick_grid(g, 8), (g->ou.nhubs + 3) / 4));
+    GuTraceRange tr("gu:brandes backward");
     for (int L = last; L >= 1; L--) {
       // 16 workgroups per compute unit: s22 BC 305 GTEPS against 292 with 8
       k_backward<NW><<<pick_grid(g, 16), 256, 0, st>>>(eb, b, L);
@@ -1630,6 +1634,7 @@ int run_persource(gu_graph *g, const int *sources, int nsrc, const HostRows &hr,
       PsLdsArgs c{LE,      g->n,   p.srcs, rows ? (int *const *)g->rowtab : nullptr,
                   sigma ? (u64 *const *)(g->rowtab + 256) : nullptr,
                   p.queues, p.depth, p.part, p.ovf, g->lds_flags, g->LE8};
+      GuTraceRange tr("gu:per-source level sets");
       k_ps_lds<true, true><<<nb, 1024, 0, st>>>(c);
       GU_HIP(hipGetLastError());
       if ((rc = fetch())) return rc;
@@ -1637,6 +1642,7 @@ int run_persource(gu_graph *g, const int *sources, int nsrc, const HostRows &hr,
       if (!lds) k_fill_i32<<<1, 64, 0, st>>>(p.ovf, 1, 0);
     }
     if (!lds) {
+      GuTraceRange tr("gu:per-source search");
       // undirected graphs pull path counts from the previous level
       if (LE) {
         if (sigma && !g->directed)
@@ -1655,6 +1661,7 @@ int run_persource(gu_graph *g, const int *sources, int nsrc, const HostRows &hr,
       GU_HIP(hipGetLastError());
     }
     if (rl) {
+      GuTraceRange tr("gu:per-source write-back");
       const dim3 ug((unsigned)std::max(1, std::min(pick_grid(g, 1), (g->n + 255) / 256)), (unsigned)nb);
       if (sigma && lds) {  // path counts written for reached vertices only; both rows in one pass
         // chunks of about 1024 vertices, a multiple of 8 per row
@@ -1740,6 +1747,7 @@ int run_persource_bc(gu_graph *g, const int *sources, int nsrc, const HostRows &
     // which a level set overflowed runs again on k_ps_bc
     bool lds = LE && !g->directed;
     if (lds) {
+      GuTraceRange tr("gu:per-source brandes level sets");
       k_ps_bc_lds<<<nb, 1024, 0, st>>>(a);
       GU_HIP(hipGetLastError());
       GU_HIP(hipMemcpyAsync(part, p.part, sizeof(u64) * 4 * nb, hipMemcpyDeviceToHost, st));
@@ -1748,6 +1756,7 @@ int run_persource_bc(gu_graph *g, const int *sources, int nsrc, const HostRows &
       if (!lds) k_fill_i32<<<1, 64, 0, st>>>(p.ovf, 1, 0);
     }
     if (!lds) {
+      GuTraceRange tr("gu:per-source brandes");
       if (LE && g->directed)
         k_ps_bc<false, true><<<nb, 1024, 0, st>>>(a);
       else if (LE)
@@ -1763,6 +1772,7 @@ int run_persource_bc(gu_graph *g, const int *sources, int nsrc, const HostRows &
       k_ps_unpermute<float><<<ug, 256, 0, st>>>((float *const *)(g->rowtab + 512),
                                                 (float *const *)(g->rowtab + 1024), g->nof, g->n);
     }
+    GuTraceRange tr_red("gu:brandes reduce");
     if (hr.bc)
       k_ps_bc_reduce<<<pick_grid(g, 8), 256, 0, st>>>((float *const *)(g->rowtab + 512),
                                                        (double *const *)(g->rowtab + 768), nb, g->n,
@@ -1929,6 +1939,7 @@ const char *gu_strerror(int s) {
     case GU_ERR_NODEVICE: return "no HIP device";
     case GU_ERR_CAPACITY: return "output capacity exceeded";
     case GU_ERR_TIMEOUT: return "timed out";
+    case GU_ERR_UNSUPPORTED: return "not supported on this host";
     default: return "unknown status";
   }
 }
@@ -2700,6 +2711,7 @@ int gu_lbs(const int32_t *counts, int64_t S, int32_t *scan, int32_t *result, int
     if (W_out) *W_out = 0;
     return GU_OK;
   }
+  GuTraceRange tr("gu:lbs");
   int dev = 0, cus = 0;
   GU_HIP(hipGetDevice(&dev));
   GU_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));

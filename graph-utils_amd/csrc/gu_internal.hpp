@@ -14,3 +14,16 @@ int gu_search_rows_wrap(gu_graph_t g, const int32_t *sources, int32_t nsrc, int3
                         gu_stats *stats);
 int gu_bc_rows_wrap(gu_graph_t g, const int32_t *sources, int32_t nsrc, float *const *delta_rows,
                     double *const *bc_targets, int32_t *wrap, const gu_config *cfg, gu_stats *stats);
+
+// A roctx range over a scope (gu_trace_push / gu_trace_pop): the engine's
+// phases (level chunks, write-backs, Brandes backward, LBS) show up as
+// ranges in `rocprofv3 --marker-trace`, with the kernels they launch.
+struct GuTraceRange {
+  explicit GuTraceRange(const char *m) : on(gu_trace_push(m) == GU_OK) {}
+  ~GuTraceRange() {
+    if (on) gu_trace_pop();
+  }
+  GuTraceRange(const GuTraceRange &) = delete;
+  GuTraceRange &operator=(const GuTraceRange &) = delete;
+  bool on;
+};

@@ -243,6 +243,32 @@ void choose_device(program_options &op) {
   }
 }
 
+power_measurer::~power_measurer() {
+  if (handle) gu_power_stop(handle, nullptr);
+}
+
+void power_measurer::start_power_sample() {
+  if (handle) return;
+  const int32_t ms = period < 1 ? 1 : (period > 60000 ? 60000 : (int32_t)period);
+  const int rc = gu_power_start(device, ms, &handle);
+  if (rc == GU_ERR_UNSUPPORTED) {
+    handle = nullptr;
+    std::cerr << "Warning: the power of this device cannot be read (AMD SMI)." << std::endl;
+    return;
+  }
+  check(rc, "start_power_sample");
+}
+
+double power_measurer::end_power_sample() {
+  if (!handle) return -1;
+  gu_power_result r;
+  gu_power_t h = handle;
+  handle = nullptr;
+  check(gu_power_stop(h, &r), "end_power_sample");
+  energy_avg = r.energy_avg_w;
+  return r.avg_w;
+}
+
 size_t configure_grid(dim3 &dimGrid, dim3 &dimBlock, int start, int end) {
   int dev = 0;
   check_hip(hipGetDevice(&dev), "configure_grid");

@@ -8,6 +8,7 @@
 //   write_edgelist_to_file
 //   device_graph                          device_graph.h:7-22
 //   start_clock, end_clock, choose_device util_device.cuh:38-41
+//   power_measurer                        util_device.cuh:47-62, util_device.cu:79-130
 //   configure_grid, transfer_result       common.cuh:9-14
 //   multi_search_*_setup                  linear_with_atomics.cuh:14,
 //                                         edge_parallel.cuh:13, warp_based.cuh:15,
@@ -55,7 +56,7 @@ class program_options {
   bool verify;
   int approx;
   char *format;
-  bool isTesla;  // kept for layout compatibility; power is not sampled here
+  bool isTesla;  // kept for layout compatibility; power_measurer asks AMD SMI instead
   bool nvml;
 };
 
@@ -104,6 +105,31 @@ class device_graph {
 void start_clock(hipEvent_t &start, hipEvent_t &end);
 float end_clock(hipEvent_t &start, hipEvent_t &end);  // seconds
 void choose_device(program_options &op);
+
+// Samples the chosen device's socket power through AMD SMI every `p` ms
+// between start_power_sample() and end_power_sample(), on a host thread.
+// end_power_sample() synchronizes the device and returns the mean of the
+// samples in W, or -1 (after a warning at start) where the board's power
+// cannot be read, as the reference does on boards other than Tesla.
+class power_measurer {
+ public:
+  power_measurer(long p, const program_options &op) : period(p), device(op.device < 0 ? 0 : op.device) {}
+  ~power_measurer();
+  power_measurer(const power_measurer &) = delete;
+  power_measurer &operator=(const power_measurer &) = delete;
+  void start_power_sample();
+  double end_power_sample();
+  // energy used between start and end divided by the interval, W (-1 when
+  // the board has no energy counter); valid after end_power_sample()
+  double energy_average() const { return energy_avg; }
+
+ private:
+  power_measurer();
+  long period;
+  int device;
+  gu_power_t handle = nullptr;
+  double energy_avg = -1.0;
+};
 
 // Grid = compute units, block = max threads per block; returns the number of
 // result rows min(end - start, compute units).

@@ -41,6 +41,7 @@ extern "C" {
 #define GU_ERR_NODEVICE (-8)    /* no HIP device */
 #define GU_ERR_CAPACITY (-9)    /* output buffer too small / over budget */
 #define GU_ERR_TIMEOUT (-10)    /* a wait ran out of time */
+#define GU_ERR_UNSUPPORTED (-11) /* the host cannot provide it (no power readings) */
 
 /* Expansion strategies for top-down levels.  Each mirrors one reference
  * kernel; all produce identical distances and path counts. */
@@ -131,6 +132,31 @@ int gu_device_count(int32_t *count);
  * the given stream. */
 int gu_timer_start(void *stream, void **timer);
 int gu_timer_stop(void *timer, double *ms);
+/* power_measurer (util_device.cuh:47-62, util_device.cu:79-130): samples
+ * the device's socket power through AMD SMI every period_ms on a host
+ * thread between start and stop (nvmlDeviceGetPowerUsage there), and reads
+ * the device's energy accumulator at both ends.  gu_power_start returns
+ * GU_ERR_UNSUPPORTED where AMD SMI cannot be loaded or reports no power for
+ * the device (the reference returns -1 on boards NVML cannot read).
+ * gu_power_stop synchronizes the device first, as end_power_sample does,
+ * fills *result and frees the handle. */
+typedef struct gu_power_s *gu_power_t;
+typedef struct {
+  double avg_w;        /* mean of the samples, W (-1 without samples) */
+  double max_w;        /* largest sample, W */
+  double energy_avg_w; /* energy used / interval, W (-1 without the counter) */
+  double energy_j;     /* energy used over the interval, J (-1 without it) */
+  double seconds;      /* wall time from start to stop */
+  int32_t samples;
+} gu_power_result;
+int gu_power_start(int32_t device, int32_t period_ms, gu_power_t *out);
+int gu_power_stop(gu_power_t p, gu_power_result *result);
+/* Trace ranges (no counterpart in the reference): roctxRangePushA /
+ * roctxRangePop, so `rocprofv3 --marker-trace` attributes kernels to the
+ * caller's phases.  The engine marks its own phases the same way.  Return
+ * GU_ERR_UNSUPPORTED when the roctx library is absent (then a no-op). */
+int gu_trace_push(const char *message);
+int gu_trace_pop(void);
 /* Memory helpers so host code needs no other runtime. */
 int gu_malloc(void **p_dev, size_t bytes);
 int gu_free(void *p_dev);

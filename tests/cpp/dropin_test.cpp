@@ -5,7 +5,7 @@
 // examples/diameter_sampling: parse -> device_graph -> *_setup -> verify the
 // stored rows (row r holds the last source s = start + r mod grid).
 //
-// usage: dropin_test -i <graph.metis> -f dimacs [-k <sources>]
+// usage: dropin_test -i <graph.metis> -f dimacs [-k <sources>] [-n]
 // Exit status 0 when every check passes.
 #include <climits>
 #include <cmath>
@@ -53,7 +53,16 @@ int main(int argc, char **argv) {
   int start = 0, end = op.approx ? std::min(op.approx, g.n) : g.n;
   std::cout << "graph n=" << g.n << " m=" << g.m << " sources=" << end - start << std::endl;
   bool ok = true;
-  ok &= check_dist(g, multi_search_linear_atomics_setup(g_d, start, end), start, end, "Linear with atomics");
+  {
+    // -n/--nvml: average power around the search (multi_search_test.cu:173-179)
+    power_measurer pmes(10, op);
+    if (op.nvml) pmes.start_power_sample();
+    std::vector<std::vector<int>> r = multi_search_linear_atomics_setup(g_d, start, end);
+    if (op.nvml)
+      std::cout << "Average power: " << pmes.end_power_sample() << " W (energy counter: " << pmes.energy_average()
+                << " W)" << std::endl;
+    ok &= check_dist(g, r, start, end, "Linear with atomics");
+  }
   ok &= check_dist(g, multi_search_edge_parallel_setup(g_d, start, end), start, end, "Edge parallel");
   ok &= check_dist(g, multi_search_warp_based_setup(g_d, start, end), start, end, "Warp based");
   ok &= check_dist(g, multi_search_scan_based_setup(g_d, start, end), start, end, "Scan based");

@@ -34,7 +34,7 @@ def test_library_exports_every_declared_symbol(gu):
 
 
 def test_status_strings(gu):
-    codes = [0, -1, -2, -3, -4, -5, -6, -7, -8, -9, -10]
+    codes = [0, -1, -2, -3, -4, -5, -6, -7, -8, -9, -10, -11]
     msgs = [gu.lib().gu_strerror(c) for c in codes]
     assert all(isinstance(m, bytes) and m for m in msgs)
     assert len(set(msgs)) == len(msgs)
