@@ -207,8 +207,16 @@ def bench_grid(args, gu, props, steps=None, warmup=None, cpu_sources=96):
         t2 = time.perf_counter()
         diam, dst = dg.diameter(dsrc, dcfg)
         dwall = time.perf_counter() - t2
-    dist.close()
-    sig.close()
+        dist.close()
+        sig.close()
+        queue = queue_leg(gu, hg, dg, args.engine, 0 if args.no_queue else args.queue_sources)
+    if queue:
+        # uneven depths: sources over one 2048^2 grid and 48 grids of 256^2
+        hu = grid_union(gu)
+        with gu.DeviceGraph(hu) as du:
+            queue = {"grid2048": queue,
+                     "grid2048_plus_48_grid256": dict(queue_leg(gu, hu, du, args.engine, args.queue_sources),
+                                                      n=hu.n)}
     units = sum(s.teps_units for s in stats)
     dev_ms = sum(s.device_ms for s in stats)
     levels = stats[-1].max_level + 1
@@ -237,7 +245,56 @@ def bench_grid(args, gu, props, steps=None, warmup=None, cpu_sources=96):
                                scope="one 256-source per-source-engine pass (traversal and the "
                                      "row write-back kernels)"),
          "diameter_sampling": {"sources": 64, "diameter": diam, "ms": round(dwall * 1e3, 3),
-                               "levels": dst.max_level + 1}, **extra})
+                               "levels": dst.max_level + 1},
+         "source_queue": queue, **extra})
+
+
+def grid_union(gu, big=2048, small=256, copies=48, drop=0.10, seed=31):
+    """One big grid and `copies` small ones as one graph (block-diagonal
+    CSR, each grid with `drop` of its edges deleted, largest component):
+    sources drawn uniformly over its vertices have very uneven depths
+    (about 2k-4k levels in the big grid, a few hundred in a small one)."""
+    parts = [gu.HostGraph.grid(big, drop, seed=seed)]
+    parts += [gu.HostGraph.grid(small, drop, seed=seed + 1 + i) for i in range(copies)]
+    Rs, Cs, off, base = [np.zeros(1, dtype=np.int64)], [], 0, 0
+    for p in parts:
+        Rs.append(p.R[1:].astype(np.int64) + base)
+        Cs.append(p.C.astype(np.int64) + off)
+        off += p.n
+        base += p.m
+    R = np.concatenate(Rs).astype(np.int32)
+    return gu.HostGraph(R, np.concatenate(Cs).astype(np.int32))
+
+
+def queue_leg(gu, hg, dg, engine, k, seed=3997):
+    """One call of k sources (several launches' worth) with path counts, as
+    launches of 256 sources and as the queue-fed grid (workgroups take
+    sources from a device counter): device time, GTEPS and the deepest and
+    mean source depth, which bound the two forms (each launch waits for its
+    deepest source; the queue only for the last ones)."""
+    if not k:
+        return None
+    n = hg.n
+    src = hg.sample_sources(k, seed=seed)
+    db, sb = gu.DeviceBuffer(4 * k * n), gu.DeviceBuffer(8 * k * n)
+    out = {"sources": k}
+    try:
+        for name, mode in (("launches_of_256", 2), ("queue", 1)):
+            cfg = gu.make_config(engine=engine, source_queue=mode)
+            dg.multi_search_device(src, db, sb, cfg)  # warm-up (workspace)
+            st = dg.multi_search_device(src, db, sb, cfg)
+            out[name] = {"ms": round(st.device_ms, 3), "gteps": round(st.teps_units / st.device_ms / 1e6, 2),
+                         "launches": st.batches, "max_level": st.max_level}
+        ecc, _ = dg.multi_search_rows(src, None, None, want_ecc=True,
+                                      config=gu.make_config(engine=engine, source_queue=1))
+        out["depth_max"], out["depth_mean"] = int(ecc.max()), round(float(ecc.mean()), 1)
+        # launches of 256 wait for each launch's deepest source
+        out["depth_sum_of_launch_maxima"] = int(sum(ecc[i:i + 256].max() for i in range(0, k, 256)))
+        out["speedup"] = round(out["launches_of_256"]["ms"] / out["queue"]["ms"], 3)
+    finally:
+        db.close()
+        sb.close()
+    return out
 
 
 def bench_bc(args, gu, props, steps=None, cpu_sources=8):
@@ -636,6 +693,10 @@ def main():
                     help="--workload lbs: spread the 2^26 items over this many Zipf-weighted segments "
                          "(default: Zipf(1.1) counts capped at 2^20, 1805 segments)")
     ap.add_argument("--no-variants", action="store_true", help="skip the per-variant sweep")
+    ap.add_argument("--queue-sources", type=int, default=1024,
+                    help="grid workload: sources of the call timed as launches of 256 and as the "
+                         "queue-fed grid")
+    ap.add_argument("--no-queue", action="store_true", help="grid workload: skip that comparison")
     ap.add_argument("--no-serve", action="store_true",
                     help="bfs workload: skip the serving stream reported beside the GTEPS figure")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (degree-profile defaults)")

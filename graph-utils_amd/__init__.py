@@ -88,7 +88,7 @@ class Config(ctypes.Structure):
         ("sync_every", ctypes.c_int32),
         ("engine", ctypes.c_int32),
         ("level_graphs", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("source_queue", ctypes.c_int32),
         ("stream", ctypes.c_void_p),
     ]
 
@@ -302,10 +302,10 @@ def _take(p, count, dtype=np.int32):
 
 
 def make_config(variant="auto", direction=DIR_AUTO, batch_width=0, wg_size=0, alpha=0,
-                beta=0, sync_every=0, engine=ENGINE_AUTO, level_graphs=0):
+                beta=0, sync_every=0, engine=ENGINE_AUTO, level_graphs=0, source_queue=0):
     v = VARIANTS[variant] if isinstance(variant, str) else int(variant)
     return Config(v, int(direction), int(batch_width), int(wg_size), int(alpha), int(beta),
-                  int(sync_every), int(engine), int(level_graphs), 0, None)
+                  int(sync_every), int(engine), int(level_graphs), int(source_queue), None)
 
 
 # --------------------------------------------------------------------------

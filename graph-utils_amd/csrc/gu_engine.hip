@@ -1028,9 +1028,17 @@ struct PsWorkspace {
   int *depth = nullptr, *ovf = nullptr;
   int *srcs = nullptr;      // [cap] sources of the launch
   u64 *part = nullptr;
+  // queue-fed calls (more sources than one launch): per-source tables
+  int jcap = 0;             // sources they hold
+  int *jsrcs = nullptr;     // [jcap] sources (cluster ids where reordered)
+  void **jrows = nullptr;   // [4][jcap] row tables: dist, sigma, ddst, sdst
+  u64 *jpart = nullptr;     // [jcap][4]
+  int *jdepth = nullptr;    // [jcap]
+  int *head = nullptr;      // the queue's counter
   size_t bytes = 0;
   void release() {
-    void *ps[] = {queues, dscratch, sscratch, xscratch, qscratch, depth, ovf, srcs, part};
+    void *ps[] = {queues, dscratch, sscratch, xscratch, qscratch, depth, ovf, srcs, part,
+                  jsrcs, jrows, jpart, jdepth, head};
     for (void *p : ps)
       if (p) (void)hipFree(p);
     *this = PsWorkspace();
@@ -1569,11 +1577,158 @@ struct HostRows {
   int32_t *wrap = nullptr;
 };
 
+int run_persource(gu_graph *g, const int *sources, int nsrc, const HostRows &hr, bool sigma,
+                  gu_stats &s, int &dmax, hipStream_t st, int queue = 0, bool allow_lds = true);
+
+int ensure_jobs(gu_graph *g, int nsrc) {
+  PsWorkspace &p = g->pws;
+  if (p.jcap >= nsrc) return GU_OK;
+  void *ps[] = {p.jsrcs, p.jrows, p.jpart, p.jdepth};
+  for (void *q : ps)
+    if (q) (void)hipFree(q);
+  p.bytes -= (size_t)p.jcap * (4 + 32 + 32 + 4);
+  p.jsrcs = nullptr, p.jrows = nullptr, p.jpart = nullptr, p.jdepth = nullptr, p.jcap = 0;
+  const int cap = std::max(nsrc, 2 * kPsCap);
+  GU_HIP(dalloc(&p.jsrcs, (size_t)cap));
+  GU_HIP(dalloc(&p.jrows, 4 * (size_t)cap));
+  GU_HIP(dalloc(&p.jpart, 4 * (size_t)cap));
+  GU_HIP(dalloc(&p.jdepth, (size_t)cap));
+  if (!p.head) GU_HIP(dalloc(&p.head, 1));
+  p.jcap = cap;
+  p.bytes += (size_t)cap * (4 + 32 + 32 + 4);
+  return GU_OK;
+}
+
+// Per-source searches of a whole call as one grid: at most kPsCap
+// workgroups (one per compute unit) take the call's sources from a device
+// queue, so a workgroup whose source finishes early starts the next one
+// instead of waiting for the launch's deepest search.  On a
+// cluster-ordered graph each workgroup searches in its own scratch rows
+// and writes its source's rows back to the caller's order when the search
+// ends.  Sources whose LDS level sets overflow are searched again on the
+// distance-reading kernel.
+int run_persource_queue(gu_graph *g, const int *sources, int nsrc, const HostRows &hr, bool sigma,
+                        gu_stats &s, int &dmax, hipStream_t st) {
+  const bool rl = g->LR != nullptr;
+  int rc = ensure_pws(g, true);
+  if (rc) return rc;
+  if ((rc = ensure_jobs(g, nsrc))) return rc;
+  PsWorkspace &p = g->pws;
+  const size_t n = (size_t)g->n;
+  if (rl && sigma && !p.sscratch) {
+    GU_HIP(dalloc(&p.sscratch, (size_t)kPsCap * n));
+    p.bytes += (size_t)kPsCap * n * 8;
+  }
+  const int32_t *nof = rl ? nof_host(g).data() : nullptr;
+  std::vector<int> srcs((size_t)nsrc);
+  std::vector<void *> rows(4 * (size_t)nsrc, nullptr);
+  for (int j = 0; j < nsrc; j++) {
+    srcs[j] = rl ? nof[sources[j]] : sources[j];
+    int *dr = hr.dist ? hr.dist[j] : nullptr;
+    u64 *sr = sigma && hr.sigma ? hr.sigma[j] : nullptr;
+    if (sigma && !sr) return GU_ERR_INVALID;  // path counts need every row
+    if (rl) {  // searched in the slot's scratch rows, written back by the workgroup
+      rows[2 * (size_t)nsrc + j] = dr;
+      rows[3 * (size_t)nsrc + j] = sr;
+    } else {  // searched in the caller's rows (the slot's scratch when no distances)
+      rows[j] = dr;
+      rows[(size_t)nsrc + j] = sr;
+    }
+  }
+  GU_HIP(hipMemcpyAsync(p.jsrcs, srcs.data(), sizeof(int) * nsrc, hipMemcpyHostToDevice, st));
+  GU_HIP(hipMemcpyAsync(p.jrows, rows.data(), sizeof(void *) * 4 * nsrc, hipMemcpyHostToDevice, st));
+  k_fill_i32<<<1, 64, 0, st>>>(p.ovf, 1, 0);
+  k_fill_i32<<<1, 64, 0, st>>>(p.head, 1, 0);
+  PsQueue q{nsrc, p.head, p.dscratch, p.sscratch, rl ? g->nof : nullptr,
+            rl ? (int *const *)(p.jrows + 2 * (size_t)nsrc) : nullptr,
+            rl && sigma ? (u64 *const *)(p.jrows + 3 * (size_t)nsrc) : nullptr};
+  void *const *dtab = (void *const *)p.jrows;
+  u64 *const *stab = sigma ? (u64 *const *)(p.jrows + (size_t)nsrc) : nullptr;
+  const int grid = std::min(nsrc, kPsCap);
+  const int4 *LE = rl ? g->LE : nullptr;
+  const bool lds = LE && !g->directed && sigma;
+  {
+    GuTraceRange tr("gu:per-source queue");
+    if (lds) {
+      PsLdsArgs c{LE, g->n, p.jsrcs, (int *const *)dtab, stab, p.queues, p.jdepth, p.jpart, p.ovf, g->lds_flags,
+                  g->LE8};
+      c.q = q;
+      k_ps_lds<true, true><<<grid, 1024, 0, st>>>(c);
+    } else {
+      PsArgs a{rl ? g->LR : g->R, rl ? g->LC : g->C, LE, g->n, p.jsrcs, (int *const *)dtab, stab, p.queues,
+               p.jdepth, p.jpart, p.ovf};
+      a.q = q;
+      if (LE) {
+        if (sigma && !g->directed)
+          k_persource<true, true, true><<<grid, 1024, 0, st>>>(a);
+        else if (sigma)
+          k_persource<true, false, true><<<grid, 1024, 0, st>>>(a);
+        else
+          k_persource<false, false, true><<<grid, 1024, 0, st>>>(a);
+      } else if (sigma && !g->directed) {
+        k_persource<true, true><<<grid, 1024, 0, st>>>(a);
+      } else if (sigma) {
+        k_persource<true, false><<<grid, 1024, 0, st>>>(a);
+      } else {
+        k_persource<false, false><<<grid, 1024, 0, st>>>(a);
+      }
+    }
+    GU_HIP(hipGetLastError());
+  }
+  std::vector<u64> part(4 * (size_t)nsrc);
+  std::vector<int> depth((size_t)nsrc);
+  GU_HIP(hipMemcpyAsync(part.data(), p.jpart, sizeof(u64) * 4 * nsrc, hipMemcpyDeviceToHost, st));
+  GU_HIP(hipMemcpyAsync(depth.data(), p.jdepth, sizeof(int) * nsrc, hipMemcpyDeviceToHost, st));
+  GU_HIP(hipMemcpyAsync(g->misc_h, p.ovf, sizeof(int), hipMemcpyDeviceToHost, st));
+  GU_HIP(hipStreamSynchronize(st));
+  s.sigma_overflow |= g->misc_h[0];
+  // sources whose level sets overflowed: searched again, rows included
+  std::vector<int> again;
+  for (int j = 0; j < nsrc; j++) {
+    if (lds && (part[4 * (size_t)j + 3] & 2)) {
+      again.push_back(j);
+      continue;
+    }
+    s.teps_units += (int64_t)part[4 * (size_t)j];
+    s.reached += (int64_t)part[4 * (size_t)j + 1];
+    s.dag_edges += (int64_t)part[4 * (size_t)j + 2];
+    dmax = std::max(dmax, depth[j]);
+    if (hr.ecc) hr.ecc[j] = depth[j];
+    if (hr.wrap) hr.wrap[j] = sigma && (part[4 * (size_t)j + 3] & 1) != 0;
+  }
+  s.batches++;
+  if (again.empty()) return GU_OK;
+  std::vector<int> asrc(again.size());
+  std::vector<int32_t *> ad(again.size());
+  std::vector<u64 *> as(again.size());
+  std::vector<int32_t> ecc2(again.size()), wrap2(again.size());
+  for (size_t i = 0; i < again.size(); i++) {
+    asrc[i] = sources[again[i]];
+    ad[i] = hr.dist ? hr.dist[again[i]] : nullptr;
+    as[i] = hr.sigma ? hr.sigma[again[i]] : nullptr;
+  }
+  HostRows ar;
+  ar.dist = hr.dist ? ad.data() : nullptr;
+  ar.sigma = hr.sigma ? as.data() : nullptr;
+  ar.ecc = hr.ecc ? ecc2.data() : nullptr;
+  ar.wrap = hr.wrap ? wrap2.data() : nullptr;
+  rc = run_persource(g, asrc.data(), (int)again.size(), ar, sigma, s, dmax, st, 2, false);
+  if (rc) return rc;
+  for (size_t i = 0; i < again.size(); i++) {
+    if (hr.ecc) hr.ecc[again[i]] = ecc2[i];
+    if (hr.wrap) hr.wrap[again[i]] = wrap2[i];
+  }
+  return GU_OK;
+}
+
 // Per-source engine over nsrc sources, kPsCap workgroups per launch.  On a
 // cluster-ordered graph the traversal runs in new ids on scratch rows and
-// k_ps_unpermute writes the caller's rows.
+// k_ps_unpermute writes the caller's rows.  queue: 0 = the queue-fed grid
+// when the sources exceed one launch, 1 = always, 2 = never; allow_lds =
+// false keeps the search on the distance-reading kernels.
 int run_persource(gu_graph *g, const int *sources, int nsrc, const HostRows &hr, bool sigma,
-                  gu_stats &s, int &dmax, hipStream_t st) {
+                  gu_stats &s, int &dmax, hipStream_t st, int queue, bool allow_lds) {
+  if (queue == 1 || (queue == 0 && nsrc > kPsCap)) return run_persource_queue(g, sources, nsrc, hr, sigma, s, dmax, st);
   const bool rl = g->LR != nullptr;
   int rc = ensure_pws(g, rl || hr.dist == nullptr);
   if (rc) return rc;
@@ -1616,6 +1771,7 @@ int run_persource(gu_graph *g, const int *sources, int nsrc, const HostRows &hr,
     const int4 *LE = rl ? g->LE : nullptr;
     PsArgs a{rl ? g->LR : g->R, rl ? g->LC : g->C, LE, g->n, p.srcs, (int *const *)g->rowtab,
              sigma ? (u64 *const *)(g->rowtab + 256) : nullptr, p.queues, p.depth, p.part, p.ovf};
+    a.q.nsrc = nb;
     auto fetch = [&]() -> int {
       GU_HIP(hipMemcpyAsync(part, p.part, sizeof(u64) * 4 * nb, hipMemcpyDeviceToHost, st));
       GU_HIP(hipMemcpyAsync(depth, p.depth, sizeof(int) * nb, hipMemcpyDeviceToHost, st));
@@ -1628,12 +1784,13 @@ int run_persource(gu_graph *g, const int *sources, int nsrc, const HostRows &hr,
     // overflowed runs again on the distance-reading kernel.  Searches
     // without path counts stay on that kernel: 64-source diameter sampling
     // on a 2048^2 grid took 34.4 ms there against 39.9 ms over level sets.
-    bool lds = LE && !g->directed && sigma;
+    bool lds = allow_lds && LE && !g->directed && sigma;
     const bool rows = hr.dist != nullptr || sigma;
     if (lds) {
       PsLdsArgs c{LE,      g->n,   p.srcs, rows ? (int *const *)g->rowtab : nullptr,
                   sigma ? (u64 *const *)(g->rowtab + 256) : nullptr,
                   p.queues, p.depth, p.part, p.ovf, g->lds_flags, g->LE8};
+      c.q.nsrc = nb;
       GuTraceRange tr("gu:per-source level sets");
       k_ps_lds<true, true><<<nb, 1024, 0, st>>>(c);
       GU_HIP(hipGetLastError());
@@ -1846,6 +2003,7 @@ int run_job(gu_graph *g, const int32_t *sources, int32_t nsrc, Job job, const Ho
   if (cfg.direction < 0 || cfg.direction > 2) return GU_ERR_INVALID;
   if (cfg.variant < 0 || cfg.variant > GU_VARIANT_AUTO) return GU_ERR_INVALID;
   if (cfg.engine < 0 || cfg.engine > GU_ENGINE_PER_SOURCE) return GU_ERR_INVALID;
+  if (cfg.source_queue < 0 || cfg.source_queue > 2) return GU_ERR_INVALID;
   if (cfg.wg_size != 0 && cfg.wg_size != 256 && cfg.wg_size != 512 && cfg.wg_size != 1024)
     return GU_ERR_INVALID;
   if (nsrc < 0 || (nsrc > 0 && !sources)) return GU_ERR_INVALID;
@@ -1871,7 +2029,7 @@ int run_job(gu_graph *g, const int32_t *sources, int32_t nsrc, Job job, const Ho
   int dmax = 0;
   if (s.engine_used == GU_ENGINE_PER_SOURCE && nsrc > 0)
     rc = job == JOB_BC ? run_persource_bc(g, sources, nsrc, hr, s, dmax, st)
-                       : run_persource(g, sources, nsrc, hr, want_sigma, s, dmax, st);
+                       : run_persource(g, sources, nsrc, hr, want_sigma, s, dmax, st, cfg.source_queue);
   for (int b0 = 0; s.engine_used == GU_ENGINE_BATCHED && b0 < nsrc && rc == GU_OK; b0 += width) {
     int nb = std::min(width, nsrc - b0);
     void **t = g->rowtab_h;

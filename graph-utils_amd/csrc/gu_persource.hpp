@@ -19,6 +19,86 @@
 
 namespace gu {
 
+// How the workgroups of a per-source launch find their sources.
+//  - One launch per batch (head == null): workgroup b takes source b
+//    (nsrc = the grid); its rows are dist[b] / sigma[b].
+//  - Queue (head != null): a grid of at most one workgroup per compute
+//    unit takes sources from the device counter *head (cleared by the
+//    host before the launch) until all nsrc are taken, so a call's sources
+//    run as one grid and no workgroup waits for another's deeper search
+//    (the reference's block-strided source loop, shuffle_based.cuh:104).
+//    A null entry of the row tables means the workgroup's scratch row
+//    (dscr / sscr + slot * n).  With nof set, the rows were searched in
+//    cluster order and the workgroup writes them back itself when its
+//    search ends: ddst[j][v] = d[nof[v]] (INT_MAX where not reached),
+//    sdst[j][v] = sigma[nof[v]] (0 where not reached); a null table or
+//    entry is not written.
+struct PsQueue {
+  int nsrc;
+  int *head;
+  int *dscr;          // [grid][n] distance scratch rows (int, or 16-bit)
+  u64 *sscr;          // [grid][n] path-count scratch rows
+  const int *nof;     // input id -> cluster id, or null (no write-back)
+  int *const *ddst;   // [nsrc] the caller's distance rows
+  u64 *const *sdst;   // [nsrc] the caller's path-count rows
+};
+
+// The next source of workgroup `slot` (uniform across the workgroup), or
+// nsrc when none is left.
+__device__ __forceinline__ int ps_next_job(const PsQueue &q, int *s_job, bool first) {
+  if (!q.head) return first ? (int)blockIdx.x : q.nsrc;
+  if (threadIdx.x == 0) *s_job = atomicAdd(q.head, 1);
+  __syncthreads();
+  return *s_job;
+}
+
+// Row j's distances and path counts back to the caller's vertex order (see
+// PsQueue), by the workgroup that searched them; `none` marks an
+// unreached vertex in the distance row.  Called after a workgroup
+// barrier, so the rows' values written by every wave are visible.
+template <class DT>
+__device__ __forceinline__ void ps_writeback(const PsQueue &q, int j, const DT *ds, const u64 *ss, int n,
+                                             DT none) {
+  int *dd = q.ddst ? q.ddst[j] : nullptr;
+  u64 *sd = q.sdst && ss ? q.sdst[j] : nullptr;
+  if (!dd && !sd) return;
+  // One workgroup moves a whole row pair, so each thread keeps U vertices'
+  // gathers in flight: the map entries, then every row gather, then the
+  // stores (the stores may alias nothing the loads read, which the
+  // compiler cannot know, so the order is written out).
+  constexpr int U = 8;
+  const int T = blockDim.x;
+  int v0 = threadIdx.x;
+  for (; v0 + (U - 1) * T < n; v0 += U * T) {
+    int x[U];
+    DT dv[U];
+    u64 sv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) x[u] = q.nof[v0 + u * T];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      dv[u] = ds[x[u]];
+      sv[u] = sd ? ss[x[u]] : 0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const bool r = dv[u] != none;
+      if (dd) dd[v0 + u * T] = r ? (int)dv[u] : INT_MAX;
+      if (sd) sd[v0 + u * T] = r ? sv[u] : 0ull;
+    }
+  }
+  for (int v = v0; v < n; v += T) {
+    const int x = q.nof[v];
+    const DT dv = ds[x];
+    const bool r = dv != none;
+    if (dd) dd[v] = r ? (int)dv : INT_MAX;
+    if (sd) {
+      const u64 sv = ss[x];
+      sd[v] = r ? sv : 0ull;
+    }
+  }
+}
+
 struct PsArgs {
   const int *R;
   const int *C;
@@ -32,6 +112,7 @@ struct PsArgs {
   u64 *part;         // [nsrc][4] (TEPS units, reached vertices, DAG edges,
                      //   1 if one of the source's path counts wrapped)
   int *ovf;          // set when a path count wrapped
+  PsQueue q;         // source queue and in-kernel write-back
 };
 
 // Relaxed atomic loads at workgroup scope for the distance and path-count
@@ -399,10 +480,12 @@ __device__ int ps_levels(const int *R, const int *C, int n, int s, int *d, u64 *
   return L;
 }
 
-// Workgroup sums of the per-thread counters; thread 0 writes the source's
-// part / depth entries (part[3]: a path count of this source wrapped).
+// Workgroup sums of the per-thread counters; thread 0 writes entry j (the
+// source's index in the call) of part / depth (part[3]: a path count of
+// this source wrapped).
 __device__ __forceinline__ void ps_finish(u64 teps, u64 dag, u64 reached, int L, int ovf, int *s_ovf,
-                                          u64 (*s_red)[2], u64 *part, int *depth, int *ovf_out) {
+                                          u64 (*s_red)[2], u64 *part, int *depth, int *ovf_out,
+                                          int j = blockIdx.x) {
   const int tid = threadIdx.x, T = blockDim.x;
   for (int o = 32; o >= 1; o >>= 1) {
     teps += __shfl_xor(teps, o, kWave);
@@ -420,11 +503,11 @@ __device__ __forceinline__ void ps_finish(u64 teps, u64 dag, u64 reached, int L,
       t += s_red[w][0];
       g += s_red[w][1];
     }
-    part[4 * blockIdx.x] = t;
-    part[4 * blockIdx.x + 1] = reached;
-    part[4 * blockIdx.x + 2] = g;
-    part[4 * blockIdx.x + 3] = *s_ovf ? 1ull : 0ull;
-    depth[blockIdx.x] = L;
+    part[4 * (size_t)j] = t;
+    part[4 * (size_t)j + 1] = reached;
+    part[4 * (size_t)j + 2] = g;
+    part[4 * (size_t)j + 3] = *s_ovf ? 1ull : 0ull;
+    depth[j] = L;
     if (*s_ovf) atomicOr(ovf_out, 1);
   }
 }
@@ -434,23 +517,28 @@ __global__ __launch_bounds__(1024) void k_persource(PsArgs a) {
   __shared__ PsFront f;
   __shared__ unsigned s_cnt[3];
   __shared__ u64 s_red[16][2];
-  __shared__ int s_ovf;
+  __shared__ int s_ovf, s_job;
   const int tid = threadIdx.x, T = blockDim.x;
-  const int s = a.srcs[blockIdx.x];
-  int *d = a.dist[blockIdx.x];
-  u64 *sg = SIGMA ? a.sigma[blockIdx.x] : nullptr;
-  int *spill = a.queues + (size_t)blockIdx.x * (2 * (size_t)a.n + 2);
-  for (int i = tid; i < a.n; i += T) {
-    d[i] = INT_MAX;
-    if (SIGMA) sg[i] = 0;
+  const size_t slot = blockIdx.x, n = (size_t)a.n;
+  int *spill = a.queues + slot * (2 * n + 2);
+  for (int j = ps_next_job(a.q, &s_job, true); j < a.q.nsrc; j = ps_next_job(a.q, &s_job, false)) {
+    const int s = a.srcs[j];
+    int *d = a.dist && a.dist[j] ? a.dist[j] : a.q.dscr + slot * n;
+    u64 *sg = SIGMA ? (a.sigma[j] ? a.sigma[j] : a.q.sscr + slot * n) : nullptr;
+    for (int i = tid; i < a.n; i += T) {
+      d[i] = INT_MAX;
+      if (SIGMA) sg[i] = 0;
+    }
+    if (tid == 0) s_ovf = 0;
+    __syncthreads();
+    int ovf = 0;
+    u64 teps = 0, reached = 1, dag = 0;
+    const int L = ps_levels<SIGMA, PULL, false, ELL>(a.R, a.C, a.n, s, d, sg, spill, nullptr, nullptr, f, s_cnt,
+                                                     ovf, teps, reached, dag, a.LE);
+    ps_finish(teps, dag, reached, L, ovf, &s_ovf, s_red, a.part, a.depth, a.ovf, j);
+    if (a.q.nof) ps_writeback<int>(a.q, j, d, sg, a.n, INT_MAX);
+    __syncthreads();  // rows and s_job read before the next source
   }
-  if (tid == 0) s_ovf = 0;
-  __syncthreads();
-  int ovf = 0;
-  u64 teps = 0, reached = 1, dag = 0;
-  const int L = ps_levels<SIGMA, PULL, false, ELL>(a.R, a.C, a.n, s, d, sg, spill, nullptr, nullptr, f, s_cnt,
-                                                   ovf, teps, reached, dag, a.LE);
-  ps_finish(teps, dag, reached, L, ovf, &s_ovf, s_red, a.part, a.depth, a.ovf);
 }
 
 // Brandes on one workgroup per source (the dependency accumulation of
@@ -865,14 +953,16 @@ struct PsLdsArgs {
   const int4 *LE;
   int n;
   const int *srcs;    // [nsrc] (cluster-order ids)
-  int *const *dist;   // [nsrc] scratch rows (16-bit distances), or null (diameter)
+  int *const *dist;   // [nsrc] scratch rows (16-bit distances), or null
+                      //   (diameter; with a queue: the slot's scratch rows)
   u64 *const *sigma;  // [nsrc] rows, or null
-  int *queues;        // [nsrc][2n+2] frontier spill
+  int *queues;        // [grid][2n+2] frontier spill
   int *depth;
   u64 *part;          // [nsrc][4], as PsArgs; part[3] bit 1: level set overflow
   int *ovf;
   int flags;          // kLds* graph facts
   const u64 *LE8;     // or null: the neighbour table as int16 deltas
+  PsQueue q;          // source queue and in-kernel write-back
 };
 
 // The search: with rows (WANT_D) the distance row, in 16 bits (kD16None:
@@ -887,22 +977,31 @@ __global__ __launch_bounds__(1024) void k_ps_lds(PsLdsArgs a) {
   __shared__ PsLds f;
   __shared__ unsigned s_cnt[3];
   __shared__ u64 s_red[16][2];
-  __shared__ int s_ovf, s_bad;
+  __shared__ int s_ovf, s_bad, s_job;
   const int tid = threadIdx.x, T = blockDim.x;
-  const int s = a.srcs[blockIdx.x];
-  unsigned short *d = WANT_D ? reinterpret_cast<unsigned short *>(a.dist[blockIdx.x]) : nullptr;
-  u64 *sg = SIGMA ? a.sigma[blockIdx.x] : nullptr;
-  int *spill = a.queues + (size_t)blockIdx.x * (2 * (size_t)a.n + 2);
-  if (WANT_D && !(a.flags & kLdsConnected))
-    for (int i = tid; i < a.n; i += T) d[i] = kD16None;
-  if (tid == 0) s_ovf = 0;
-  __syncthreads();
-  int ovf = 0;
-  u64 teps = 0, reached = 1, dag = 0;
-  const int L = ps_levels_lds<SIGMA, false, WANT_D>(a.LE, a.LE8, a.n, s, d, sg, spill, nullptr, nullptr, f, s_cnt, &s_bad,
-                                                    ovf, teps, reached, dag, (a.flags & kLdsBipartite) != 0);
-  ps_finish(teps, dag, reached, L < 0 ? 0 : L, ovf, &s_ovf, s_red, a.part, a.depth, a.ovf);
-  if (tid == 0 && L < 0) a.part[4 * blockIdx.x + 3] |= 2ull;
+  const size_t slot = blockIdx.x, n = (size_t)a.n;
+  int *spill = a.queues + slot * (2 * n + 2);
+  for (int j = ps_next_job(a.q, &s_job, true); j < a.q.nsrc; j = ps_next_job(a.q, &s_job, false)) {
+    const int s = a.srcs[j];
+    unsigned short *d = nullptr;
+    if (WANT_D)
+      d = reinterpret_cast<unsigned short *>(a.dist && a.dist[j] ? a.dist[j] : a.q.dscr + slot * n);
+    u64 *sg = SIGMA ? (a.sigma[j] ? a.sigma[j] : a.q.sscr + slot * n) : nullptr;
+    if (WANT_D && !(a.flags & kLdsConnected))
+      for (int i = tid; i < a.n; i += T) d[i] = kD16None;
+    if (tid == 0) s_ovf = 0;
+    __syncthreads();
+    int ovf = 0;
+    u64 teps = 0, reached = 1, dag = 0;
+    const int L = ps_levels_lds<SIGMA, false, WANT_D>(a.LE, a.LE8, a.n, s, d, sg, spill, nullptr, nullptr, f,
+                                                      s_cnt, &s_bad, ovf, teps, reached, dag,
+                                                      (a.flags & kLdsBipartite) != 0);
+    ps_finish(teps, dag, reached, L < 0 ? 0 : L, ovf, &s_ovf, s_red, a.part, a.depth, a.ovf, j);
+    if (tid == 0 && L < 0) a.part[4 * (size_t)j + 3] |= 2ull;
+    // an overflowed source is searched again by the host, rows included
+    if (WANT_D && a.q.nof && L >= 0) ps_writeback<unsigned short>(a.q, j, d, sg, a.n, kD16None);
+    __syncthreads();  // rows and s_job read before the next source
+  }
 }
 
 // Brandes over LDS level sets: the forward above with the BFS order in S

@@ -87,7 +87,10 @@ typedef struct {
                            capture in progress makes HIP refuse legacy
                            (NULL) stream work from other threads, so graphs
                            are opt-in. */
-  int32_t reserved;
+  int32_t source_queue; /* per-source engine (searches): 0 = auto (one grid
+                           taking the call's sources from a device queue
+                           when they exceed one launch of 256), 1 = always
+                           the queue, 2 = launches of 256 sources */
   void *stream;         /* hipStream_t; NULL = the default stream */
 } gu_config;
 

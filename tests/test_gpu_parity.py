@@ -509,9 +509,9 @@ def test_persource_rmat14_and_hubs(gu, orc, gpu, rmat14):
 
 def test_persource_grid_and_diameter(gu, orc, gpu):
     hg = gu.HostGraph.grid(48, 0.10, seed=3)
-    src = hg.sample_sources(300, seed=4)  # more than one 256-source launch
+    src = hg.sample_sources(300, seed=4)  # more than one 256-source launch: the queue-fed grid
     st = _ps_compare(gu, orc, hg, src, "grid48")
-    assert st.batches == 2
+    assert st.batches == 1
     with gu.DeviceGraph(hg) as dg:
         diam, st = dg.diameter(src[:64], gu.make_config(engine=PS))
     assert st.engine_used == PS
