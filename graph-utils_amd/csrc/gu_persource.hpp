@@ -309,7 +309,7 @@ __device__ __forceinline__ void ps_visit_ell(const int4 *LE, int *d, u64 *sg, co
   unsigned claims = 0;
 #pragma unroll
   for (int j = 0; j < B; j++) claims += o[j] == INT_MAX;
-  unsigned pos = claims ? atomicAdd(cnt, claims) : 0u;
+  unsigned pos = wave_reserve(cnt, claims);
   if (SIGMA && !PULL) {
 #pragma unroll
     for (int j = 0; j < B; j++)
@@ -852,7 +852,7 @@ __device__ __forceinline__ void ps_visit_lds(const int4 q, DT *d, u64 *sg, int v
   unsigned claims = 0;
 #pragma unroll
   for (int j = 0; j < 4; j++) claims += got[j];
-  unsigned pos = claims ? atomicAdd(cnt, claims) : 0u;
+  unsigned pos = wave_reserve(cnt, claims);
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     if (!got[j]) continue;

@@ -28,6 +28,23 @@ __device__ __forceinline__ uint64_t readfirst64(uint64_t v) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+// First of `k` consecutive positions for this lane in a list whose length
+// counter is *cnt, with one atomic per wave instead of one per lane (a
+// wave prefix sum of the lanes' k places them).  Every lane of a full wave
+// must call it (wave-uniform control flow).  Up to 1024 lanes of a
+// workgroup taking positions from one LDS counter serialise on its
+// address; 16 wave atomics do not.
+__device__ __forceinline__ unsigned wave_reserve(unsigned *cnt, unsigned k) {
+  const int incl = wave_inclusive_sum((int)k);
+  const int tot = __builtin_amdgcn_readlane(incl, kWave - 1);
+  unsigned base = 0;
+  if (tot) {
+    if (lane_id() == kWave - 1) base = atomicAdd(cnt, (unsigned)tot);
+    base = (unsigned)__builtin_amdgcn_readlane((int)base, kWave - 1);
+  }
+  return base + (unsigned)(incl - (int)k);
+}
+
 // 64-bit OR over the wave (xor-butterfly of shuffles).
 __device__ __forceinline__ uint64_t wave_or(uint64_t x) {
 #pragma unroll
