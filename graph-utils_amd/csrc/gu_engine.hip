@@ -609,7 +609,7 @@ __global__ __launch_bounds__(256) void k_transpose_out(Eng<NW> e, const T *src, 
     // are loaded side by side; no load waits on another
     T x[16];
     u64 vm[16];
-    T *rp[16];
+    gp<T> rp[16];
 #pragma unroll
     for (int k = 0; k < 16; k++) {
       const long long v = v0 + row0 + 4 * k;
@@ -617,7 +617,7 @@ __global__ __launch_bounds__(256) void k_transpose_out(Eng<NW> e, const T *src, 
       x[k] = in ? __builtin_nontemporal_load(&src[(size_t)v * (kWL * NW) + j * kWL + col]) : fill;
       vm[k] = in ? e.visit[(size_t)v * NW + j] : 0ull;
       const int b = j * kWL + row0 + 4 * k;
-      rp[k] = b < e.nb ? rows[b] : nullptr;
+      rp[k] = glob(b < e.nb ? rows[b] : nullptr);
     }
 #pragma unroll
     for (int k = 0; k < 16; k++) tile[row0 + 4 * k][col] = ((vm[k] >> col) & 1ull) ? x[k] : fill;
@@ -626,7 +626,7 @@ __global__ __launch_bounds__(256) void k_transpose_out(Eng<NW> e, const T *src, 
     if (v < e.n) {
 #pragma unroll
       for (int k = 0; k < 16; k++) {
-        T *r = rp[k];
+        gp<T> r = rp[k];
         if (r) __builtin_nontemporal_store(tile[col][row0 + 4 * k], &r[v]);
       }
     }
@@ -930,7 +930,7 @@ __global__ __launch_bounds__(256) void k_bc_out(Eng<NW> e, const float *dl, floa
           int b = j * kWL + bb;
           long long v = v0 + col;
           if (b < e.nb && v < e.n) {
-            float *r = drows[b];
+            gp<float> r = glob(drows[b]);
             if (r) r[v] = tl[col][bb];
           }
         }
@@ -942,8 +942,8 @@ __global__ __launch_bounds__(256) void k_bc_out(Eng<NW> e, const float *dl, floa
           const int b = j * kWL + c;
           if (b >= e.nb) break;
           acc += (double)tl[tid][c];
-          double *t = bct[b];
-          if (b + 1 == e.nb || bct[b + 1] != t) {
+          gp<double> t = glob(bct[b]);
+          if (b + 1 == e.nb || glob(bct[b + 1]) != t) {
             if (t && v0 + tid < e.n) t[v0 + tid] += acc;
             acc = 0.0;
           }
@@ -1653,7 +1653,10 @@ int run_persource_queue(gu_graph *g, const int *sources, int nsrc, const HostRow
       PsLdsArgs c{LE, g->n, p.jsrcs, (int *const *)dtab, stab, p.queues, p.jdepth, p.jpart, p.ovf, g->lds_flags,
                   g->LE8};
       c.q = q;
-      k_ps_lds<true, true><<<grid, 1024, 0, st>>>(c);
+      if (g->LE8)
+        k_ps_lds<true, true, true><<<grid, 1024, 0, st>>>(c);
+      else
+        k_ps_lds<true, true, false><<<grid, 1024, 0, st>>>(c);
     } else {
       PsArgs a{rl ? g->LR : g->R, rl ? g->LC : g->C, LE, g->n, p.jsrcs, (int *const *)dtab, stab, p.queues,
                p.jdepth, p.jpart, p.ovf};
@@ -1792,7 +1795,10 @@ int run_persource(gu_graph *g, const int *sources, int nsrc, const HostRows &hr,
                   p.queues, p.depth, p.part, p.ovf, g->lds_flags, g->LE8};
       c.q.nsrc = nb;
       GuTraceRange tr("gu:per-source level sets");
-      k_ps_lds<true, true><<<nb, 1024, 0, st>>>(c);
+      if (g->LE8)
+        k_ps_lds<true, true, true><<<nb, 1024, 0, st>>>(c);
+      else
+        k_ps_lds<true, true, false><<<nb, 1024, 0, st>>>(c);
       GU_HIP(hipGetLastError());
       if ((rc = fetch())) return rc;
       for (int i = 0; i < nb; i++) lds &= (part[4 * i + 3] & 2) == 0;
@@ -1905,7 +1911,10 @@ int run_persource_bc(gu_graph *g, const int *sources, int nsrc, const HostRows &
     bool lds = LE && !g->directed;
     if (lds) {
       GuTraceRange tr("gu:per-source brandes level sets");
-      k_ps_bc_lds<<<nb, 1024, 0, st>>>(a);
+      if (g->LE8)
+        k_ps_bc_lds<true><<<nb, 1024, 0, st>>>(a);
+      else
+        k_ps_bc_lds<false><<<nb, 1024, 0, st>>>(a);
       GU_HIP(hipGetLastError());
       GU_HIP(hipMemcpyAsync(part, p.part, sizeof(u64) * 4 * nb, hipMemcpyDeviceToHost, st));
       GU_HIP(hipStreamSynchronize(st));

@@ -57,10 +57,11 @@ __device__ __forceinline__ int ps_next_job(const PsQueue &q, int *s_job, bool fi
 // unreached vertex in the distance row.  Called after a workgroup
 // barrier, so the rows' values written by every wave are visible.
 template <class DT>
-__device__ __forceinline__ void ps_writeback(const PsQueue &q, int j, const DT *ds, const u64 *ss, int n,
+__device__ __forceinline__ void ps_writeback(const PsQueue &q, int j, gp<const DT> ds, gp<const u64> ss, int n,
                                              DT none) {
-  int *dd = q.ddst ? q.ddst[j] : nullptr;
-  u64 *sd = q.sdst && ss ? q.sdst[j] : nullptr;
+  gp<int> dd = glob(q.ddst ? q.ddst[j] : nullptr);
+  gp<u64> sd = glob(q.sdst && ss ? q.sdst[j] : nullptr);
+  gp<const int> nof = glob(q.nof);
   if (!dd && !sd) return;
   // One workgroup moves a whole row pair, so each thread keeps U vertices'
   // gathers in flight: the map entries, then every row gather, then the
@@ -74,7 +75,7 @@ __device__ __forceinline__ void ps_writeback(const PsQueue &q, int j, const DT *
     DT dv[U];
     u64 sv[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) x[u] = q.nof[v0 + u * T];
+    for (int u = 0; u < U; u++) x[u] = nof[v0 + u * T];
 #pragma unroll
     for (int u = 0; u < U; u++) {
       dv[u] = ds[x[u]];
@@ -88,7 +89,7 @@ __device__ __forceinline__ void ps_writeback(const PsQueue &q, int j, const DT *
     }
   }
   for (int v = v0; v < n; v += T) {
-    const int x = q.nof[v];
+    const int x = nof[v];
     const DT dv = ds[x];
     const bool r = dv != none;
     if (dd) dd[v] = r ? (int)dv : INT_MAX;
@@ -129,6 +130,12 @@ __device__ __forceinline__ u64 ld_wg(const u64 *p) {
 __device__ __forceinline__ int ld_wg(const int *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
+__device__ __forceinline__ u64 ld_wg(gp<const u64> p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ int ld_wg(gp<const int> p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 
 #ifndef GU_PS_CAP
 #define GU_PS_CAP 3072
@@ -148,7 +155,7 @@ struct PsFront {
 constexpr int kPsSerialDeg = 32;
 
 // Reads entry k of buffer b: (vertex, adjacency bounds).
-__device__ __forceinline__ void ps_entry(const PsFront &f, const int *spill, const int *R, int b,
+__device__ __forceinline__ void ps_entry(const PsFront &f, gp<const int> spill, const int *R, int b,
                                          unsigned k, int &v, int &r, int &re) {
   if (k < (unsigned)kPsCapLds) {
     v = f.v[b][k];
@@ -163,7 +170,7 @@ __device__ __forceinline__ void ps_entry(const PsFront &f, const int *spill, con
 
 // Appends claimed vertex w at position p of the next buffer.  The bounds
 // are loaded here, while the level's path-count adds are in flight.
-__device__ __forceinline__ void ps_push(PsFront &f, int *spill, int *S, const int *R, int b, unsigned p,
+__device__ __forceinline__ void ps_push(PsFront &f, gp<int> spill, gp<int> S, const int *R, int b, unsigned p,
                                         int w) {
   if (S) S[p] = w;
   if (p < (unsigned)kPsCapLds) {
@@ -183,10 +190,10 @@ __device__ __forceinline__ void ps_push(PsFront &f, int *spill, int *S, const in
 // one line for the whole wave (re-reading a line the wave requests anyway
 // measured 4% slower with 256 sources in flight).
 template <bool AT, int B>
-__device__ __forceinline__ void ps_load_d(const int *d, const int *w, int *o) {
+__device__ __forceinline__ void ps_load_d(gp<const int> d, const int *w, int *o) {
 #pragma unroll
   for (int j = 0; j < B; j++) {
-    const int *p = d + (w[j] >= 0 ? w[j] : 0);
+    gp<const int> p = d + (w[j] >= 0 ? w[j] : 0);
     const int t = AT ? ld_wg(p) : *p;
     o[j] = w[j] >= 0 ? t : -1;
   }
@@ -197,7 +204,7 @@ __device__ __forceinline__ void ps_load_d(const int *d, const int *w, int *o) {
 // before the claims and these loads issue beside them), branch-free as in
 // ps_load_d (the other lanes read entry 0); x = 0 off the parents.
 template <int B>
-__device__ __forceinline__ void ps_load_parents(const u64 *sg, const int *w, const int *o, int L, bool *par,
+__device__ __forceinline__ void ps_load_parents(gp<const u64> sg, const int *w, const int *o, int L, bool *par,
                                                 u64 *x) {
 #pragma unroll
   for (int j = 0; j < B; j++) par[j] = w[j] >= 0 && L >= 1 && o[j] == L - 1;
@@ -225,9 +232,9 @@ __device__ __forceinline__ void ps_load_parents(const u64 *sg, const int *w, con
 // Lanes past the adjacency carry w = -1 and o = -1; every use of o that a
 // level can equal (o == L-1 is -1 at level 0) is guarded by w >= 0.
 template <bool SIGMA, bool PULL>
-__device__ __forceinline__ void ps_visit(const int *R, const int *C, int *d, u64 *sg, int v, int r, int re,
+__device__ __forceinline__ void ps_visit(const int *R, const int *C, gp<int> d, gp<u64> sg, int v, int r, int re,
                                          int L, u64 sv, int &ovf, u64 &dag, unsigned *cnt, PsFront &f,
-                                         int *spill, int *S, int nbuf) {
+                                         gp<int> spill, gp<int> S, int nbuf) {
   constexpr int B = 8;
   u64 acc = 0;
   for (; r < re; r += B) {
@@ -243,14 +250,14 @@ __device__ __forceinline__ void ps_visit(const int *R, const int *C, int *d, u64
       for (int j = 0; j < B; j++) par[j] = false, x[j] = 0;
 #pragma unroll
     for (int j = 0; j < B; j++)
-      if (o[j] == INT_MAX) o[j] = atomicCAS(&d[w[j]], INT_MAX, L + 1);  // INT_MAX back: claimed
+      if (o[j] == INT_MAX) o[j] = cas_g(d + w[j], INT_MAX, L + 1);  // INT_MAX back: claimed
     unsigned claims = 0;
 #pragma unroll
     for (int j = 0; j < B; j++) claims += o[j] == INT_MAX;
     unsigned pos = claims ? atomicAdd(cnt, claims) : 0u;
     if (SIGMA && !PULL) {
 #pragma unroll
-      for (int j = 0; j < B; j++) x[j] = (o[j] == INT_MAX || o[j] == L + 1) ? atomicAdd(&sg[w[j]], sv) : 0ull;
+      for (int j = 0; j < B; j++) x[j] = (o[j] == INT_MAX || o[j] == L + 1) ? add_g(sg + w[j], sv) : 0ull;
     }
 #pragma unroll
     for (int j = 0; j < B; j++)
@@ -278,9 +285,9 @@ __device__ __forceinline__ void ps_visit(const int *R, const int *C, int *d, u64
 // vertex enters the next frontier as its id alone, with no row-offset loads
 // before the level's barrier.
 template <bool SIGMA, bool PULL, int NV>
-__device__ __forceinline__ void ps_visit_ell(const int4 *LE, int *d, u64 *sg, const int *vv, int L,
+__device__ __forceinline__ void ps_visit_ell(const int4 *LE, gp<int> d, gp<u64> sg, const int *vv, int L,
                                              const u64 *sv, int &ovf, u64 &dag, u64 &teps, unsigned *cnt,
-                                             PsFront &f, int *spill, int *S, int nbuf) {
+                                             PsFront &f, gp<int> spill, gp<int> S, int nbuf) {
   constexpr int B = 4 * NV;
   int w[B], o[B];
 #pragma unroll
@@ -305,7 +312,7 @@ __device__ __forceinline__ void ps_visit_ell(const int4 *LE, int *d, u64 *sg, co
     for (int j = 0; j < B; j++) par[j] = false, x[j] = 0;
 #pragma unroll
   for (int j = 0; j < B; j++)
-    if (o[j] == INT_MAX) o[j] = atomicCAS(&d[w[j]], INT_MAX, L + 1);  // INT_MAX back: claimed
+    if (o[j] == INT_MAX) o[j] = cas_g(d + w[j], INT_MAX, L + 1);  // INT_MAX back: claimed
   unsigned claims = 0;
 #pragma unroll
   for (int j = 0; j < B; j++) claims += o[j] == INT_MAX;
@@ -313,7 +320,7 @@ __device__ __forceinline__ void ps_visit_ell(const int4 *LE, int *d, u64 *sg, co
   if (SIGMA && !PULL) {
 #pragma unroll
     for (int j = 0; j < B; j++)
-      x[j] = (o[j] == INT_MAX || o[j] == L + 1) ? atomicAdd(&sg[w[j]], sv[j / 4]) : 0ull;
+      x[j] = (o[j] == INT_MAX || o[j] == L + 1) ? add_g(sg + w[j], sv[j / 4]) : 0ull;
   }
 #pragma unroll
   for (int j = 0; j < B; j++) {
@@ -351,9 +358,9 @@ __device__ __forceinline__ void ps_visit_ell(const int4 *LE, int *d, u64 *sg, co
 // warp_based.cu:109-145); in PULL mode the lanes' partial path counts are
 // summed across the wave and the elected lane stores them.
 template <bool SIGMA, bool PULL>
-__device__ __forceinline__ void ps_visit_wave(const int *R, const int *C, int *d, u64 *sg, int v, int rr,
+__device__ __forceinline__ void ps_visit_wave(const int *R, const int *C, gp<int> d, gp<u64> sg, int v, int rr,
                                               int rre, int L, u64 sv, int &ovf, u64 &dag, unsigned *cnt,
-                                              PsFront &f, int *spill, int *S, int nbuf) {
+                                              PsFront &f, gp<int> spill, gp<int> S, int nbuf) {
   const int lane = lane_id();
   while (true) {
     const bool pend = rr < rre;
@@ -369,8 +376,8 @@ __device__ __forceinline__ void ps_visit_wave(const int *R, const int *C, int *d
       int w = -1, o = -1;
       if (ed < rew) {
         w = C[ed];
-        o = (SIGMA && PULL) ? ld_wg(&d[w]) : d[w];
-        if (o == INT_MAX) o = atomicCAS(&d[w], INT_MAX, L + 1);
+        o = (SIGMA && PULL) ? ld_wg(d + w) : d[w];
+        if (o == INT_MAX) o = cas_g(d + w, INT_MAX, L + 1);
       }
       const bool claimed = o == INT_MAX;
       uint64_t m = ballot(claimed);
@@ -381,13 +388,13 @@ __device__ __forceinline__ void ps_visit_wave(const int *R, const int *C, int *d
         b = (unsigned)readlane((int)b, leader);
       }
       if (SIGMA && PULL && w >= 0 && L >= 1 && o == L - 1) {
-        const u64 x = ld_wg(&sg[w]);
+        const u64 x = ld_wg(sg + w);
         acc += x;
         wrap |= acc < x;
         dag++;
       }
       if (SIGMA && !PULL && (claimed || o == L + 1)) {
-        const u64 old = atomicAdd(&sg[w], svw);
+        const u64 old = add_g(sg + w, svw);
         ovf |= old + svw < svw;
         dag++;
       }
@@ -412,9 +419,9 @@ __device__ __forceinline__ void ps_visit_wave(const int *R, const int *C, int *d
 // and a frontier's entries beyond the LDS capacity are read back from S;
 // otherwise they go to the two `spill` queues.  Returns the deepest level.
 template <bool SIGMA, bool PULL, bool BC, bool ELL = false>
-__device__ int ps_levels(const int *R, const int *C, int n, int s, int *d, u64 *sg, int *spill0, int *S,
-                         int *E, PsFront &f, unsigned *s_cnt, int &ovf, u64 &teps, u64 &reached, u64 &dag,
-                         const int4 *LE = nullptr) {
+__device__ __forceinline__ int ps_levels(const int *R, const int *C, int n, int s, gp<int> d, gp<u64> sg, gp<int> spill0,
+                         gp<int> S, gp<int> E, PsFront &f, unsigned *s_cnt, int &ovf, u64 &teps, u64 &reached,
+                         u64 &dag, const int4 *LE = nullptr) {
   const int tid = threadIdx.x, T = blockDim.x;
   if (tid < 3) s_cnt[tid] = 0;
   if (tid == 0) {
@@ -436,9 +443,9 @@ __device__ int ps_levels(const int *R, const int *C, int n, int s, int *d, u64 *
   while (true) {
     unsigned *cnt = &s_cnt[L % 3];
     if (tid == 0) s_cnt[(L + 1) % 3] = 0;
-    int *Snext = BC ? S + qs : nullptr;
-    int *sp_cur = BC ? S + qb + kPsCapLds : spill0 + (size_t)b * n;
-    int *sp_next = BC ? S + qs + kPsCapLds : spill0 + (size_t)(b ^ 1) * n;
+    gp<int> Snext = BC ? S + qs : nullptr;
+    gp<int> sp_cur = BC ? S + qb + kPsCapLds : spill0 + (size_t)b * n;
+    gp<int> sp_next = BC ? S + qs + kPsCapLds : spill0 + (size_t)(b ^ 1) * n;
     if (ELL) {  // one frontier vertex per thread and slice (2 or 3, with
                 // their loads side by side, measured 8-9% slower on a grid)
       constexpr int NV = 1;
@@ -449,7 +456,7 @@ __device__ int ps_levels(const int *R, const int *C, int n, int s, int *d, u64 *
         for (int i = 0; i < NV; i++) {
           const unsigned k = base + i * T + tid;
           vv[i] = k >= qn ? -1 : (k < (unsigned)kPsCapLds ? f.v[b][k] : sp_cur[k - kPsCapLds]);
-          sv[i] = (SIGMA && !PULL && vv[i] >= 0) ? ld_wg(&sg[vv[i]]) : 0ull;
+          sv[i] = (SIGMA && !PULL && vv[i] >= 0) ? ld_wg(sg + vv[i]) : 0ull;
         }
         ps_visit_ell<SIGMA, PULL, NV>(LE, d, sg, vv, L, sv, ovf, dag, teps, cnt, f, sp_next, Snext, b ^ 1);
       }
@@ -458,7 +465,7 @@ __device__ int ps_levels(const int *R, const int *C, int n, int s, int *d, u64 *
       const unsigned k = base + tid;
       int v = -1, r = 0, re = 0;
       if (k < qn) ps_entry(f, sp_cur, R, b, k, v, r, re);
-      const u64 sv = (SIGMA && !PULL && v >= 0) ? ld_wg(&sg[v]) : 0ull;
+      const u64 sv = (SIGMA && !PULL && v >= 0) ? ld_wg(sg + v) : 0ull;
       teps += (u64)(re - r);
       const bool big = re - r > kPsSerialDeg;
       ps_visit<SIGMA, PULL>(R, C, d, sg, v, big ? 0 : r, big ? 0 : re, L, sv, ovf, dag, cnt, f, sp_next,
@@ -520,11 +527,11 @@ __global__ __launch_bounds__(1024) void k_persource(PsArgs a) {
   __shared__ int s_ovf, s_job;
   const int tid = threadIdx.x, T = blockDim.x;
   const size_t slot = blockIdx.x, n = (size_t)a.n;
-  int *spill = a.queues + slot * (2 * n + 2);
+  gp<int> spill = glob(a.queues + slot * (2 * n + 2));
   for (int j = ps_next_job(a.q, &s_job, true); j < a.q.nsrc; j = ps_next_job(a.q, &s_job, false)) {
     const int s = a.srcs[j];
-    int *d = a.dist && a.dist[j] ? a.dist[j] : a.q.dscr + slot * n;
-    u64 *sg = SIGMA ? (a.sigma[j] ? a.sigma[j] : a.q.sscr + slot * n) : nullptr;
+    gp<int> d = glob(a.dist && a.dist[j] ? a.dist[j] : a.q.dscr + slot * n);
+    gp<u64> sg = glob(SIGMA ? (a.sigma[j] ? a.sigma[j] : a.q.sscr + slot * n) : nullptr);
     for (int i = tid; i < a.n; i += T) {
       d[i] = INT_MAX;
       if (SIGMA) sg[i] = 0;
@@ -582,7 +589,7 @@ __device__ __forceinline__ void ps_neumaier(float &sum, float &comp, float x) {
 // loads (the others read entry 0) and the values are selected afterwards;
 // qw = 0 off the successors.
 template <int B>
-__device__ __forceinline__ void ps_successors(const int *d, const float *ql, const int *w, int l, bool *on,
+__device__ __forceinline__ void ps_successors(gp<const int> d, gp<const float> ql, const int *w, int l, bool *on,
                                               float *qw) {
   int o[B];
   ps_load_d<true, B>(d, w, o);
@@ -603,12 +610,12 @@ __global__ __launch_bounds__(1024) void k_ps_bc(PsBcArgs a) {
   __shared__ int s_ovf;
   const int tid = threadIdx.x, T = blockDim.x;
   const int s = a.srcs[blockIdx.x];
-  int *d = a.dist[blockIdx.x];
-  u64 *sg = a.sigma[blockIdx.x];
-  float *dl = a.delta[blockIdx.x];
-  float *ql = a.quot[blockIdx.x];
-  int *S = a.stack + (size_t)blockIdx.x * (2 * (size_t)a.n + 2);
-  int *E = S + a.n;
+  gp<int> d = glob(a.dist[blockIdx.x]);
+  gp<u64> sg = glob(a.sigma[blockIdx.x]);
+  gp<float> dl = glob(a.delta[blockIdx.x]);
+  gp<float> ql = glob(a.quot[blockIdx.x]);
+  gp<int> S = glob(a.stack + (size_t)blockIdx.x * (2 * (size_t)a.n + 2));
+  gp<int> E = S + a.n;
   // frontier entries beyond the LDS capacity are read back from S itself:
   // a level's vertices occupy S[E[l], E[l+1]) in frontier order
   for (int i = tid; i < a.n; i += T) {
@@ -630,14 +637,14 @@ __global__ __launch_bounds__(1024) void k_ps_bc(PsBcArgs a) {
   // its successors' q together
   for (int i = E[L] + tid; i >= 0 && L >= 1 && i < E[L + 1]; i += T) {
     const int v = S[i];
-    ql[v] = (1.0f + 0.0f) / (float)ld_wg(&sg[v]);
+    ql[v] = (1.0f + 0.0f) / (float)ld_wg(sg + v);
   }
   __syncthreads();
   for (int l = L - 1; l >= 1; l--) {
     const int b0 = E[l], b1 = E[l + 1];
     for (int i = b0 + tid; i < b1; i += T) {
       const int v = S[i];
-      const float sv = (float)ld_wg(&sg[v]);
+      const float sv = (float)ld_wg(sg + v);
       float sum = 0.0f, comp = 0.0f;
       if (ELL) {  // the neighbours in one load, in adjacency order
         const int4 q4 = a.LE[v];
@@ -689,9 +696,9 @@ __global__ __launch_bounds__(256) void k_ps_bc_reduce(float *const *rows, double
     const size_t x = nof ? (size_t)nof[v] : v;
     double acc = 0.0;
     for (int b = 0; b < nb; b++) {
-      acc += (double)rows[b][x];
-      double *t = tgt[b];
-      if (b + 1 == nb || tgt[b + 1] != t) {
+      acc += (double)glob(rows[b])[x];
+      gp<double> t = glob(tgt[b]);
+      if (b + 1 == nb || glob(tgt[b + 1]) != t) {
         if (t) t[v] += acc;
         acc = 0.0;
       }
@@ -704,9 +711,9 @@ __global__ __launch_bounds__(256) void k_ps_bc_reduce(float *const *rows, double
 // blockIdx.y selects the row.
 template <class T>
 __global__ __launch_bounds__(256) void k_ps_unpermute(T *const *src, T *const *dst, const int *nof, int n) {
-  T *d = dst[blockIdx.y];
+  gp<T> d = glob(dst[blockIdx.y]);
   if (!d) return;
-  const T *s = src[blockIdx.y];
+  gp<const T> s = glob((const T *)src[blockIdx.y]);
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < (size_t)n; v += stride) d[v] = s[nof[v]];
 }
@@ -732,6 +739,7 @@ constexpr int kLdsBucketBits = 11;
 constexpr int kLdsBuckets = 1 << kLdsBucketBits;  // 4-slot buckets per level set (8192 slots)
 constexpr int kLdsProbe = 16;                     // buckets probed before giving up
 constexpr int kLdsFront = 3072;                   // LDS frontier entries per buffer
+constexpr int kLdsSlots = 3;                      // frontier vertices a thread loads at once
 // Facts about the graph that the host measured when it built the handle.
 // Bipartite: BFS levels are independent sets, so a neighbour of a level-L
 // vertex is never at level L and the same-level lookup is skipped.
@@ -753,6 +761,26 @@ __device__ __forceinline__ int4 le8_decode(u64 p, int v) {
 // The neighbours of x (x >= 0) from whichever table the graph has.
 __device__ __forceinline__ int4 le_load(const int4 *LE, const u64 *LE8, int x) {
   return LE8 ? le8_decode(LE8[x], x) : LE[x];
+}
+// The same in two steps, so that a load issued ahead is not waited for
+// until its value is used: the table entry of x (entry 0 for x < 0, which
+// decodes to no neighbours), then its neighbours.  The table is a template
+// argument (E8: the int16-delta table), so no run-time branch between the
+// two loads sits in front of the wait-count pass.
+template <bool E8>
+__device__ __forceinline__ int4 le_fetch(const int4 *LE, const u64 *LE8, int x) {
+  const int y = x >= 0 ? x : 0;
+  if (E8) {
+    const int2 t = reinterpret_cast<const int2 *>(LE8)[y];
+    return make_int4(t.x, t.y, 0, 0);
+  }
+  return LE[y];
+}
+template <bool E8>
+__device__ __forceinline__ int4 le_decode(int x, const int4 r) {
+  if (x < 0) return make_int4(-1, -1, -1, -1);
+  if (E8) return le8_decode(((u64)(unsigned)r.y << 32) | (unsigned)r.x, x);
+  return r;
 }
 constexpr int kLdsBipartite = 1;
 constexpr int kLdsConnected = 2;
@@ -822,10 +850,11 @@ __device__ __forceinline__ int lds_insert(int4 *t, int w) {
 // neighbours q (loaded by the caller).  (Issuing the parents' path-count
 // loads ahead of the claims, or expanding two vertices per thread, measured
 // slower: 61.4 and 80.4 against 56.4 ms per 2048^2 pass.)
-template <bool SIGMA, bool WANT_D, class DT>
-__device__ __forceinline__ void ps_visit_lds(const int4 q, DT *d, u64 *sg, int v, int L, const int4 *tp,
+template <bool SIGMA, bool WANT_D, class DT, bool STORE = true>
+__device__ __forceinline__ void ps_visit_lds(const int4 q, gp<DT> d, gp<u64> sg, int v, int L, const int4 *tp,
                                              const int4 *tl, int4 *tn, int &ovf, int &bad, u64 &dag, u64 &teps,
-                                             unsigned *cnt, PsLds &f, int *spill, int *S, int nbuf, bool bip) {
+                                             unsigned *cnt, PsLds &f, gp<int> spill, gp<int> S, int nbuf,
+                                             bool bip, u64 *sv_out = nullptr) {
   const int w[4] = {q.x, q.y, q.z, q.w};
   bool par[4], got[4];
 #pragma unroll
@@ -873,9 +902,10 @@ __device__ __forceinline__ void ps_visit_lds(const int4 q, DT *d, u64 *sg, int v
         dag++;
       }
     }
-    if (L >= 1 && v >= 0) sg[v] = acc;
+    if (STORE && L >= 1 && v >= 0) sg[v] = acc;
+    if (!STORE) *sv_out = acc;
   }
-  if (WANT_D && v >= 0) d[v] = (DT)L;
+  if (STORE && WANT_D && v >= 0) d[v] = (DT)L;
 }
 
 // The level loop over LDS level sets (one workgroup barrier a level).  With
@@ -884,10 +914,10 @@ __device__ __forceinline__ void ps_visit_lds(const int4 q, DT *d, u64 *sg, int v
 // the two `spill` halves.  Returns the deepest level, or -1 when a level set
 // overflowed (or, with 16-bit distances, the search went deeper than
 // kD16Max levels).
-template <bool SIGMA, bool BC, bool WANT_D, class DT>
-__device__ int ps_levels_lds(const int4 *LE, const u64 *LE8, int n, int s, DT *d, u64 *sg, int *spill0, int *S,
-                             int *E, PsLds &f, unsigned *s_cnt, int *s_bad, int &ovf, u64 &teps, u64 &reached,
-                             u64 &dag, bool bip) {
+template <bool SIGMA, bool BC, bool WANT_D, class DT, bool E8>
+__device__ __forceinline__ int ps_levels_lds(const int4 *LE, const u64 *LE8, int n, int s, gp<DT> d, gp<u64> sg, gp<int> spill0,
+                             gp<int> S, gp<int> E, PsLds &f, unsigned *s_cnt, int *s_bad, int &ovf, u64 &teps,
+                             u64 &reached, u64 &dag, bool bip) {
   const int tid = threadIdx.x, T = blockDim.x;
   for (int i = tid; i < 4 * kLdsBuckets; i += T) (&f.tab[0][0])[i] = make_int4(-1, -1, -1, -1);
   if (tid < 3) s_cnt[tid] = 0;
@@ -913,24 +943,58 @@ __device__ int ps_levels_lds(const int4 *LE, const u64 *LE8, int n, int s, DT *d
     const int4 *tp = f.tab[(L + 3) & 3], *tl = f.tab[L & 3];
     int4 *tn = f.tab[(L + 1) & 3], *tc = f.tab[(L + 2) & 3];
     for (int i = tid; i < kLdsBuckets; i += T) tc[i] = make_int4(-1, -1, -1, -1);  // held level L-2
-    int *Snext = BC ? S + qs : nullptr;
-    const int *sp_cur = BC ? S + qb + kLdsFront : spill0 + (size_t)b * n;
-    int *sp_next = BC ? S + qs + kLdsFront : spill0 + (size_t)(b ^ 1) * n;
-    // each thread's next frontier vertex and its neighbours are loaded
-    // before the current one is expanded, so that load overlaps the
-    // current lookups and path-count loads (62.5 -> 56.4 ms per 2048^2 pass)
-    auto entry = [&](unsigned k) -> int {
-      return k >= qn ? -1 : (k < (unsigned)kLdsFront ? f.fv[b][k] : sp_cur[k - kLdsFront]);
-    };
-    int v = entry((unsigned)tid);
-    int4 q = v >= 0 ? le_load(LE, LE8, v) : make_int4(-1, -1, -1, -1);
-    for (unsigned base = 0; base < qn; base += T) {
-      const int vn = entry(base + T + tid);
-      const int4 qn4 = vn >= 0 ? le_load(LE, LE8, vn) : make_int4(-1, -1, -1, -1);
-      ps_visit_lds<SIGMA, WANT_D || BC>(q, d, sg, v, L, tp, tl, tn, ovf, bad, dag, teps, cnt, f, sp_next, Snext,
+    gp<int> Snext = BC ? S + qs : nullptr;
+    gp<const int> sp_cur = BC ? S + qb + kLdsFront : spill0 + (size_t)b * n;
+    gp<int> sp_next = BC ? S + qs + kLdsFront : spill0 + (size_t)(b ^ 1) * n;
+    // The frontier entries held in LDS first: each thread's next vertex
+    // and its table entry are loaded before the current one is expanded,
+    // and decoded only after it, so that load overlaps the current lookups
+    // and path-count loads (62.5 -> 56.4 ms per 2048^2 pass; decoding it at
+    // once made the compiler wait for the load where it was issued).  No
+    // global load feeds a frontier entry in this loop: an entry that might
+    // come from HBM made every iteration wait for all of the previous one's
+    // global stores (one in-order counter covers loads and stores).
+    const unsigned qlds = min(qn, (unsigned)kLdsFront);
+    for (unsigned base = 0; base < qlds; base += kLdsSlots * T) {
+      // up to kLdsSlots frontier vertices per thread: their table entries
+      // are all loaded first, then each vertex is expanded, and their path
+      // counts and distances are stored after the last expansion.  On this
+      // hardware one in-order counter covers loads and stores, so a load
+      // waited for after a store also waits for the store: stores between
+      // the expansions made each expansion's path-count loads wait for the
+      // previous one's stores.
+      int vv[kLdsSlots];
+      int4 raw[kLdsSlots];
+      u64 sv[kLdsSlots];
+#pragma unroll
+      for (int i = 0; i < kLdsSlots; i++) {
+        const unsigned k = base + i * T + tid;
+        vv[i] = k < qlds ? f.fv[b][k] : -1;
+      }
+#pragma unroll
+      for (int i = 0; i < kLdsSlots; i++) raw[i] = le_fetch<E8>(LE, LE8, vv[i]);
+#pragma unroll
+      for (int i = 0; i < kLdsSlots; i++) {
+        sv[i] = 0;
+        if (base + i * T < qlds)  // uniform
+          ps_visit_lds<SIGMA, WANT_D || BC, DT, false>(le_decode<E8>(vv[i], raw[i]), d, sg, vv[i], L, tp, tl, tn,
+                                                        ovf, bad, dag, teps, cnt, f, sp_next, Snext, b ^ 1, bip,
+                                                        &sv[i]);
+      }
+#pragma unroll
+      for (int i = 0; i < kLdsSlots; i++) {
+        if (vv[i] < 0) continue;
+        if (SIGMA && L >= 1) sg[vv[i]] = sv[i];
+        if (WANT_D || BC) d[vv[i]] = (DT)L;
+      }
+    }
+    // then the entries beyond the LDS frontier (levels larger than it)
+    for (unsigned base = qlds; base < qn; base += T) {
+      const unsigned k = base + tid;
+      const int vs = k < qn ? sp_cur[k - kLdsFront] : -1;
+      const int4 q = le_decode<E8>(vs, le_fetch<E8>(LE, LE8, vs));
+      ps_visit_lds<SIGMA, WANT_D || BC>(q, d, sg, vs, L, tp, tl, tn, ovf, bad, dag, teps, cnt, f, sp_next, Snext,
                                         b ^ 1, bip);
-      v = vn;
-      q = qn4;
     }
     if (bad) *s_bad = 1;
     __syncthreads();
@@ -972,7 +1036,7 @@ struct PsLdsArgs {
 // On a connected graph every vertex's distance is written by the search, so
 // the fill is skipped.  A source deeper than kD16Max levels stops as a
 // level-set overflow does (the host reruns the launch).
-template <bool SIGMA, bool WANT_D>
+template <bool SIGMA, bool WANT_D, bool E8>
 __global__ __launch_bounds__(1024) void k_ps_lds(PsLdsArgs a) {
   __shared__ PsLds f;
   __shared__ unsigned s_cnt[3];
@@ -980,20 +1044,21 @@ __global__ __launch_bounds__(1024) void k_ps_lds(PsLdsArgs a) {
   __shared__ int s_ovf, s_bad, s_job;
   const int tid = threadIdx.x, T = blockDim.x;
   const size_t slot = blockIdx.x, n = (size_t)a.n;
-  int *spill = a.queues + slot * (2 * n + 2);
+  gp<int> spill = glob(a.queues + slot * (2 * n + 2));
   for (int j = ps_next_job(a.q, &s_job, true); j < a.q.nsrc; j = ps_next_job(a.q, &s_job, false)) {
     const int s = a.srcs[j];
-    unsigned short *d = nullptr;
+    gp<unsigned short> d = nullptr;
     if (WANT_D)
-      d = reinterpret_cast<unsigned short *>(a.dist && a.dist[j] ? a.dist[j] : a.q.dscr + slot * n);
-    u64 *sg = SIGMA ? (a.sigma[j] ? a.sigma[j] : a.q.sscr + slot * n) : nullptr;
+      d = glob(reinterpret_cast<unsigned short *>(a.dist && a.dist[j] ? a.dist[j] : a.q.dscr + slot * n));
+    gp<u64> sg = glob(SIGMA ? (a.sigma[j] ? a.sigma[j] : a.q.sscr + slot * n) : nullptr);
     if (WANT_D && !(a.flags & kLdsConnected))
       for (int i = tid; i < a.n; i += T) d[i] = kD16None;
     if (tid == 0) s_ovf = 0;
     __syncthreads();
     int ovf = 0;
     u64 teps = 0, reached = 1, dag = 0;
-    const int L = ps_levels_lds<SIGMA, false, WANT_D>(a.LE, a.LE8, a.n, s, d, sg, spill, nullptr, nullptr, f,
+    const int L = ps_levels_lds<SIGMA, false, WANT_D, unsigned short, E8>(a.LE, a.LE8, a.n, s, d, sg, spill,
+                                                                          nullptr, nullptr, f,
                                                       s_cnt, &s_bad, ovf, teps, reached, dag,
                                                       (a.flags & kLdsBipartite) != 0);
     ps_finish(teps, dag, reached, L < 0 ? 0 : L, ovf, &s_ovf, s_red, a.part, a.depth, a.ovf, j);
@@ -1009,6 +1074,7 @@ __global__ __launch_bounds__(1024) void k_ps_lds(PsLdsArgs a) {
 // (successors d = l+1; on an undirected graph every neighbour of a reached
 // vertex is reached, so only written distances are read).  The dependency
 // row is set to 0 first (the BC reduce and write-back read every vertex).
+template <bool E8>
 __global__ __launch_bounds__(1024) void k_ps_bc_lds(PsBcArgs a) {
   __shared__ PsLds f;
   __shared__ unsigned s_cnt[3];
@@ -1016,18 +1082,19 @@ __global__ __launch_bounds__(1024) void k_ps_bc_lds(PsBcArgs a) {
   __shared__ int s_ovf, s_bad;
   const int tid = threadIdx.x, T = blockDim.x;
   const int s = a.srcs[blockIdx.x];
-  int *d = a.dist[blockIdx.x];
-  u64 *sg = a.sigma[blockIdx.x];
-  float *dl = a.delta[blockIdx.x];
-  float *ql = a.quot[blockIdx.x];
-  int *S = a.stack + (size_t)blockIdx.x * (2 * (size_t)a.n + 2);
-  int *E = S + a.n;
+  gp<int> d = glob(a.dist[blockIdx.x]);
+  gp<u64> sg = glob(a.sigma[blockIdx.x]);
+  gp<float> dl = glob(a.delta[blockIdx.x]);
+  gp<float> ql = glob(a.quot[blockIdx.x]);
+  gp<int> S = glob(a.stack + (size_t)blockIdx.x * (2 * (size_t)a.n + 2));
+  gp<int> E = S + a.n;
   for (int i = tid; i < a.n; i += T) dl[i] = 0.0f;
   if (tid == 0) s_ovf = 0;
   __syncthreads();
   int ovf = 0;
   u64 teps = 0, reached = 1, dag = 0;
-  const int L = ps_levels_lds<true, true, true>(a.LE, a.LE8, a.n, s, d, sg, nullptr, S, E, f, s_cnt, &s_bad, ovf, teps,
+  const int L = ps_levels_lds<true, true, true, int, E8>(a.LE, a.LE8, a.n, s, d, sg, nullptr, S, E, f, s_cnt, &s_bad,
+                                                         ovf, teps,
                                                 reached, dag, (a.lds_flags & kLdsBipartite) != 0);
   __syncthreads();
   if (L < 0) {  // rerun by the host on k_ps_bc
@@ -1037,14 +1104,14 @@ __global__ __launch_bounds__(1024) void k_ps_bc_lds(PsBcArgs a) {
   }
   for (int i = E[L] + tid; L >= 1 && i < E[L + 1]; i += T) {
     const int v = S[i];
-    ql[v] = 1.0f / (float)ld_wg(&sg[v]);
+    ql[v] = 1.0f / (float)ld_wg(sg + v);
   }
   __syncthreads();
   for (int l = L - 1; l >= 1; l--) {
     for (int i = E[l] + tid; i < E[l + 1]; i += T) {
       const int v = S[i];
-      const float sv = (float)ld_wg(&sg[v]);
-      const int4 q4 = le_load(a.LE, a.LE8, v);
+      const float sv = (float)ld_wg(sg + v);
+      const int4 q4 = le_decode<E8>(v, le_fetch<E8>(a.LE, a.LE8, v));
       const int w[4] = {q4.x, q4.y, q4.z, q4.w};
       bool on[4];
       float qw[4];
@@ -1074,10 +1141,10 @@ __global__ __launch_bounds__(1024) void k_ps_bc_lds(PsBcArgs a) {
 // (and L2) instead of being fetched by every XCD.
 __global__ __launch_bounds__(256) void k_ps_unpermute_ds(int *const *dsrc, u64 *const *ssrc, int *const *ddst,
                                                          u64 *const *sdst, const int *nof, int n) {
-  int *dd = ddst ? ddst[blockIdx.y] : nullptr;
-  u64 *sd = sdst[blockIdx.y];
-  const unsigned short *ds = reinterpret_cast<const unsigned short *>(dsrc[blockIdx.y]);
-  const u64 *ss = ssrc[blockIdx.y];
+  gp<int> dd = glob(ddst ? ddst[blockIdx.y] : nullptr);
+  gp<u64> sd = glob(sdst[blockIdx.y]);
+  gp<const unsigned short> ds = glob(reinterpret_cast<const unsigned short *>(dsrc[blockIdx.y]));
+  gp<const u64> ss = glob((const u64 *)ssrc[blockIdx.y]);
   const unsigned G = gridDim.x, b = blockIdx.x;
   const unsigned c = (b % 8) * (G / 8) + b / 8;
   const size_t chunk = ((size_t)n + G - 1) / G;

@@ -28,6 +28,27 @@ __device__ __forceinline__ uint64_t readfirst64(uint64_t v) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+// The same pointer, declared global.  A row pointer read from a device
+// table (or chosen between such pointers) is generic to the compiler, which
+// then emits flat loads, stores and atomics; flat operations also count
+// against the LDS wait counter, so every wait for an LDS result would wait
+// for them too.  Every row, queue and stack of the engines lives in global
+// memory, and this cast lets the compiler emit global_* instructions.
+// Functions that take such pointers take them as gp<T> (a cast to a
+// generic pointer and back would be folded away and the knowledge lost).
+template <class T>
+using gp = __attribute__((address_space(1))) T *;
+template <class T>
+__device__ __forceinline__ gp<T> glob(T *p) {
+  return (gp<T>)p;
+}
+// Atomics on such pointers (the generic forms, whose addresses the
+// compiler then knows to be global).
+__device__ __forceinline__ int cas_g(gp<int> p, int cmp, int val) { return atomicCAS((int *)p, cmp, val); }
+__device__ __forceinline__ unsigned long long add_g(gp<unsigned long long> p, unsigned long long v) {
+  return atomicAdd((unsigned long long *)p, v);
+}
+
 // First of `k` consecutive positions for this lane in a list whose length
 // counter is *cnt, with one atomic per wave instead of one per lane (a
 // wave prefix sum of the lanes' k places them).  Every lane of a full wave
