@@ -697,6 +697,7 @@ def main():
                     help="grid workload: sources of the call timed as launches of 256 and as the "
                          "queue-fed grid")
     ap.add_argument("--no-queue", action="store_true", help="grid workload: skip that comparison")
+    ap.add_argument("--no-power", action="store_true", help="bfs workload: skip the power leg")
     ap.add_argument("--no-serve", action="store_true",
                     help="bfs workload: skip the serving stream reported beside the GTEPS figure")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (degree-profile defaults)")
@@ -774,8 +775,8 @@ def main():
             e2e_units += st.teps_units
         e2e_gteps = e2e_units / (time.perf_counter() - t2) / 1e9
 
-        power = power_leg(gu, lambda i: dg.multi_search_device(batches[args.warmup + i % args.steps],
-                                                               dist, sig, cfg).teps_units)
+        power = None if args.no_power else power_leg(
+            gu, lambda i: dg.multi_search_device(batches[args.warmup + i % args.steps], dist, sig, cfg).teps_units)
 
         variants, variants_td, td_levels = {}, {}, {}
         if not args.no_variants:

@@ -1278,9 +1278,14 @@ int ensure_ws(gu_graph *g, int NW, bool stack, size_t want_S, hipStream_t st) {
       (void)hipFree(q2);
       return fail(e);
     }
-    GU_HIP(hipMemcpyAsync(q2, w.q, w.S_cap * 4, hipMemcpyDeviceToDevice, st));
-    GU_HIP(hipMemcpyAsync(m2, w.smask, w.S_cap * w.NW * 8, hipMemcpyDeviceToDevice, st));
-    GU_HIP(hipStreamSynchronize(st));
+    hipError_t e = hipMemcpyAsync(q2, w.q, w.S_cap * 4, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(m2, w.smask, w.S_cap * w.NW * 8, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {  // the old stack stays in place
+      (void)hipFree(q2);
+      (void)hipFree(m2);
+      return fail(e);
+    }
     (void)hipFree(w.q);
     (void)hipFree(w.smask);
     clear_level_graphs(g);
