@@ -1149,8 +1149,31 @@ __global__ __launch_bounds__(256) void k_ps_unpermute_ds(int *const *dsrc, u64 *
   const unsigned c = (b % 8) * (G / 8) + b / 8;
   const size_t chunk = ((size_t)n + G - 1) / G;
   const size_t lo = (size_t)c * chunk, hi = min((size_t)n, lo + chunk);
-  for (size_t v = lo + threadIdx.x; v < hi; v += blockDim.x) {
-    const int x = nof[v];
+  gp<const int> nf = glob(nof);
+  // four vertices per thread with every load issued before the first
+  // store (the map entries, then the gathers): one wait per four
+  constexpr int U = 4;
+  const size_t T = blockDim.x;
+  size_t v0 = lo + threadIdx.x;
+  for (; v0 + (U - 1) * T < hi; v0 += U * T) {
+    int x[U];
+    unsigned short dv[U];
+    u64 sv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) x[u] = nf[v0 + u * T];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      dv[u] = ds[x[u]];
+      sv[u] = ss[x[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (dd) dd[v0 + u * T] = dv[u] != kD16None ? (int)dv[u] : INT_MAX;
+      sd[v0 + u * T] = dv[u] != kD16None ? sv[u] : 0ull;
+    }
+  }
+  for (size_t v = v0; v < hi; v += T) {
+    const int x = nf[v];
     const unsigned short dv = ds[x];
     const u64 sv = ss[x];
     if (dd) dd[v] = dv != kD16None ? (int)dv : INT_MAX;
