@@ -1585,6 +1585,24 @@ struct HostRows {
 int run_persource(gu_graph *g, const int *sources, int nsrc, const HostRows &hr, bool sigma,
                   gu_stats &s, int &dmax, hipStream_t st, int queue = 0, bool allow_lds = true);
 
+// The level-set search for what the call wants: path counts (and distance
+// rows), distance rows only, or eccentricities only; e8: the graph has the
+// int16-delta neighbour table.
+void launch_ps_lds(const PsLdsArgs &c, bool sigma, bool rows, bool e8, int grid, hipStream_t st) {
+  if (sigma && e8)
+    k_ps_lds<true, true, true><<<grid, 1024, 0, st>>>(c);
+  else if (sigma)
+    k_ps_lds<true, true, false><<<grid, 1024, 0, st>>>(c);
+  else if (rows && e8)
+    k_ps_lds<false, true, true><<<grid, 1024, 0, st>>>(c);
+  else if (rows)
+    k_ps_lds<false, true, false><<<grid, 1024, 0, st>>>(c);
+  else if (e8)
+    k_ps_lds<false, false, true><<<grid, 1024, 0, st>>>(c);
+  else
+    k_ps_lds<false, false, false><<<grid, 1024, 0, st>>>(c);
+}
+
 int ensure_jobs(gu_graph *g, int nsrc) {
   PsWorkspace &p = g->pws;
   if (p.jcap >= nsrc) return GU_OK;
@@ -1651,17 +1669,14 @@ int run_persource_queue(gu_graph *g, const int *sources, int nsrc, const HostRow
   u64 *const *stab = sigma ? (u64 *const *)(p.jrows + (size_t)nsrc) : nullptr;
   const int grid = std::min(nsrc, kPsCap);
   const int4 *LE = rl ? g->LE : nullptr;
-  const bool lds = LE && !g->directed && sigma;
+  const bool lds = LE && !g->directed;
   {
     GuTraceRange tr("gu:per-source queue");
     if (lds) {
       PsLdsArgs c{LE, g->n, p.jsrcs, (int *const *)dtab, stab, p.queues, p.jdepth, p.jpart, p.ovf, g->lds_flags,
                   g->LE8};
       c.q = q;
-      if (g->LE8)
-        k_ps_lds<true, true, true><<<grid, 1024, 0, st>>>(c);
-      else
-        k_ps_lds<true, true, false><<<grid, 1024, 0, st>>>(c);
+      launch_ps_lds(c, sigma, hr.dist != nullptr || sigma, g->LE8 != nullptr, grid, st);
     } else {
       PsArgs a{rl ? g->LR : g->R, rl ? g->LC : g->C, LE, g->n, p.jsrcs, (int *const *)dtab, stab, p.queues,
                p.jdepth, p.jpart, p.ovf};
@@ -1792,7 +1807,7 @@ int run_persource(gu_graph *g, const int *sources, int nsrc, const HostRows &hr,
     // overflowed runs again on the distance-reading kernel.  Searches
     // without path counts stay on that kernel: 64-source diameter sampling
     // on a 2048^2 grid took 34.4 ms there against 39.9 ms over level sets.
-    bool lds = allow_lds && LE && !g->directed && sigma;
+    bool lds = allow_lds && LE && !g->directed;
     const bool rows = hr.dist != nullptr || sigma;
     if (lds) {
       PsLdsArgs c{LE,      g->n,   p.srcs, rows ? (int *const *)g->rowtab : nullptr,
@@ -1800,10 +1815,7 @@ int run_persource(gu_graph *g, const int *sources, int nsrc, const HostRows &hr,
                   p.queues, p.depth, p.part, p.ovf, g->lds_flags, g->LE8};
       c.q.nsrc = nb;
       GuTraceRange tr("gu:per-source level sets");
-      if (g->LE8)
-        k_ps_lds<true, true, true><<<nb, 1024, 0, st>>>(c);
-      else
-        k_ps_lds<true, true, false><<<nb, 1024, 0, st>>>(c);
+      launch_ps_lds(c, sigma, rows, g->LE8 != nullptr, nb, st);
       GU_HIP(hipGetLastError());
       if ((rc = fetch())) return rc;
       for (int i = 0; i < nb; i++) lds &= (part[4 * i + 3] & 2) == 0;
@@ -1831,14 +1843,15 @@ int run_persource(gu_graph *g, const int *sources, int nsrc, const HostRows &hr,
     if (rl) {
       GuTraceRange tr("gu:per-source write-back");
       const dim3 ug((unsigned)std::max(1, std::min(pick_grid(g, 1), (g->n + 255) / 256)), (unsigned)nb);
-      if (sigma && lds) {  // path counts written for reached vertices only; both rows in one pass
+      if (lds && rows) {  // 16-bit distances; path counts written for reached vertices only
         // chunks of about 1024 vertices, a multiple of 8 per row
         const unsigned wg = (unsigned)std::max<long long>(8, ((g->n + 1023) / 1024 + 7) / 8 * 8);
         k_ps_unpermute_ds<<<dim3(wg, (unsigned)nb), 256, 0, st>>>(
-            (int *const *)g->rowtab, (u64 *const *)(g->rowtab + 256),
-            hr.dist ? (int *const *)(g->rowtab + 1024) : nullptr, (u64 *const *)(g->rowtab + 1280), g->nof, g->n);
+            (int *const *)g->rowtab, sigma ? (u64 *const *)(g->rowtab + 256) : nullptr,
+            hr.dist ? (int *const *)(g->rowtab + 1024) : nullptr,
+            sigma ? (u64 *const *)(g->rowtab + 1280) : nullptr, g->nof, g->n);
       }
-      if (hr.dist && !(sigma && lds))
+      if (hr.dist && !lds)
         k_ps_unpermute<int><<<ug, 256, 0, st>>>((int *const *)g->rowtab, (int *const *)(g->rowtab + 1024),
                                                 g->nof, g->n);
       if (sigma && !lds)

@@ -1132,7 +1132,8 @@ __global__ __launch_bounds__(1024) void k_ps_bc_lds(PsBcArgs a) {
 // Both rows of an LDS level-set search back to the caller's order in one
 // pass: dd[b][v] = dist (INT_MAX where the 16-bit row says not reached),
 // sd[b][v] = path count where reached, else 0; the distance destination
-// may be null.
+// may be null, and so may the path counts (ssrc and sdst both null: a
+// search for distances only).
 // blockIdx.y selects the row.  XCD-aware: blocks are dispatched to the 8
 // XCDs round-robin, so block b takes contiguous chunk (b % 8) * (G / 8) +
 // b / 8 of the row's G chunks (G = gridDim.x, a multiple of 8) and each
@@ -1142,9 +1143,9 @@ __global__ __launch_bounds__(1024) void k_ps_bc_lds(PsBcArgs a) {
 __global__ __launch_bounds__(256) void k_ps_unpermute_ds(int *const *dsrc, u64 *const *ssrc, int *const *ddst,
                                                          u64 *const *sdst, const int *nof, int n) {
   gp<int> dd = glob(ddst ? ddst[blockIdx.y] : nullptr);
-  gp<u64> sd = glob(sdst[blockIdx.y]);
+  gp<u64> sd = glob(sdst ? sdst[blockIdx.y] : nullptr);
   gp<const unsigned short> ds = glob(reinterpret_cast<const unsigned short *>(dsrc[blockIdx.y]));
-  gp<const u64> ss = glob((const u64 *)ssrc[blockIdx.y]);
+  gp<const u64> ss = glob(ssrc ? (const u64 *)ssrc[blockIdx.y] : nullptr);
   const unsigned G = gridDim.x, b = blockIdx.x;
   const unsigned c = (b % 8) * (G / 8) + b / 8;
   const size_t chunk = ((size_t)n + G - 1) / G;
@@ -1164,20 +1165,20 @@ __global__ __launch_bounds__(256) void k_ps_unpermute_ds(int *const *dsrc, u64 *
 #pragma unroll
     for (int u = 0; u < U; u++) {
       dv[u] = ds[x[u]];
-      sv[u] = ss[x[u]];
+      sv[u] = ss ? ss[x[u]] : 0ull;
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
       if (dd) dd[v0 + u * T] = dv[u] != kD16None ? (int)dv[u] : INT_MAX;
-      sd[v0 + u * T] = dv[u] != kD16None ? sv[u] : 0ull;
+      if (sd) sd[v0 + u * T] = dv[u] != kD16None ? sv[u] : 0ull;
     }
   }
   for (size_t v = v0; v < hi; v += T) {
     const int x = nf[v];
     const unsigned short dv = ds[x];
-    const u64 sv = ss[x];
+    const u64 sv = ss ? ss[x] : 0ull;
     if (dd) dd[v] = dv != kD16None ? (int)dv : INT_MAX;
-    sd[v] = dv != kD16None ? sv : 0ull;
+    if (sd) sd[v] = dv != kD16None ? sv : 0ull;
   }
 }
 

@@ -658,14 +658,39 @@ def test_persource_lds_graph_kinds(gu, orc, gpu, kind, facts):
     _check_bc(gu, orc, hg, hg.sample_sources(45, seed=96), 64, kind, engine=PS)
 
 
-def test_persource_path_deeper_than_16bit_levels(gu, orc, gpu):
+@pytest.mark.parametrize("sigma", [True, False])
+def test_persource_path_deeper_than_16bit_levels(gu, orc, gpu, sigma):
     """A path of 70000 vertices searched from an end: 69999 levels, past
     what the level-set search's 16-bit distance rows hold (65534), so the
-    launch is rerun on the distance kernel; rows and counts still match."""
+    launch is rerun on the distance kernel; rows and counts still match,
+    with path counts and for distances only."""
     n = 70000
     hg = _undirected_csr(gu, n, np.stack([np.arange(n - 1), np.arange(1, n)], 1))
-    st = _ps_compare(gu, orc, hg, np.array([0, n - 1, n // 2, 3], dtype=np.int32), "path70000")
+    st = _ps_compare(gu, orc, hg, np.array([0, n - 1, n // 2, 3], dtype=np.int32), "path70000", sigma=sigma)
     assert st.max_level == n - 1
+
+
+def test_persource_diameter_on_level_sets(gu, orc, gpu):
+    """Diameter sampling (eccentricities only) runs the level-set search
+    without rows; sources whose search overflows (the 70000-vertex path's
+    ends, deeper than 16-bit levels; a binary tree's 16384-vertex level) are
+    searched again on the distance kernel."""
+    n = 70000
+    path = _undirected_csr(gu, n, np.stack([np.arange(n - 1), np.arange(1, n)], 1))
+    tree = _kary_tree(gu, 2, 15)
+    grid = gu.HostGraph.grid(48, 0.10, seed=3)
+    cases = [(path, np.array([n // 2, 0, 17], dtype=np.int32), "path70000"),
+             (tree, np.array([0, 5, tree.n - 1], dtype=np.int32), "tree2-15"),
+             (grid, grid.sample_sources(40, seed=8), "grid48")]
+    for hg, src, label in cases:
+        announce("diameter on level sets %s nsrc=%d" % (label, len(src)))
+        with gu.DeviceGraph(hg) as dg:
+            diam, st = dg.diameter(src, gu.make_config(engine=PS))
+            ecc, _ = dg.multi_search_rows(src, None, None, want_ecc=True, config=gu.make_config(engine=PS))
+        assert st.engine_used == PS
+        expect = [orc.eccentricity(hg.R, hg.C, int(s)) for s in src]
+        assert list(ecc) == expect, label
+        assert diam == max(expect), label
 
 
 def test_persource_distances_only(gu, orc, gpu):
