@@ -2,8 +2,10 @@
 // over the HIP engine.  Each slot owns a HIP stream and, per registered
 // graph, a shared handle (gu_graph_share) with its own search workspace, so
 // launches of different slots overlap on the device while reading one
-// resident copy of the graph.  The output arena is one device allocation
-// made when the server is created; nothing is allocated per query.
+// resident copy of the graph.  A server with a host arena also has one
+// delivery stream, on which every device-to-host copy of results runs.  The
+// output arenas are allocated when the server is created; nothing is
+// allocated per query.
 #include <hip/hip_runtime.h>
 
 #include <new>
@@ -22,6 +24,7 @@ struct GpuExec final : gus::Exec {
   int slots = 0;
   gu_config base{};
   std::vector<hipStream_t> streams;
+  hipStream_t dstream = nullptr;  // the delivery thread's copies to the host
   gu_graph_t views[gus::kMaxGraphs][64] = {};
   int ngraphs = 0;
 
@@ -51,9 +54,15 @@ struct GpuExec final : gus::Exec {
     if (hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st) != hipSuccess) return GU_ERR_HIP;
     return hipStreamSynchronize(st) == hipSuccess ? GU_OK : GU_ERR_HIP;
   }
+  int deliver(void *dst, const void *src, size_t bytes) override {
+    if (hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, dstream) != hipSuccess) return GU_ERR_HIP;
+    return hipStreamSynchronize(dstream) == hipSuccess ? GU_OK : GU_ERR_HIP;
+  }
   void free_all() {
     for (hipStream_t x : streams) (void)hipStreamDestroy(x);
     streams.clear();
+    if (dstream) (void)hipStreamDestroy(dstream);
+    dstream = nullptr;
     if (arena) (void)hipFree(arena);
     if (host) (void)hipHostFree(host);
     arena = nullptr;
@@ -104,6 +113,12 @@ int gu_server_create(const gu_server_config *cfgp, gu_server_t *out) {
     s->ex.free_all();
     delete s;
     return GU_ERR_NOMEM;
+  }
+  if (c.host_budget > 0 && hipStreamCreateWithFlags(&s->ex.dstream, hipStreamNonBlocking) != hipSuccess) {
+    s->ex.dstream = nullptr;
+    s->ex.free_all();
+    delete s;
+    return GU_ERR_HIP;
   }
   s->ex.slots = slots;
   s->ex.base.variant = c.variant ? c.variant : GU_VARIANT_AUTO;
