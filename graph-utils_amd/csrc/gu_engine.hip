@@ -1035,10 +1035,11 @@ struct PsWorkspace {
   u64 *jpart = nullptr;     // [jcap][4]
   int *jdepth = nullptr;    // [jcap]
   int *head = nullptr;      // the queue's counter
+  u64 *slog = nullptr;      // [kPsCap][2 * kLdsLog] path-count logs of the level-set kernels
   size_t bytes = 0;
   void release() {
     void *ps[] = {queues, dscratch, sscratch, xscratch, qscratch, depth, ovf, srcs, part,
-                  jsrcs, jrows, jpart, jdepth, head};
+                  jsrcs, jrows, jpart, jdepth, head, slog};
     for (void *p : ps)
       if (p) (void)hipFree(p);
     *this = PsWorkspace();
@@ -1603,6 +1604,23 @@ void launch_ps_lds(const PsLdsArgs &c, bool sigma, bool rows, bool e8, int grid,
     k_ps_lds<false, false, false><<<grid, 1024, 0, st>>>(c);
 }
 
+// The level-set kernels' path-count logs, one pair of levels per workgroup
+// (32 MB).
+int ensure_slog(gu_graph *g) {
+  PsWorkspace &p = g->pws;
+  if (p.slog) return GU_OK;
+  GU_HIP(dalloc(&p.slog, (size_t)kPsCap * 2 * kLdsLog));
+  p.bytes += (size_t)kPsCap * 2 * kLdsLog * 8;
+  return GU_OK;
+}
+
+// Whether a search of g can run on the LDS level sets: an undirected graph
+// with the degree-4 neighbour table, whose ids fit the keyed sets when the
+// search counts paths.
+bool lds_ok(const gu_graph *g, const int4 *LE, bool sigma) {
+  return LE && !g->directed && (!sigma || g->n <= kLdsMaxN);
+}
+
 int ensure_jobs(gu_graph *g, int nsrc) {
   PsWorkspace &p = g->pws;
   if (p.jcap >= nsrc) return GU_OK;
@@ -1669,13 +1687,15 @@ int run_persource_queue(gu_graph *g, const int *sources, int nsrc, const HostRow
   u64 *const *stab = sigma ? (u64 *const *)(p.jrows + (size_t)nsrc) : nullptr;
   const int grid = std::min(nsrc, kPsCap);
   const int4 *LE = rl ? g->LE : nullptr;
-  const bool lds = LE && !g->directed;
+  const bool lds = lds_ok(g, LE, sigma);
+  if (lds && sigma && (rc = ensure_slog(g))) return rc;
   {
     GuTraceRange tr("gu:per-source queue");
     if (lds) {
       PsLdsArgs c{LE, g->n, p.jsrcs, (int *const *)dtab, stab, p.queues, p.jdepth, p.jpart, p.ovf, g->lds_flags,
                   g->LE8};
       c.q = q;
+      c.slog = p.slog;
       launch_ps_lds(c, sigma, hr.dist != nullptr || sigma, g->LE8 != nullptr, grid, st);
     } else {
       PsArgs a{rl ? g->LR : g->R, rl ? g->LC : g->C, LE, g->n, p.jsrcs, (int *const *)dtab, stab, p.queues,
@@ -1807,13 +1827,15 @@ int run_persource(gu_graph *g, const int *sources, int nsrc, const HostRows &hr,
     // overflowed runs again on the distance-reading kernel.  Searches
     // without path counts stay on that kernel: 64-source diameter sampling
     // on a 2048^2 grid took 34.4 ms there against 39.9 ms over level sets.
-    bool lds = allow_lds && LE && !g->directed;
+    bool lds = allow_lds && lds_ok(g, LE, sigma);
     const bool rows = hr.dist != nullptr || sigma;
+    if (lds && sigma && (rc = ensure_slog(g))) return rc;
     if (lds) {
       PsLdsArgs c{LE,      g->n,   p.srcs, rows ? (int *const *)g->rowtab : nullptr,
                   sigma ? (u64 *const *)(g->rowtab + 256) : nullptr,
                   p.queues, p.depth, p.part, p.ovf, g->lds_flags, g->LE8};
       c.q.nsrc = nb;
+      c.slog = p.slog;
       GuTraceRange tr("gu:per-source level sets");
       launch_ps_lds(c, sigma, rows, g->LE8 != nullptr, nb, st);
       GU_HIP(hipGetLastError());
@@ -1926,7 +1948,9 @@ int run_persource_bc(gu_graph *g, const int *sources, int nsrc, const HostRows &
                (float *const *)(g->rowtab + 1280), p.queues, p.depth, p.part, p.ovf, g->lds_flags, g->LE8};
     // undirected degree <= 4 graphs: the LDS level-set forward; a launch in
     // which a level set overflowed runs again on k_ps_bc
-    bool lds = LE && !g->directed;
+    bool lds = lds_ok(g, LE, true);
+    if (lds && (rc = ensure_slog(g))) return rc;
+    a.slog = p.slog;
     if (lds) {
       GuTraceRange tr("gu:per-source brandes level sets");
       if (g->LE8)

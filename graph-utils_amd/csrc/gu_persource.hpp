@@ -573,6 +573,7 @@ struct PsBcArgs {
   int *ovf;
   int lds_flags;        // kLds* graph facts for the level-set kernels
   const u64 *LE8;       // or null: the neighbour table as int16 deltas
+  u64 *slog;            // level-set kernels: [grid][2 * kLdsLog] path-count logs
 };
 
 __device__ __forceinline__ void ps_neumaier(float &sum, float &comp, float x) {
@@ -730,9 +731,13 @@ __global__ __launch_bounds__(256) void k_ps_unpermute(T *const *src, T *const *d
 // the set of L+1 with an LDS compare-and-swap (each vertex claimed once),
 // and clears the set that held L-2, which takes the claims of L+2 after
 // this level's barrier.  Per frontier vertex the only global reads are its
-// int4 of neighbours and its parents' path counts (written one level
-// before); distances and counts are stored once, when a vertex is
-// expanded.  A level set that cannot place a claim within kLdsProbe buckets
+// int4 of neighbours and its parents' path counts; distances and counts
+// are stored once, when a vertex is expanded.  The parents' counts are read
+// from the previous level's log: a vertex's count goes both to its row and
+// to entry k of its level's log (k: its index in the level's frontier),
+// and a level set keeps each vertex's k beside its key, so the reads of a
+// level fall in one compact block written one level before (8 bytes per
+// vertex of that level) rather than across the row.  A level set that cannot place a claim within kLdsProbe buckets
 // stops the source (flag 2 in part[3]) and the host reruns the launch on the
 // distance-reading kernels, so results never depend on LDS capacity.
 constexpr int kLdsBucketBits = 11;
@@ -740,6 +745,22 @@ constexpr int kLdsBuckets = 1 << kLdsBucketBits;  // 4-slot buckets per level se
 constexpr int kLdsProbe = 16;                     // buckets probed before giving up
 constexpr int kLdsFront = 3072;                   // LDS frontier entries per buffer
 constexpr int kLdsSlots = 3;                      // frontier vertices a thread loads at once
+// Slots.  Searches without path counts keep vertex ids in the slots (home
+// bucket: the top 11 bits of w * kLdsMul).  With path counts the sets are
+// keyed and hold vertex ids below 2^23: h = w * kLdsMul mod 2^23 is a
+// bijection on them (odd multiplier); its top 11 bits are w's home bucket,
+// and its low 12 bits r stay in the slot with the probe distance p (the
+// bucket's distance from home): key = r | p << 12.  Two ids with one key in
+// one bucket have the same h, so they are the same id.  A keyed slot holds
+// key | k << 16 (k < 8192, the vertex's index in its level's frontier, set
+// by the claim's winner once known), a value below 2^29; -1 marks an empty
+// slot, and kLdsKeyMask keeps the key bits and the bits set only there.
+constexpr int kLdsIdBits = 23;
+constexpr int kLdsMaxN = 1 << kLdsIdBits;  // graphs the keyed level sets can hold
+constexpr int kLdsRBits = kLdsIdBits - kLdsBucketBits;
+constexpr unsigned kLdsMul = 0x9E3779B1u;
+constexpr unsigned kLdsKeyMask = 0xE000FFFFu;
+constexpr int kLdsLog = 4 * kLdsBuckets;  // path-count log entries per level (one per slot)
 // Facts about the graph that the host measured when it built the handle.
 // Bipartite: BFS levels are independent sets, so a neighbour of a level-L
 // vertex is never at level L and the same-level lookup is skipped.
@@ -789,10 +810,10 @@ struct PsLds {
   int fv[2][kLdsFront];      // 24 KB; entries beyond go to HBM
 };
 
-__device__ __forceinline__ unsigned lds_bucket(int w) {
-  return ((unsigned)w * 0x9E3779B1u) >> (32 - kLdsBucketBits);
-}
+__device__ __forceinline__ unsigned lds_bucket(int w) { return ((unsigned)w * kLdsMul) >> (32 - kLdsBucketBits); }
 __device__ __forceinline__ bool q4_has(const int4 q, int w) { return q.x == w || q.y == w || q.z == w || q.w == w; }
+__device__ __forceinline__ unsigned lds_hash(int w) { return ((unsigned)w * kLdsMul) & ((1u << kLdsIdBits) - 1); }
+__device__ __forceinline__ bool lds_match(int x, unsigned key) { return ((unsigned)x & kLdsKeyMask) == key; }
 __device__ __forceinline__ bool q4_open(const int4 q) { return q.x < 0 || q.y < 0 || q.z < 0 || q.w < 0; }
 
 // Buckets are probed linearly, one 16-byte LDS read each, and fill from
@@ -800,48 +821,93 @@ __device__ __forceinline__ bool q4_open(const int4 q) { return q.x < 0 || q.y < 
 // is full, and nothing leaves a set during a level, so a bucket with a free
 // slot ends the search.  At most kLdsProbe buckets (an insert gives up
 // beyond).  With levels of up to 4096 vertices (load <= 1/2) almost every
-// lookup reads one bucket, so the lanes of a wave stay in step.
-__device__ __forceinline__ bool lds_has(const int4 *t, int w) {
-  unsigned b = lds_bucket(w);
-  for (int p = 0; p < kLdsProbe; p++) {
+// lookup reads one bucket, so the lanes of a wave stay in step.  Returns
+// w's frontier index (KEYED; else 0) when w is in the set, else -1.
+template <bool KEYED>
+__device__ __forceinline__ int lds_find(const int4 *t, int w) {
+  if (!KEYED) {
+    unsigned b = lds_bucket(w);
+    for (int p = 0; p < kLdsProbe; p++) {
+      const int4 q = t[b];
+      if (q4_has(q, w)) return 0;
+      if (q4_open(q)) return -1;
+      b = (b + 1) & (kLdsBuckets - 1);
+    }
+    return -1;
+  }
+  const unsigned h = lds_hash(w);
+  const unsigned r = h & ((1u << kLdsRBits) - 1);
+  unsigned b = h >> kLdsRBits;
+  // the home bucket, where almost every lookup ends, then the others in a
+  // rolled loop: sixteen unrolled probes per keyed lookup made the
+  // expansion loop too large to unroll (a rolled loop from the home bucket
+  // on measured 1% slower)
+  {
     const int4 q = t[b];
-    if (q4_has(q, w)) return true;
-    if (q4_open(q)) return false;
+    if (lds_match(q.x, r)) return q.x >> 16;
+    if (lds_match(q.y, r)) return q.y >> 16;
+    if (lds_match(q.z, r)) return q.z >> 16;
+    if (lds_match(q.w, r)) return q.w >> 16;
+    if (q4_open(q)) return -1;
     b = (b + 1) & (kLdsBuckets - 1);
   }
-  return false;
+#pragma unroll 1
+  for (int p = 1; p < kLdsProbe; p++) {
+    const int4 q = t[b];
+    const unsigned key = r | (unsigned)p << 12;
+    if (lds_match(q.x, key)) return q.x >> 16;
+    if (lds_match(q.y, key)) return q.y >> 16;
+    if (lds_match(q.z, key)) return q.z >> 16;
+    if (lds_match(q.w, key)) return q.w >> 16;
+    if (q4_open(q)) return -1;
+    b = (b + 1) & (kLdsBuckets - 1);
+  }
+  return -1;
 }
 
-// Inserts w: 1 when this call placed it, 0 when it was there, -1 when no
+// Inserts w (frontier index 0 for now): 1 when this call placed it, with
+// its slot in `at` (an int index into t), 0 when it was there, -1 when no
 // slot within kLdsProbe buckets was free.  The home bucket's slots are
 // claimed in order by compare-and-swap with no read first (at load <= 1/2
 // the first swap usually settles it); its swaps keep the fill order, since
 // slot k is tried only after slots 0..k-1 were seen taken.  Further buckets
 // are read first; a lost race on a slot re-reads the bucket.
-__device__ __forceinline__ int lds_insert(int4 *t, int w) {
-  unsigned b = lds_bucket(w);
+template <bool KEYED>
+__device__ __forceinline__ int lds_insert(int4 *t, int w, int &at) {
+  const unsigned h = lds_hash(w);
+  const unsigned r = h & ((1u << kLdsRBits) - 1);
+  unsigned b = KEYED ? h >> kLdsRBits : lds_bucket(w);
+  // the slot value at probe distance p, and whether slot value x is w's
+  auto val = [&](int p) { return KEYED ? (int)(r | (unsigned)p << 12) : w; };
+  auto same = [&](int x, int p) { return KEYED ? lds_match(x, r | (unsigned)p << 12) : x == w; };
   {
     int *sl = reinterpret_cast<int *>(&t[b]);
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      const int x = atomicCAS(sl + k, -1, w);
-      if (x < 0) return 1;
-      if (x == w) return 0;
+      const int x = atomicCAS(sl + k, -1, val(0));
+      if (x < 0) {
+        at = 4 * (int)b + k;
+        return 1;
+      }
+      if (same(x, 0)) return 0;
     }
     b = (b + 1) & (kLdsBuckets - 1);
   }
   for (int p = 1, tries = 0; p < kLdsProbe && tries < 4 * kLdsProbe; tries++) {
     const int4 q = t[b];
-    if (q4_has(q, w)) return 0;
+    if (same(q.x, p) || same(q.y, p) || same(q.z, p) || same(q.w, p)) return 0;
     int k = q.x < 0 ? 0 : (q.y < 0 ? 1 : (q.z < 0 ? 2 : (q.w < 0 ? 3 : 4)));
     if (k == 4) {
       b = (b + 1) & (kLdsBuckets - 1);
       p++;
       continue;
     }
-    const int x = atomicCAS(reinterpret_cast<int *>(&t[b]) + k, -1, w);
-    if (x < 0) return 1;
-    if (x == w) return 0;
+    const int x = atomicCAS(reinterpret_cast<int *>(&t[b]) + k, -1, val(p));
+    if (x < 0) {
+      at = 4 * (int)b + k;
+      return 1;
+    }
+    if (same(x, p)) return 0;
   }
   return -1;
 }
@@ -851,30 +917,37 @@ __device__ __forceinline__ int lds_insert(int4 *t, int w) {
 // loads ahead of the claims, or expanding two vertices per thread, measured
 // slower: 61.4 and 80.4 against 56.4 ms per 2048^2 pass.)
 template <bool SIGMA, bool WANT_D, class DT, bool STORE = true>
-__device__ __forceinline__ void ps_visit_lds(const int4 q, gp<DT> d, gp<u64> sg, int v, int L, const int4 *tp,
-                                             const int4 *tl, int4 *tn, int &ovf, int &bad, u64 &dag, u64 &teps,
-                                             unsigned *cnt, PsLds &f, gp<int> spill, gp<int> S, int nbuf,
-                                             bool bip, u64 *sv_out = nullptr) {
+__device__ __forceinline__ void ps_visit_lds(const int4 q, gp<DT> d, gp<u64> sg, gp<const u64> lp, gp<u64> lc, int k,
+                                             int v, int L, const int4 *tp, const int4 *tl, int4 *tn, int &ovf,
+                                             int &bad, u64 &dag, u64 &teps, unsigned *cnt, PsLds &f, gp<int> spill,
+                                             gp<int> S, int nbuf, bool bip, u64 *sv_out = nullptr) {
   const int w[4] = {q.x, q.y, q.z, q.w};
   bool par[4], got[4];
+  int pk[4], at[4];
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     par[j] = got[j] = false;
+    pk[j] = 0;
+    at[j] = 0;
     if (w[j] < 0) continue;
     teps++;
-    if (L >= 1 && lds_has(tp, w[j])) {
+    const int kp = L >= 1 ? lds_find<SIGMA>(tp, w[j]) : -1;
+    if (kp >= 0) {
       par[j] = true;
-    } else if (bip || !lds_has(tl, w[j])) {
-      const int r = lds_insert(tn, w[j]);
+      pk[j] = kp;
+    } else if (bip || lds_find<SIGMA>(tl, w[j]) < 0) {
+      const int r = lds_insert<SIGMA>(tn, w[j], at[j]);
       got[j] = r > 0;
       bad |= r < 0;
     }
   }
+  // the parents' path counts from the previous level's log (every lane
+  // loads, entry 0 off the parents)
   u64 x[4];
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     if (SIGMA) {
-      const u64 t = ld_wg(sg + (par[j] ? w[j] : 0));
+      const u64 t = ld_wg(lp + pk[j]);
       x[j] = par[j] ? t : 0ull;
     }
   }
@@ -885,6 +958,8 @@ __device__ __forceinline__ void ps_visit_lds(const int4 q, gp<DT> d, gp<u64> sg,
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     if (!got[j]) continue;
+    // the claim's frontier index into its slot (the index bits are 0 until now)
+    if (SIGMA) atomicOr(reinterpret_cast<int *>(tn) + at[j], (int)(pos << 16));
     if (S) S[pos] = w[j];
     if (pos < (unsigned)kLdsFront)
       f.fv[nbuf][pos] = w[j];
@@ -902,7 +977,10 @@ __device__ __forceinline__ void ps_visit_lds(const int4 q, gp<DT> d, gp<u64> sg,
         dag++;
       }
     }
-    if (STORE && L >= 1 && v >= 0) sg[v] = acc;
+    if (STORE && L >= 1 && v >= 0) {
+      sg[v] = acc;
+      lc[k] = acc;
+    }
     if (!STORE) *sv_out = acc;
   }
   if (STORE && WANT_D && v >= 0) d[v] = (DT)L;
@@ -913,19 +991,28 @@ __device__ __forceinline__ void ps_visit_lds(const int4 q, gp<DT> d, gp<u64> sg,
 // entries beyond the LDS capacity are read back from S; otherwise they go to
 // the two `spill` halves.  Returns the deepest level, or -1 when a level set
 // overflowed (or, with 16-bit distances, the search went deeper than
-// kD16Max levels).
+// kD16Max levels).  With path counts, lg holds the path-count logs of two
+// levels (2 * kLdsLog entries; level L's at lg + (L & 1) * kLdsLog).
 template <bool SIGMA, bool BC, bool WANT_D, class DT, bool E8>
-__device__ __forceinline__ int ps_levels_lds(const int4 *LE, const u64 *LE8, int n, int s, gp<DT> d, gp<u64> sg, gp<int> spill0,
-                             gp<int> S, gp<int> E, PsLds &f, unsigned *s_cnt, int *s_bad, int &ovf, u64 &teps,
-                             u64 &reached, u64 &dag, bool bip) {
+__device__ __forceinline__ int ps_levels_lds(const int4 *LE, const u64 *LE8, int n, int s, gp<DT> d, gp<u64> sg,
+                                             gp<u64> lg, gp<int> spill0, gp<int> S, gp<int> E, PsLds &f,
+                                             unsigned *s_cnt, int *s_bad, int &ovf, u64 &teps, u64 &reached,
+                                             u64 &dag, bool bip) {
   const int tid = threadIdx.x, T = blockDim.x;
   for (int i = tid; i < 4 * kLdsBuckets; i += T) (&f.tab[0][0])[i] = make_int4(-1, -1, -1, -1);
   if (tid < 3) s_cnt[tid] = 0;
   if (tid == 0) *s_bad = 0;
   __syncthreads();
   if (tid == 0) {
-    f.tab[0][lds_bucket(s)].x = s;
-    if (SIGMA) sg[s] = 1;
+    const unsigned h = lds_hash(s);  // keyed: key r, probe distance 0, frontier index 0
+    if (SIGMA)
+      f.tab[0][h >> kLdsRBits].x = (int)(h & ((1u << kLdsRBits) - 1));
+    else
+      f.tab[0][lds_bucket(s)].x = s;
+    if (SIGMA) {
+      sg[s] = 1;
+      lg[0] = 1;
+    }
     f.fv[0][0] = s;
     if (BC) {
       S[0] = s;
@@ -942,6 +1029,8 @@ __device__ __forceinline__ int ps_levels_lds(const int4 *LE, const u64 *LE8, int
     if (tid == 0) s_cnt[(L + 1) % 3] = 0;
     const int4 *tp = f.tab[(L + 3) & 3], *tl = f.tab[L & 3];
     int4 *tn = f.tab[(L + 1) & 3], *tc = f.tab[(L + 2) & 3];
+    gp<const u64> lp = SIGMA ? lg + ((L + 1) & 1) * kLdsLog : nullptr;  // level L-1's log
+    gp<u64> lc = SIGMA ? lg + (L & 1) * kLdsLog : nullptr;
     for (int i = tid; i < kLdsBuckets; i += T) tc[i] = make_int4(-1, -1, -1, -1);  // held level L-2
     gp<int> Snext = BC ? S + qs : nullptr;
     gp<const int> sp_cur = BC ? S + qb + kLdsFront : spill0 + (size_t)b * n;
@@ -977,14 +1066,17 @@ __device__ __forceinline__ int ps_levels_lds(const int4 *LE, const u64 *LE8, int
       for (int i = 0; i < kLdsSlots; i++) {
         sv[i] = 0;
         if (base + i * T < qlds)  // uniform
-          ps_visit_lds<SIGMA, WANT_D || BC, DT, false>(le_decode<E8>(vv[i], raw[i]), d, sg, vv[i], L, tp, tl, tn,
-                                                        ovf, bad, dag, teps, cnt, f, sp_next, Snext, b ^ 1, bip,
-                                                        &sv[i]);
+          ps_visit_lds<SIGMA, WANT_D || BC, DT, false>(le_decode<E8>(vv[i], raw[i]), d, sg, lp, lc, 0, vv[i], L,
+                                                        tp, tl, tn, ovf, bad, dag, teps, cnt, f, sp_next, Snext,
+                                                        b ^ 1, bip, &sv[i]);
       }
 #pragma unroll
       for (int i = 0; i < kLdsSlots; i++) {
         if (vv[i] < 0) continue;
-        if (SIGMA && L >= 1) sg[vv[i]] = sv[i];
+        if (SIGMA && L >= 1) {
+          sg[vv[i]] = sv[i];
+          lc[base + i * T + tid] = sv[i];
+        }
         if (WANT_D || BC) d[vv[i]] = (DT)L;
       }
     }
@@ -993,8 +1085,8 @@ __device__ __forceinline__ int ps_levels_lds(const int4 *LE, const u64 *LE8, int
       const unsigned k = base + tid;
       const int vs = k < qn ? sp_cur[k - kLdsFront] : -1;
       const int4 q = le_decode<E8>(vs, le_fetch<E8>(LE, LE8, vs));
-      ps_visit_lds<SIGMA, WANT_D || BC>(q, d, sg, vs, L, tp, tl, tn, ovf, bad, dag, teps, cnt, f, sp_next, Snext,
-                                        b ^ 1, bip);
+      ps_visit_lds<SIGMA, WANT_D || BC>(q, d, sg, lp, lc, (int)k, vs, L, tp, tl, tn, ovf, bad, dag, teps, cnt, f,
+                                        sp_next, Snext, b ^ 1, bip);
     }
     if (bad) *s_bad = 1;
     __syncthreads();
@@ -1027,6 +1119,7 @@ struct PsLdsArgs {
   int flags;          // kLds* graph facts
   const u64 *LE8;     // or null: the neighbour table as int16 deltas
   PsQueue q;          // source queue and in-kernel write-back
+  u64 *slog;          // with path counts: [grid][2 * kLdsLog] path-count logs
 };
 
 // The search: with rows (WANT_D) the distance row, in 16 bits (kD16None:
@@ -1051,13 +1144,14 @@ __global__ __launch_bounds__(1024) void k_ps_lds(PsLdsArgs a) {
     if (WANT_D)
       d = glob(reinterpret_cast<unsigned short *>(a.dist && a.dist[j] ? a.dist[j] : a.q.dscr + slot * n));
     gp<u64> sg = glob(SIGMA ? (a.sigma[j] ? a.sigma[j] : a.q.sscr + slot * n) : nullptr);
+    gp<u64> lg = glob(SIGMA ? a.slog + slot * (2 * kLdsLog) : nullptr);
     if (WANT_D && !(a.flags & kLdsConnected))
       for (int i = tid; i < a.n; i += T) d[i] = kD16None;
     if (tid == 0) s_ovf = 0;
     __syncthreads();
     int ovf = 0;
     u64 teps = 0, reached = 1, dag = 0;
-    const int L = ps_levels_lds<SIGMA, false, WANT_D, unsigned short, E8>(a.LE, a.LE8, a.n, s, d, sg, spill,
+    const int L = ps_levels_lds<SIGMA, false, WANT_D, unsigned short, E8>(a.LE, a.LE8, a.n, s, d, sg, lg, spill,
                                                                           nullptr, nullptr, f,
                                                       s_cnt, &s_bad, ovf, teps, reached, dag,
                                                       (a.flags & kLdsBipartite) != 0);
@@ -1093,8 +1187,9 @@ __global__ __launch_bounds__(1024) void k_ps_bc_lds(PsBcArgs a) {
   __syncthreads();
   int ovf = 0;
   u64 teps = 0, reached = 1, dag = 0;
-  const int L = ps_levels_lds<true, true, true, int, E8>(a.LE, a.LE8, a.n, s, d, sg, nullptr, S, E, f, s_cnt, &s_bad,
-                                                         ovf, teps,
+  gp<u64> lg = glob(a.slog + (size_t)blockIdx.x * (2 * kLdsLog));
+  const int L = ps_levels_lds<true, true, true, int, E8>(a.LE, a.LE8, a.n, s, d, sg, lg, nullptr, S, E, f, s_cnt,
+                                                         &s_bad, ovf, teps,
                                                 reached, dag, (a.lds_flags & kLdsBipartite) != 0);
   __syncthreads();
   if (L < 0) {  // rerun by the host on k_ps_bc
