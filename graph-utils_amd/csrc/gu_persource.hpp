@@ -63,39 +63,57 @@ __device__ __forceinline__ void ps_writeback(const PsQueue &q, int j, gp<const D
   gp<u64> sd = glob(q.sdst && ss ? q.sdst[j] : nullptr);
   gp<const int> nof = glob(q.nof);
   if (!dd && !sd) return;
-  // One workgroup moves a whole row pair, so each thread keeps U vertices'
-  // gathers in flight: the map entries, then every row gather, then the
-  // stores (the stores may alias nothing the loads read, which the
-  // compiler cannot know, so the order is written out).
+  // One workgroup moves a whole row pair, so the steps (U vertices a
+  // thread) are pipelined: step i+2's map entries and step i+1's row
+  // gathers are issued before step i's stores.  One in-order counter covers
+  // loads and stores on this hardware, so a load issued after a store is
+  // waited for only with that store; in this order no wait covers a store.
+  // Every load is issued (ids past the row read entry n-1), so the loop has
+  // no branch around its loads.
   constexpr int U = 8;
-  const int T = blockDim.x;
-  int v0 = threadIdx.x;
-  for (; v0 + (U - 1) * T < n; v0 += U * T) {
-    int x[U];
-    DT dv[U];
-    u64 sv[U];
+  const int T = blockDim.x, tid = threadIdx.x;
+  const int steps = (n + U * T - 1) / (U * T);
+  auto at = [&](int i, int u) { return i * U * T + u * T + tid; };
+  auto cl = [&](int v) { return v < n ? v : n - 1; };
+  int xa[U];
+  DT da[U];
+  u64 sa[U];
+  {
+    int x0[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) x[u] = nof[v0 + u * T];
+    for (int u = 0; u < U; u++) x0[u] = nof[cl(at(0, u))];
+#pragma unroll
+    for (int u = 0; u < U; u++) xa[u] = nof[cl(at(1, u))];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      dv[u] = ds[x[u]];
-      sv[u] = sd ? ss[x[u]] : 0ull;
-    }
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const bool r = dv[u] != none;
-      if (dd) dd[v0 + u * T] = r ? (int)dv[u] : INT_MAX;
-      if (sd) sd[v0 + u * T] = r ? sv[u] : 0ull;
+      da[u] = ds[x0[u]];
+      sa[u] = sd ? ss[x0[u]] : 0ull;
     }
   }
-  for (int v = v0; v < n; v += T) {
-    const int x = nof[v];
-    const DT dv = ds[x];
-    const bool r = dv != none;
-    if (dd) dd[v] = r ? (int)dv : INT_MAX;
-    if (sd) {
-      const u64 sv = ss[x];
-      sd[v] = r ? sv : 0ull;
+  for (int i = 0; i < steps; i++) {
+    int xb[U];
+    DT db[U];
+    u64 sb[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) xb[u] = nof[cl(at(i + 2, u))];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      db[u] = ds[xa[u]];
+      sb[u] = sd ? ss[xa[u]] : 0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int v = at(i, u);
+      if (v >= n) continue;
+      const bool r = da[u] != none;
+      if (dd) dd[v] = r ? (int)da[u] : INT_MAX;
+      if (sd) sd[v] = r ? sa[u] : 0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      xa[u] = xb[u];
+      da[u] = db[u];
+      sa[u] = sb[u];
     }
   }
 }
