@@ -755,9 +755,10 @@ __global__ __launch_bounds__(256) void k_ps_unpermute(T *const *src, T *const *d
 // to entry k of its level's log (k: its index in the level's frontier),
 // and a level set keeps each vertex's k beside its key, so the reads of a
 // level fall in one compact block written one level before (8 bytes per
-// vertex of that level) rather than across the row.  A level set that cannot place a claim within kLdsProbe buckets
-// stops the source (flag 2 in part[3]) and the host reruns the launch on the
-// distance-reading kernels, so results never depend on LDS capacity.
+// vertex of that level) rather than across the row.  A level set that
+// cannot place a claim within kLdsProbe buckets stops the source (flag 2
+// in part[3]) and the host reruns the launch on the distance-reading
+// kernels, so results never depend on LDS capacity.
 constexpr int kLdsBucketBits = 11;
 constexpr int kLdsBuckets = 1 << kLdsBucketBits;  // 4-slot buckets per level set (8192 slots)
 constexpr int kLdsProbe = 16;                     // buckets probed before giving up
@@ -1053,19 +1054,18 @@ __device__ __forceinline__ int ps_levels_lds(const int4 *LE, const u64 *LE8, int
     gp<int> Snext = BC ? S + qs : nullptr;
     gp<const int> sp_cur = BC ? S + qb + kLdsFront : spill0 + (size_t)b * n;
     gp<int> sp_next = BC ? S + qs + kLdsFront : spill0 + (size_t)(b ^ 1) * n;
-    // The frontier entries held in LDS first: each thread's next vertex
-    // and its table entry are loaded before the current one is expanded,
-    // and decoded only after it, so that load overlaps the current lookups
-    // and path-count loads (62.5 -> 56.4 ms per 2048^2 pass; decoding it at
-    // once made the compiler wait for the load where it was issued).  No
-    // global load feeds a frontier entry in this loop: an entry that might
+    // The frontier entries held in LDS first.  No global load feeds a
+    // frontier entry in this loop: an entry that might
     // come from HBM made every iteration wait for all of the previous one's
     // global stores (one in-order counter covers loads and stores).
     const unsigned qlds = min(qn, (unsigned)kLdsFront);
     for (unsigned base = 0; base < qlds; base += kLdsSlots * T) {
       // up to kLdsSlots frontier vertices per thread: their table entries
-      // are all loaded first, then each vertex is expanded, and their path
-      // counts and distances are stored after the last expansion.  On this
+      // are all loaded first (a table entry loaded before the expansion
+      // that needs it, and decoded only there, took a 2048^2 pass from 62.5
+      // to 56.4 ms), then each vertex is expanded, and their path counts (to
+      // the row and to the level's log) and distances are stored after the
+      // last expansion.  On this
       // hardware one in-order counter covers loads and stores, so a load
       // waited for after a store also waits for the store: stores between
       // the expansions made each expansion's path-count loads wait for the
