@@ -670,6 +670,23 @@ def test_persource_path_deeper_than_16bit_levels(gu, orc, gpu, sigma):
     assert st.max_level == n - 1
 
 
+@pytest.mark.parametrize("side", [2896, 2897])
+def test_persource_level_set_key_range(gu, orc, gpu, side):
+    """Intact grids on either side of the id range of the keyed level sets
+    (2^23 = 8,388,608 vertices), which a search with path counts uses:
+    2896^2 = 8,386,816 vertices run them with ids near the top of the range;
+    2897^2 = 8,392,609 take the distance kernel for path counts, and plain
+    level sets (no range limit) for distances only.  Corner, centre and
+    interior sources against the oracle (path counts wrap modulo 2^64 on an
+    intact grid)."""
+    hg = gu.HostGraph.grid(side, 0.0, seed=1)
+    assert hg.n == side * side and hg.degrees().max() <= 4
+    n = hg.n
+    src = np.array([0, n - 1, n // 2, 12345], dtype=np.int32)
+    _ps_compare(gu, orc, hg, src, "grid%d" % side)
+    _ps_compare(gu, orc, hg, src[:2], "grid%d-d" % side, sigma=False)
+
+
 def test_persource_diameter_on_level_sets(gu, orc, gpu):
     """Diameter sampling (eccentricities only) runs the level-set search
     without rows; sources whose search overflows (the 70000-vertex path's
