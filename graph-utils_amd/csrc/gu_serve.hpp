@@ -105,6 +105,8 @@ class Arena {
 struct Exec {
   virtual ~Exec() {}
   virtual char *arena_base() = 0;
+  // Called first on every server thread: the slot workers with their slot,
+  // the delivery thread with the slot count.
   virtual void thread_init(int slot) { (void)slot; }
   // wrap[k] (host) receives per source 1 when one of its path counts
   // wrapped modulo 2^64
