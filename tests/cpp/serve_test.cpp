@@ -68,10 +68,15 @@ struct FakeExec final : gus::Exec {
   char *arena_base() override { return arena.data(); }
   char *host_base() override { return host.data(); }
   std::atomic<int64_t> copied{0};
+  int deliver_us = 0;  // time each delivery copy takes
   int to_host(int, void *dst, const void *src, size_t bytes) override {
     std::memcpy(dst, src, bytes);
     copied += (int64_t)bytes;
     return GU_OK;
+  }
+  int deliver(void *dst, const void *src, size_t bytes) override {
+    if (deliver_us) std::this_thread::sleep_for(std::chrono::microseconds(deliver_us));
+    return to_host(0, dst, src, bytes);
   }
   void busy(int k) {
     int c = ++concurrent;
@@ -437,6 +442,28 @@ int main() {
           (long long)a.hbm_in_use, (long long)a.host_in_use);
     CHECK(ex.copied.load() == (int64_t)(3 * gs[0].n * 12), "copied %lld bytes", (long long)ex.copied.load());
     CHECK(srv.release(t) == 0 && srv.stats().host_in_use == 0, "host block released");
+    // deliveries still queued when the server stops are made before stop()
+    // returns: slow copies, a launch per source, and a stop while the
+    // workers are ahead of the delivery thread
+    {
+      FakeExec slow(1 << 20, 1 << 20);
+      slow.graphs = gs;
+      slow.deliver_us = 4000;
+      gus::Server ss(&slow, 1 << 20, 2, 1, 1 << 20);
+      ss.add_graph(gs[0].n, &id);
+      std::vector<int32_t> many = {1, 2, 3, 4, 5, 6};
+      gus::Spec sp = s;
+      sp.sources = many.data();
+      sp.nsrc = 6;
+      int64_t tq;
+      CHECK(ss.submit(sp, &tq) == 0, "submit slow host query");
+      std::this_thread::sleep_for(std::chrono::milliseconds(6));
+      ss.stop();
+      const gus::Stats st = ss.stats();
+      CHECK(st.sources >= 1 && slow.copied.load() == st.sources * (int64_t)gs[0].n * 12,
+            "every launched source delivered by stop (%lld sources, %lld bytes)", (long long)st.sources,
+            (long long)slow.copied.load());
+    }
     // a server without a host arena cannot hold host results
     gus::Server none(&ex, 1 << 20, 1, 8);
     none.add_graph(gs[0].n, &id);
