@@ -568,21 +568,32 @@ class Server {
             break;
         }
       }
-      // results wanted on the host: the launch goes to the delivery
-      // thread, which copies them while this slot runs its next launch
-      // (the rows belong to the queries' own blocks, which the next launch
-      // does not touch); its items complete after the copies
-      bool deliver = false;
-      for (const Item &it : p.items) deliver |= rc == 0 && it.q->to_host;
-      if (deliver && delivering_) {
+      // results wanted on the host: those items go to the delivery thread,
+      // which copies them while this slot runs its next launch (the rows
+      // belong to the queries' own blocks, which the next launch does not
+      // touch) and completes them after the copies; the launch's other
+      // items complete now
+      Delivery dl{{}, p.cls, rc, n, {}, {}};
+      std::vector<Item> now;
+      std::vector<int32_t> now_ecc, now_wrap;
+      size_t pos = 0;
+      for (const Item &it : p.items) {
+        const bool host = rc == 0 && delivering_ && it.q->to_host;
+        std::vector<Item> &to = host ? dl.items : now;
+        std::vector<int32_t> &e = host ? dl.ecc : now_ecc, &w = host ? dl.wrap : now_wrap;
+        to.push_back(it);
+        e.insert(e.end(), ecc.begin() + pos, ecc.begin() + pos + it.count);
+        w.insert(w.end(), wrap.begin() + pos, wrap.begin() + pos + it.count);
+        pos += (size_t)it.count;
+      }
+      if (!now.empty()) complete(now, p.cls, now_ecc, now_wrap, rc);
+      if (!dl.items.empty()) {
         {
           std::lock_guard<std::mutex> lk(mu_);
-          deliveries_.push_back(Delivery{p.items, p.cls, rc, n, ecc, wrap});
+          deliveries_.push_back(std::move(dl));
         }
         cv_deliver_.notify_one();
-        continue;
       }
-      complete(p.items, p.cls, ecc, wrap, rc);
     }
   }
 

@@ -205,11 +205,13 @@ std::vector<QSpec> make_queries(const std::vector<Graph> &gs, int count, uint64_
   return qs;
 }
 
-bool g_to_host = false;  // submit every query with results delivered to the host
+// results delivered to the host: 0 for no query, 1 for every query, 2 for
+// the queries whose first source is odd (launches mixing both kinds)
+int g_to_host = 0;
 
 gus::Spec spec_of(const QSpec &q) {
   gus::Spec s;
-  s.to_host = g_to_host;
+  s.to_host = g_to_host == 1 || (g_to_host == 2 && (q.src[0] & 1));
   s.graph = q.graph;
   s.cls = q.cls;
   s.nsrc = q.nsrc;
@@ -410,15 +412,21 @@ int main() {
   // results delivered to the host: the same answers, the device block given
   // back once a query is done, admission throttled by the host budget too
   {
-    g_to_host = true;
+    g_to_host = 1;
     gus::Stats hst;
     int hconc = 0;
     const std::vector<QOut> hmix = run_mix(gs, qs, largest * 3, 4, 16, &hconc, &hst, largest * 4);
-    g_to_host = false;
     compare(gs, qs, alone, hmix, "host mix");
     CHECK(hst.completed == (int64_t)qs.size(), "host mix completed %lld", (long long)hst.completed);
     CHECK(hst.hbm_in_use == 0 && hst.host_in_use == 0, "host mix: everything released");
     CHECK(hst.host_peak > 0 && hst.host_peak <= (int64_t)(largest * 4), "host peak %lld", (long long)hst.host_peak);
+    // host and device destinations in the same launches
+    g_to_host = 2;
+    const std::vector<QOut> bmix = run_mix(gs, qs, largest * 3, 4, 16, &hconc, &hst, largest * 4);
+    g_to_host = 0;
+    compare(gs, qs, alone, bmix, "host and device mix");
+    CHECK(hst.completed == (int64_t)qs.size(), "host and device mix completed %lld", (long long)hst.completed);
+    CHECK(hst.hbm_in_use == 0 && hst.host_in_use == 0, "host and device mix: everything released");
 
     FakeExec ex(1 << 20, 1 << 20);
     ex.graphs = gs;
