@@ -761,24 +761,28 @@ __global__ __launch_bounds__(256) void k_ps_unpermute(T *const *src, T *const *d
 // kernels, so results never depend on LDS capacity.
 constexpr int kLdsBucketBits = 11;
 constexpr int kLdsBuckets = 1 << kLdsBucketBits;  // 4-slot buckets per level set (8192 slots)
-constexpr int kLdsProbe = 16;                     // buckets probed before giving up
+constexpr int kLdsProbe = 15;                     // buckets probed before giving up
 constexpr int kLdsFront = 3072;                   // LDS frontier entries per buffer
 constexpr int kLdsSlots = 3;                      // frontier vertices a thread loads at once
 // Slots.  Searches without path counts keep vertex ids in the slots (home
-// bucket: the top 11 bits of w * kLdsMul).  With path counts the sets are
-// keyed and hold vertex ids below 2^23: h = w * kLdsMul mod 2^23 is a
-// bijection on them (odd multiplier); its top 11 bits are w's home bucket,
+// bucket: the top 11 bits of w * kLdsMul mod 2^32).  With path counts the
+// sets are keyed and hold vertex ids below 2^23: h = w * kLdsKeyMul mod
+// 2^23 is a bijection on them (odd multiplier); both multipliers are the
+// word size over the golden ratio, so consecutive ids spread evenly over
+// the buckets.  h's top 11 bits are w's home bucket,
 // and its low 12 bits r stay in the slot with the probe distance p (the
 // bucket's distance from home): key = r | p << 12.  Two ids with one key in
 // one bucket have the same h, so they are the same id.  A keyed slot holds
 // key | k << 16 (k < 8192, the vertex's index in its level's frontier, set
 // by the claim's winner once known), a value below 2^29; -1 marks an empty
-// slot, and kLdsKeyMask keeps the key bits and the bits set only there.
+// slot.  With p < 15 a key stays below 0xF000, so a slot matches a key
+// when its low 16 bits equal it (one 16-bit compare): an empty slot's low
+// half, 0xFFFF, is no key.
 constexpr int kLdsIdBits = 23;
 constexpr int kLdsMaxN = 1 << kLdsIdBits;  // graphs the keyed level sets can hold
 constexpr int kLdsRBits = kLdsIdBits - kLdsBucketBits;
-constexpr unsigned kLdsMul = 0x9E3779B1u;
-constexpr unsigned kLdsKeyMask = 0xE000FFFFu;
+constexpr unsigned kLdsMul = 0x9E3779B1u;     // 2^32 / golden ratio, odd
+constexpr unsigned kLdsKeyMul = 0x4F1BBDu;    // 2^23 / golden ratio, odd
 constexpr int kLdsLog = 4 * kLdsBuckets;  // path-count log entries per level (one per slot)
 // Facts about the graph that the host measured when it built the handle.
 // Bipartite: BFS levels are independent sets, so a neighbour of a level-L
@@ -831,8 +835,8 @@ struct PsLds {
 
 __device__ __forceinline__ unsigned lds_bucket(int w) { return ((unsigned)w * kLdsMul) >> (32 - kLdsBucketBits); }
 __device__ __forceinline__ bool q4_has(const int4 q, int w) { return q.x == w || q.y == w || q.z == w || q.w == w; }
-__device__ __forceinline__ unsigned lds_hash(int w) { return ((unsigned)w * kLdsMul) & ((1u << kLdsIdBits) - 1); }
-__device__ __forceinline__ bool lds_match(int x, unsigned key) { return ((unsigned)x & kLdsKeyMask) == key; }
+__device__ __forceinline__ unsigned lds_hash(int w) { return ((unsigned)w * kLdsKeyMul) & ((1u << kLdsIdBits) - 1); }
+__device__ __forceinline__ bool lds_match(int x, unsigned key) { return (unsigned short)x == (unsigned short)key; }
 __device__ __forceinline__ bool q4_open(const int4 q) { return q.x < 0 || q.y < 0 || q.z < 0 || q.w < 0; }
 
 // Buckets are probed linearly, one 16-byte LDS read each, and fill from
