@@ -1866,8 +1866,8 @@ int run_persource(gu_graph *g, const int *sources, int nsrc, const HostRows &hr,
       GuTraceRange tr("gu:per-source write-back");
       const dim3 ug((unsigned)std::max(1, std::min(pick_grid(g, 1), (g->n + 255) / 256)), (unsigned)nb);
       if (lds && rows) {  // 16-bit distances; path counts written for reached vertices only
-        // chunks of about 1024 vertices, a multiple of 8 per row
-        const unsigned wg = (unsigned)std::max<long long>(8, ((g->n + 1023) / 1024 + 7) / 8 * 8);
+        // chunks of about 4096 vertices (16 a thread), a multiple of 8 per row
+        const unsigned wg = (unsigned)std::max<long long>(8, ((g->n + 4095) / 4096 + 7) / 8 * 8);
         k_ps_unpermute_ds<<<dim3(wg, (unsigned)nb), 256, 0, st>>>(
             (int *const *)g->rowtab, sigma ? (u64 *const *)(g->rowtab + 256) : nullptr,
             hr.dist ? (int *const *)(g->rowtab + 1024) : nullptr,

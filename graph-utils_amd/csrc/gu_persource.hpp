@@ -1268,9 +1268,12 @@ __global__ __launch_bounds__(256) void k_ps_unpermute_ds(int *const *dsrc, u64 *
   const size_t chunk = ((size_t)n + G - 1) / G;
   const size_t lo = (size_t)c * chunk, hi = min((size_t)n, lo + chunk);
   gp<const int> nf = glob(nof);
-  // four vertices per thread with every load issued before the first
-  // store (the map entries, then the gathers): one wait per four
-  constexpr int U = 4;
+  // sixteen vertices per thread with every load issued before the first
+  // store (the map entries, then the gathers): one wait per sixteen.  On
+  // the 2048^2 grid's 256-source pass, 16 took the pass from 40.4 to 38.9
+  // ms against 4 (2: 40.7, 8: 40.5, 24: 41.2, 32: 39.1; two steps of 16
+  // per thread: 39.9)
+  constexpr int U = 16;
   const size_t T = blockDim.x;
   size_t v0 = lo + threadIdx.x;
   for (; v0 + (U - 1) * T < hi; v0 += U * T) {
