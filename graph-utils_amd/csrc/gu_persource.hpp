@@ -862,7 +862,7 @@ __device__ __forceinline__ int lds_find(const int4 *t, int w) {
   const unsigned r = h & ((1u << kLdsRBits) - 1);
   unsigned b = h >> kLdsRBits;
   // the home bucket, where almost every lookup ends, then the others in a
-  // rolled loop: sixteen unrolled probes per keyed lookup made the
+  // rolled loop: fifteen unrolled probes per keyed lookup made the
   // expansion loop too large to unroll (a rolled loop from the home bucket
   // on measured 1% slower)
   {
