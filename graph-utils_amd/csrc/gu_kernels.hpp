@@ -731,23 +731,54 @@ __device__ void pull_group(const Eng<NW> &e, int L, int w_l, const u64 *tgt_l, i
     disc |= 1ull << slot;
   };
 
+  // A step is 64 in-edges of the group.  The next step's in-edge is loaded
+  // before the row loads of the step in hand; with one mask word (64
+  // lanes) the next step's frontier masks are too, and the in-edge two
+  // steps ahead, so a step waits only for its rows (+0.9% per s20 pass).
+  // Wider masks keep the one-step form: the extra registers would cost
+  // them occupancy.
+  constexpr bool AHEAD = NW == 1;
   int t = lane;
   bool valid = t < T;
   int own = valid ? owner_of(g.off, t) : 0;
   int v = valid ? e.CP[g.r0[own] + t - g.off[own]] : 0;
+  u64 fcv[NW];
+  int tn = 0, ownn = 0, vn = 0;
+  bool validn = false;
+  if constexpr (AHEAD) {
+#pragma unroll
+    for (int j = 0; j < NW; j++) fcv[j] = valid ? fc[(size_t)v * NW + j] : 0ull;
+    tn = t + kWave;
+    validn = tn < T;
+    ownn = validn ? owner_of(g.off, tn) : 0;
+    vn = validn ? e.CP[g.r0[ownn] + tn - g.off[ownn]] : 0;
+  }
   for (int t0 = 0; t0 < T; t0 += kWave) {
     u64 m[NW];
     bool has = false;
 #pragma unroll
     for (int j = 0; j < NW; j++) {
-      m[j] = valid ? (fc[(size_t)v * NW + j] & g.tgt[own][j]) : 0ull;
+      if constexpr (AHEAD)
+        m[j] = fcv[j] & g.tgt[own][j];
+      else
+        m[j] = valid ? (fc[(size_t)v * NW + j] & g.tgt[own][j]) : 0ull;
       has |= m[j] != 0;
     }
-    // start the next step's in-edge load before the row loads of this one
-    const int tn = t + kWave;
-    const bool validn = tn < T;
-    const int ownn = validn ? owner_of(g.off, tn) : 0;
-    const int vn = validn ? e.CP[g.r0[ownn] + tn - g.off[ownn]] : 0;
+    int tnn = 0, ownnn = 0, vnn = 0;
+    bool validnn = false;
+    if constexpr (AHEAD) {
+#pragma unroll
+      for (int j = 0; j < NW; j++) fcv[j] = validn ? fc[(size_t)vn * NW + j] : 0ull;
+      tnn = tn + kWave;
+      validnn = tnn < T;
+      ownnn = validnn ? owner_of(g.off, tnn) : 0;
+      vnn = validnn ? e.CP[g.r0[ownnn] + tnn - g.off[ownnn]] : 0;
+    } else {
+      tn = t + kWave;
+      validn = tn < T;
+      ownn = validn ? owner_of(g.off, tn) : 0;
+      vn = validn ? e.CP[g.r0[ownn] + tn - g.off[ownn]] : 0;
+    }
     if (OR_FOUND) {
 #pragma unroll
       for (int j = 0; j < NW; j++)
@@ -804,6 +835,12 @@ __device__ void pull_group(const Eng<NW> &e, int L, int w_l, const u64 *tgt_l, i
     valid = validn;
     own = ownn;
     v = vn;
+    if constexpr (AHEAD) {
+      tn = tnn;
+      validn = validnn;
+      ownn = ownnn;
+      vn = vnn;
+    }
   }
   if (cur >= 0) flush(cur);
   if (BU) pb_push<NW>(e, L, pb, (disc >> lane) & 1ull, g.w[lane]);
