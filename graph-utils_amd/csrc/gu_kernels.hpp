@@ -665,9 +665,11 @@ __device__ void pull_group(const Eng<NW> &e, int L, int w_l, const u64 *tgt_l, i
   constexpr bool BU = MODE == PM_BU;
   constexpr bool OR_FOUND = MODE != PM_TD;
   // predecessor rows in flight, measured on s20 BFS: 64 lanes 12 (8: -0.5%,
-  // 16: equal at 80 VGPRs), 128 lanes 8 (4: -2%, 12: -4%), 256 lanes 8
-  // (4: -0.5%, 12: -45%)
-  constexpr int D = NW == 1 ? 12 : 8;
+  // 16: equal at 80 VGPRs), 128 lanes 8 (4: -2%, 12: -4%); 256 lanes 6:
+  // with 8 the expansion kernel spilled registers, and 1024-source
+  // Brandes on s22 at width 256 took 409.8 ms against 401.0 with 6 (4:
+  // 402.1)
+  constexpr int D = NW == 1 ? 12 : (NW == 2 ? 8 : 6);
   const int lane = lane_id();
   const int inc = wave_inclusive_sum(deg_l);
   const int T = wave_total(inc);
