@@ -1604,6 +1604,19 @@ void launch_ps_lds(const PsLdsArgs &c, bool sigma, bool rows, bool e8, int grid,
     k_ps_lds<false, false, false><<<grid, 1024, 0, st>>>(c);
 }
 
+// Path-count scratch rows, kPsCap of them.  On a graph with the degree-4
+// neighbour table a row holds 2n words: the level-set search with path
+// counts keeps a 16-byte record per vertex there (ps_rec); the other
+// kernels use the first n words of each row.
+int ensure_sscratch(gu_graph *g) {
+  PsWorkspace &p = g->pws;
+  if (p.sscratch) return GU_OK;
+  const size_t n = (size_t)g->n, words = (g->LE ? 2 : 1) * n;
+  GU_HIP(dalloc(&p.sscratch, (size_t)kPsCap * words));
+  p.bytes += (size_t)kPsCap * words * 8;
+  return GU_OK;
+}
+
 // The level-set kernels' path-count logs, one pair of levels per workgroup
 // (32 MB).
 int ensure_slog(gu_graph *g) {
@@ -1656,10 +1669,7 @@ int run_persource_queue(gu_graph *g, const int *sources, int nsrc, const HostRow
   if ((rc = ensure_jobs(g, nsrc))) return rc;
   PsWorkspace &p = g->pws;
   const size_t n = (size_t)g->n;
-  if (rl && sigma && !p.sscratch) {
-    GU_HIP(dalloc(&p.sscratch, (size_t)kPsCap * n));
-    p.bytes += (size_t)kPsCap * n * 8;
-  }
+  if (rl && sigma && (rc = ensure_sscratch(g))) return rc;
   const int32_t *nof = rl ? nof_host(g).data() : nullptr;
   std::vector<int> srcs((size_t)nsrc);
   std::vector<void *> rows(4 * (size_t)nsrc, nullptr);
@@ -1777,10 +1787,8 @@ int run_persource(gu_graph *g, const int *sources, int nsrc, const HostRows &hr,
   if (rc) return rc;
   PsWorkspace &p = g->pws;
   const size_t n = (size_t)g->n;
-  if (rl && sigma && !p.sscratch) {
-    GU_HIP(dalloc(&p.sscratch, (size_t)kPsCap * n));
-    p.bytes += (size_t)kPsCap * n * 8;
-  }
+  if (rl && sigma && (rc = ensure_sscratch(g))) return rc;
+  const size_t srow = (g->LE ? 2 : 1) * n;  // words a path-count scratch row takes
   const int32_t *nof = rl ? nof_host(g).data() : nullptr;
   u64 *part = g->part_h;
   int *depth = g->misc_h + 64;
@@ -1795,7 +1803,7 @@ int run_persource(gu_graph *g, const int *sources, int nsrc, const HostRows &hr,
       if (sigma && !sr) return GU_ERR_INVALID;  // path counts need every row
       if (rl) {
         t[i] = p.dscratch + (size_t)i * n;
-        t[256 + i] = sigma ? (void *)(p.sscratch + (size_t)i * n) : nullptr;
+        t[256 + i] = sigma ? (void *)(p.sscratch + (size_t)i * srow) : nullptr;
         t[1024 + i] = dr;
         t[1280 + i] = sr;
       } else {
@@ -1868,10 +1876,14 @@ int run_persource(gu_graph *g, const int *sources, int nsrc, const HostRows &hr,
       if (lds && rows) {  // 16-bit distances; path counts written for reached vertices only
         // chunks of about 4096 vertices (16 a thread), a multiple of 8 per row
         const unsigned wg = (unsigned)std::max<long long>(8, ((g->n + 4095) / 4096 + 7) / 8 * 8);
-        k_ps_unpermute_ds<<<dim3(wg, (unsigned)nb), 256, 0, st>>>(
-            (int *const *)g->rowtab, sigma ? (u64 *const *)(g->rowtab + 256) : nullptr,
-            hr.dist ? (int *const *)(g->rowtab + 1024) : nullptr,
-            sigma ? (u64 *const *)(g->rowtab + 1280) : nullptr, g->nof, g->n);
+        if (sigma)  // records (distance and path count) of the search with path counts
+          k_ps_unpermute_rec<<<dim3(wg, (unsigned)nb), 256, 0, st>>>(
+              (const ps_rec_t *const *)(g->rowtab + 256), hr.dist ? (int *const *)(g->rowtab + 1024) : nullptr,
+              (u64 *const *)(g->rowtab + 1280), g->nof, g->n);
+        else
+          k_ps_unpermute_ds<<<dim3(wg, (unsigned)nb), 256, 0, st>>>(
+              (int *const *)g->rowtab, nullptr, hr.dist ? (int *const *)(g->rowtab + 1024) : nullptr, nullptr,
+              g->nof, g->n);
       }
       if (hr.dist && !lds)
         k_ps_unpermute<int><<<ug, 256, 0, st>>>((int *const *)g->rowtab, (int *const *)(g->rowtab + 1024),
@@ -1906,10 +1918,7 @@ int run_persource_bc(gu_graph *g, const int *sources, int nsrc, const HostRows &
   if (rc) return rc;
   PsWorkspace &p = g->pws;
   const size_t n = (size_t)g->n;
-  if (!p.sscratch) {
-    GU_HIP(dalloc(&p.sscratch, (size_t)kPsCap * n));
-    p.bytes += (size_t)kPsCap * n * 8;
-  }
+  if ((rc = ensure_sscratch(g))) return rc;
   if (!p.qscratch) {
     GU_HIP(dalloc(&p.qscratch, (size_t)kPsCap * n));
     p.bytes += (size_t)kPsCap * n * 4;

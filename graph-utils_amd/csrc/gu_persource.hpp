@@ -791,6 +791,18 @@ constexpr int kLdsLog = 4 * kLdsBuckets;  // path-count log entries per level (o
 // 16-bit distances of the level-set search's scratch rows.
 constexpr unsigned short kD16None = 0xffff;
 constexpr int kD16Max = 0xfffe;
+// With path counts the level-set search keeps one 16-byte record per
+// vertex instead of a distance row and a path-count row: (path count low
+// and high words, distance, unused), distance -1 where not reached.  One
+// store writes a vertex's pair into one 32-byte sector; a 2-byte distance
+// store to a row of its own dirtied a second sector per vertex (2.5 ms of
+// a 256-source 2048^2 pass).
+// (A clang vector type: it can be stored through a global-address-space
+// pointer, which HIP's int4 class cannot.)
+typedef int ps_rec_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ ps_rec_t ps_rec(u64 sigma, int d) {
+  return ps_rec_t{(int)(unsigned)sigma, (int)(unsigned)(sigma >> 32), d, 0};
+}
 
 // Compact neighbour table: four int16 deltas w - v in one u64 (lowest 16
 // bits first), kLe8None for an empty slot.
@@ -939,11 +951,12 @@ __device__ __forceinline__ int lds_insert(int4 *t, int w, int &at) {
 // neighbours q (loaded by the caller).  (Issuing the parents' path-count
 // loads ahead of the claims, or expanding two vertices per thread, measured
 // slower: 61.4 and 80.4 against 56.4 ms per 2048^2 pass.)
-template <bool SIGMA, bool WANT_D, class DT, bool STORE = true>
-__device__ __forceinline__ void ps_visit_lds(const int4 q, gp<DT> d, gp<u64> sg, gp<const u64> lp, gp<u64> lc, int k,
-                                             int v, int L, const int4 *tp, const int4 *tl, int4 *tn, int &ovf,
-                                             int &bad, u64 &dag, u64 &teps, unsigned *cnt, PsLds &f, gp<int> spill,
-                                             gp<int> S, int nbuf, bool bip, u64 *sv_out = nullptr) {
+template <bool SIGMA, bool WANT_D, class DT, bool REC, bool STORE = true>
+__device__ __forceinline__ void ps_visit_lds(const int4 q, gp<DT> d, gp<u64> sg, gp<ps_rec_t> rec, gp<const u64> lp,
+                                             gp<u64> lc, int k, int v, int L, const int4 *tp, const int4 *tl,
+                                             int4 *tn, int &ovf, int &bad, u64 &dag, u64 &teps, unsigned *cnt,
+                                             PsLds &f, gp<int> spill, gp<int> S, int nbuf, bool bip,
+                                             u64 *sv_out = nullptr) {
   const int w[4] = {q.x, q.y, q.z, q.w};
   bool par[4], got[4];
   int pk[4], at[4];
@@ -1001,12 +1014,15 @@ __device__ __forceinline__ void ps_visit_lds(const int4 q, gp<DT> d, gp<u64> sg,
       }
     }
     if (STORE && L >= 1 && v >= 0) {
-      sg[v] = acc;
+      if (REC)
+        rec[v] = ps_rec(acc, L);
+      else
+        sg[v] = acc;
       lc[k] = acc;
     }
     if (!STORE) *sv_out = acc;
   }
-  if (STORE && WANT_D && v >= 0) d[v] = (DT)L;
+  if (STORE && WANT_D && !REC && v >= 0) d[v] = (DT)L;
 }
 
 // The level loop over LDS level sets (one workgroup barrier a level).  With
@@ -1016,11 +1032,13 @@ __device__ __forceinline__ void ps_visit_lds(const int4 q, gp<DT> d, gp<u64> sg,
 // overflowed (or, with 16-bit distances, the search went deeper than
 // kD16Max levels).  With path counts, lg holds the path-count logs of two
 // levels (2 * kLdsLog entries; level L's at lg + (L & 1) * kLdsLog).
-template <bool SIGMA, bool BC, bool WANT_D, class DT, bool E8>
+// With REC the distances and path counts go to the records rec (ps_rec)
+// instead of the rows d and sg.
+template <bool SIGMA, bool BC, bool WANT_D, class DT, bool E8, bool REC = false>
 __device__ __forceinline__ int ps_levels_lds(const int4 *LE, const u64 *LE8, int n, int s, gp<DT> d, gp<u64> sg,
-                                             gp<u64> lg, gp<int> spill0, gp<int> S, gp<int> E, PsLds &f,
-                                             unsigned *s_cnt, int *s_bad, int &ovf, u64 &teps, u64 &reached,
-                                             u64 &dag, bool bip) {
+                                             gp<ps_rec_t> rec, gp<u64> lg, gp<int> spill0, gp<int> S, gp<int> E,
+                                             PsLds &f, unsigned *s_cnt, int *s_bad, int &ovf, u64 &teps,
+                                             u64 &reached, u64 &dag, bool bip) {
   const int tid = threadIdx.x, T = blockDim.x;
   for (int i = tid; i < 4 * kLdsBuckets; i += T) (&f.tab[0][0])[i] = make_int4(-1, -1, -1, -1);
   if (tid < 3) s_cnt[tid] = 0;
@@ -1033,7 +1051,10 @@ __device__ __forceinline__ int ps_levels_lds(const int4 *LE, const u64 *LE8, int
     else
       f.tab[0][lds_bucket(s)].x = s;
     if (SIGMA) {
-      sg[s] = 1;
+      if (REC)
+        rec[s] = ps_rec(1, 0);
+      else
+        sg[s] = 1;
       lg[0] = 1;
     }
     f.fv[0][0] = s;
@@ -1088,13 +1109,21 @@ __device__ __forceinline__ int ps_levels_lds(const int4 *LE, const u64 *LE8, int
       for (int i = 0; i < kLdsSlots; i++) {
         sv[i] = 0;
         if (base + i * T < qlds)  // uniform
-          ps_visit_lds<SIGMA, WANT_D || BC, DT, false>(le_decode<E8>(vv[i], raw[i]), d, sg, lp, lc, 0, vv[i], L,
-                                                        tp, tl, tn, ovf, bad, dag, teps, cnt, f, sp_next, Snext,
-                                                        b ^ 1, bip, &sv[i]);
+          ps_visit_lds<SIGMA, WANT_D || BC, DT, REC, false>(le_decode<E8>(vv[i], raw[i]), d, sg, rec, lp, lc, 0,
+                                                             vv[i], L, tp, tl, tn, ovf, bad, dag, teps, cnt, f,
+                                                             sp_next, Snext, b ^ 1, bip, &sv[i]);
       }
 #pragma unroll
       for (int i = 0; i < kLdsSlots; i++) {
         if (vv[i] < 0) continue;
+        if (REC) {
+          // the source's record was written before the first level
+          if (L >= 1) {
+            rec[vv[i]] = ps_rec(sv[i], L);
+            lc[base + i * T + tid] = sv[i];
+          }
+          continue;
+        }
         if (SIGMA && L >= 1) {
           sg[vv[i]] = sv[i];
           lc[base + i * T + tid] = sv[i];
@@ -1107,8 +1136,8 @@ __device__ __forceinline__ int ps_levels_lds(const int4 *LE, const u64 *LE8, int
       const unsigned k = base + tid;
       const int vs = k < qn ? sp_cur[k - kLdsFront] : -1;
       const int4 q = le_decode<E8>(vs, le_fetch<E8>(LE, LE8, vs));
-      ps_visit_lds<SIGMA, WANT_D || BC>(q, d, sg, lp, lc, (int)k, vs, L, tp, tl, tn, ovf, bad, dag, teps, cnt, f,
-                                        sp_next, Snext, b ^ 1, bip);
+      ps_visit_lds<SIGMA, WANT_D || BC, DT, REC>(q, d, sg, rec, lp, lc, (int)k, vs, L, tp, tl, tn, ovf, bad, dag,
+                                                 teps, cnt, f, sp_next, Snext, b ^ 1, bip);
     }
     if (bad) *s_bad = 1;
     __syncthreads();
@@ -1127,13 +1156,60 @@ __device__ __forceinline__ int ps_levels_lds(const int4 *LE, const u64 *LE8, int
   return L;
 }
 
+// ps_writeback for the records of a search with path counts.
+__device__ __forceinline__ void ps_writeback_rec(const PsQueue &q, int j, gp<const ps_rec_t> rs, int n) {
+  gp<int> dd = glob(q.ddst ? q.ddst[j] : nullptr);
+  gp<u64> sd = glob(q.sdst ? q.sdst[j] : nullptr);
+  gp<const int> nof = glob(q.nof);
+  if (!dd && !sd) return;
+  constexpr int U = 6;
+  const int T = blockDim.x, tid = threadIdx.x;
+  const int steps = (n + U * T - 1) / (U * T);
+  auto at = [&](int i, int u) { return i * U * T + u * T + tid; };
+  auto cl = [&](int v) { return v < n ? v : n - 1; };
+  int xa[U];
+  ps_rec_t ra[U];
+  {
+    int x0[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) x0[u] = nof[cl(at(0, u))];
+#pragma unroll
+    for (int u = 0; u < U; u++) xa[u] = nof[cl(at(1, u))];
+#pragma unroll
+    for (int u = 0; u < U; u++) ra[u] = rs[x0[u]];
+  }
+  for (int i = 0; i < steps; i++) {
+    int xb[U];
+    ps_rec_t rb[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) xb[u] = nof[cl(at(i + 2, u))];
+#pragma unroll
+    for (int u = 0; u < U; u++) rb[u] = rs[xa[u]];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int v = at(i, u);
+      if (v >= n) continue;
+      const bool r = ra[u].z >= 0;
+      if (dd) dd[v] = r ? ra[u].z : INT_MAX;
+      if (sd) sd[v] = r ? ((u64)(unsigned)ra[u].y << 32 | (unsigned)ra[u].x) : 0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      xa[u] = xb[u];
+      ra[u] = rb[u];
+    }
+  }
+}
+
 struct PsLdsArgs {
   const int4 *LE;
   int n;
   const int *srcs;    // [nsrc] (cluster-order ids)
   int *const *dist;   // [nsrc] scratch rows (16-bit distances), or null
                       //   (diameter; with a queue: the slot's scratch rows)
-  u64 *const *sigma;  // [nsrc] rows, or null
+  u64 *const *sigma;  // [nsrc] with path counts: record rows (ps_rec, 16
+                      //   bytes a vertex; with a queue: the slot's scratch
+                      //   records at q.sscr + slot * 2n), or null
   int *queues;        // [grid][2n+2] frontier spill
   int *depth;
   u64 *part;          // [nsrc][4], as PsArgs; part[3] bit 1: level set overflow
@@ -1144,15 +1220,17 @@ struct PsLdsArgs {
   u64 *slog;          // with path counts: [grid][2 * kLdsLog] path-count logs
 };
 
-// The search: with rows (WANT_D) the distance row, in 16 bits (kD16None:
-// not reached), is first set to kD16None, the only row initialised: it
-// tells reached vertices apart when the rows are written back
-// (k_ps_unpermute_ds); path counts are written for reached vertices only.
-// On a connected graph every vertex's distance is written by the search, so
-// the fill is skipped.  A source deeper than kD16Max levels stops as a
-// level-set overflow does (the host reruns the launch).
+// The search: with distance rows only (WANT_D) the row, in 16 bits
+// (kD16None: not reached), is first set to kD16None, the only row
+// initialised: it tells reached vertices apart when the rows are written
+// back (k_ps_unpermute_ds).  With path counts the records (ps_rec) are set
+// to distance -1 first and written for reached vertices
+// (k_ps_unpermute_rec).  On a connected graph every vertex is written by
+// the search, so the fill is skipped.  A source deeper than kD16Max levels
+// stops as a level-set overflow does (the host reruns the launch).
 template <bool SIGMA, bool WANT_D, bool E8>
 __global__ __launch_bounds__(1024) void k_ps_lds(PsLdsArgs a) {
+  constexpr bool REC = SIGMA && WANT_D;
   __shared__ PsLds f;
   __shared__ unsigned s_cnt[3];
   __shared__ u64 s_red[16][2];
@@ -1163,24 +1241,29 @@ __global__ __launch_bounds__(1024) void k_ps_lds(PsLdsArgs a) {
   for (int j = ps_next_job(a.q, &s_job, true); j < a.q.nsrc; j = ps_next_job(a.q, &s_job, false)) {
     const int s = a.srcs[j];
     gp<unsigned short> d = nullptr;
-    if (WANT_D)
+    if (WANT_D && !REC)
       d = glob(reinterpret_cast<unsigned short *>(a.dist && a.dist[j] ? a.dist[j] : a.q.dscr + slot * n));
-    gp<u64> sg = glob(SIGMA ? (a.sigma[j] ? a.sigma[j] : a.q.sscr + slot * n) : nullptr);
+    gp<ps_rec_t> rec = nullptr;
+    if (REC) rec = glob(reinterpret_cast<ps_rec_t *>(a.sigma[j] ? a.sigma[j] : a.q.sscr + slot * 2 * n));
     gp<u64> lg = glob(SIGMA ? a.slog + slot * (2 * kLdsLog) : nullptr);
-    if (WANT_D && !(a.flags & kLdsConnected))
-      for (int i = tid; i < a.n; i += T) d[i] = kD16None;
+    if (!(a.flags & kLdsConnected)) {
+      if (REC)
+        for (int i = tid; i < a.n; i += T) rec[i] = ps_rec(0, -1);
+      else if (WANT_D)
+        for (int i = tid; i < a.n; i += T) d[i] = kD16None;
+    }
     if (tid == 0) s_ovf = 0;
     __syncthreads();
     int ovf = 0;
     u64 teps = 0, reached = 1, dag = 0;
-    const int L = ps_levels_lds<SIGMA, false, WANT_D, unsigned short, E8>(a.LE, a.LE8, a.n, s, d, sg, lg, spill,
-                                                                          nullptr, nullptr, f,
-                                                      s_cnt, &s_bad, ovf, teps, reached, dag,
-                                                      (a.flags & kLdsBipartite) != 0);
+    const int L = ps_levels_lds<SIGMA, false, WANT_D, unsigned short, E8, REC>(
+        a.LE, a.LE8, a.n, s, d, nullptr, rec, lg, spill, nullptr, nullptr, f, s_cnt, &s_bad, ovf, teps, reached, dag,
+        (a.flags & kLdsBipartite) != 0);
     ps_finish(teps, dag, reached, L < 0 ? 0 : L, ovf, &s_ovf, s_red, a.part, a.depth, a.ovf, j);
     if (tid == 0 && L < 0) a.part[4 * (size_t)j + 3] |= 2ull;
     // an overflowed source is searched again by the host, rows included
-    if (WANT_D && a.q.nof && L >= 0) ps_writeback<unsigned short>(a.q, j, d, sg, a.n, kD16None);
+    if (REC && a.q.nof && L >= 0) ps_writeback_rec(a.q, j, rec, a.n);
+    if (WANT_D && !REC && a.q.nof && L >= 0) ps_writeback<unsigned short>(a.q, j, d, nullptr, a.n, kD16None);
     __syncthreads();  // rows and s_job read before the next source
   }
 }
@@ -1210,9 +1293,9 @@ __global__ __launch_bounds__(1024) void k_ps_bc_lds(PsBcArgs a) {
   int ovf = 0;
   u64 teps = 0, reached = 1, dag = 0;
   gp<u64> lg = glob(a.slog + (size_t)blockIdx.x * (2 * kLdsLog));
-  const int L = ps_levels_lds<true, true, true, int, E8>(a.LE, a.LE8, a.n, s, d, sg, lg, nullptr, S, E, f, s_cnt,
-                                                         &s_bad, ovf, teps,
-                                                reached, dag, (a.lds_flags & kLdsBipartite) != 0);
+  const int L = ps_levels_lds<true, true, true, int, E8>(a.LE, a.LE8, a.n, s, d, sg, nullptr, lg, nullptr, S, E, f,
+                                                         s_cnt, &s_bad, ovf, teps, reached, dag,
+                                                         (a.lds_flags & kLdsBipartite) != 0);
   __syncthreads();
   if (L < 0) {  // rerun by the host on k_ps_bc
     ps_finish(teps, dag, reached, 0, ovf, &s_ovf, s_red, a.part, a.depth, a.ovf);
@@ -1299,6 +1382,43 @@ __global__ __launch_bounds__(256) void k_ps_unpermute_ds(int *const *dsrc, u64 *
     const u64 sv = ss ? ss[x] : 0ull;
     if (dd) dd[v] = dv != kD16None ? (int)dv : INT_MAX;
     if (sd) sd[v] = dv != kD16None ? sv : 0ull;
+  }
+}
+
+// The same for the records of a search with path counts (ps_rec; distance
+// -1 where not reached).
+__global__ __launch_bounds__(256) void k_ps_unpermute_rec(const ps_rec_t *const *rsrc, int *const *ddst,
+                                                          u64 *const *sdst, const int *nof, int n) {
+  gp<int> dd = glob(ddst ? ddst[blockIdx.y] : nullptr);
+  gp<u64> sd = glob(sdst ? sdst[blockIdx.y] : nullptr);
+  gp<const ps_rec_t> rs = glob(rsrc[blockIdx.y]);
+  const unsigned G = gridDim.x, b = blockIdx.x;
+  const unsigned c = (b % 8) * (G / 8) + b / 8;
+  const size_t chunk = ((size_t)n + G - 1) / G;
+  const size_t lo = (size_t)c * chunk, hi = min((size_t)n, lo + chunk);
+  gp<const int> nf = glob(nof);
+  constexpr int U = 16;
+  const size_t T = blockDim.x;
+  size_t v0 = lo + threadIdx.x;
+  for (; v0 + (U - 1) * T < hi; v0 += U * T) {
+    int x[U];
+    ps_rec_t r[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) x[u] = nf[v0 + u * T];
+#pragma unroll
+    for (int u = 0; u < U; u++) r[u] = rs[x[u]];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const bool ok = r[u].z >= 0;
+      if (dd) dd[v0 + u * T] = ok ? r[u].z : INT_MAX;
+      if (sd) sd[v0 + u * T] = ok ? ((u64)(unsigned)r[u].y << 32 | (unsigned)r[u].x) : 0ull;
+    }
+  }
+  for (size_t v = v0; v < hi; v += T) {
+    const ps_rec_t r = rs[nf[v]];
+    const bool ok = r.z >= 0;
+    if (dd) dd[v] = ok ? r.z : INT_MAX;
+    if (sd) sd[v] = ok ? ((u64)(unsigned)r.y << 32 | (unsigned)r.x) : 0ull;
   }
 }
 
