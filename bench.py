@@ -445,49 +445,58 @@ def _pct(x, p):
     return round(float(np.percentile(np.asarray(x), p)), 2) if len(x) else None
 
 
-def _served(gu, dgs, qs, arrivals_ms, budget, slots, max_pack, host_budget=0):
+def _warm(gu, srv, ids, dgs, slots, host):
+    """Every slot sees every kind at full width on every graph, with results
+    in HBM and, when the server has a host arena, in host memory."""
+    warm = []
+    for g, dg in enumerate(dgs):
+        src = np.arange(256, dtype=np.int32) % dg.n
+        for kind, fl in ((gu.QUERY_BFS, gu.QF_SIGMA), (gu.QUERY_BC, 0), (gu.QUERY_DIAMETER, 0)):
+            for extra in ((0, gu.QF_HOST) if host else (0,)):
+                for _ in range(slots):
+                    warm.append(srv.submit(ids[g], kind, src, fl | extra))
+    for t in warm:
+        srv.wait(t)
+        srv.release(t)
+
+
+_LEG_COUNTERS = ("submitted", "completed", "rejected", "launches", "sources")
+
+
+def _served(gu, srv, ids, qs, arrivals_ms, host):
     """Runs the stream through the server; returns (latencies ms, makespan ms,
-    stats).  With host_budget > 0 every query asks for its results in host
-    memory (GU_QF_HOST: the server copies them to its pinned host arena
-    before reporting the query done), else they stay in HBM.  A collector
-    thread takes finished queries as they come and releases them; latency =
-    done (results where they were asked for) - scheduled arrival, on the
-    server clock."""
+    stats).  With host every query asks for its results in host memory
+    (GU_QF_HOST: the server copies them to its pinned host arena before
+    reporting the query done), else they stay in HBM.  A collector thread
+    takes finished queries as they come and releases them; latency = done
+    (results where they were asked for) - scheduled arrival, on the server
+    clock.  The stats' counters cover this stream only; the peaks cover the
+    server's life so far."""
     import threading
 
-    host = host_budget > 0
-    with gu.Server(hbm_budget=budget, slots=slots, max_pack=max_pack, host_budget=host_budget) as srv:
-        ids = [srv.add_graph(dg) for dg in dgs]
-        # warm-up: every slot sees every kind at full width on every graph
-        warm = []
-        for g, dg in enumerate(dgs):
-            src = np.arange(256, dtype=np.int32) % dg.n
-            for kind, fl in ((gu.QUERY_BFS, gu.QF_SIGMA), (gu.QUERY_BC, 0), (gu.QUERY_DIAMETER, 0)):
-                for _ in range(slots):
-                    warm.append(srv.submit(ids[g], kind, src, fl | (gu.QF_HOST if host else 0)))
-        for t in warm:
-            srv.wait(t)
+    st0 = srv.stats()
+    done = {}
+    nq = len(qs)
+
+    def collect():
+        for _ in range(nq):
+            t, r = srv.next_done(600000)
+            done[t] = (r.done_ms, r.status, r.launches)
             srv.release(t)
-        done = {}
-        nq = len(qs)
 
-        def collect():
-            for _ in range(nq):
-                t, r = srv.next_done(600000)
-                done[t] = (r.done_ms, r.status, r.launches)
-                srv.release(t)
-
-        th = threading.Thread(target=collect)
-        th.start()
-        tick = [0] * nq
-        start = srv.now_ms() + 2.0
-        for i, (g, kind, src, fl) in enumerate(qs):
-            wait = start + arrivals_ms[i] - srv.now_ms()
-            if wait > 0.2:
-                time.sleep(wait / 1e3)
-            tick[i] = srv.submit(ids[g], kind, src, fl | (gu.QF_HOST if host else 0))
-        th.join()
-        st = srv.stats()
+    th = threading.Thread(target=collect)
+    th.start()
+    tick = [0] * nq
+    start = srv.now_ms() + 2.0
+    for i, (g, kind, src, fl) in enumerate(qs):
+        wait = start + arrivals_ms[i] - srv.now_ms()
+        if wait > 0.2:
+            time.sleep(wait / 1e3)
+        tick[i] = srv.submit(ids[g], kind, src, fl | (gu.QF_HOST if host else 0))
+    th.join()
+    st = srv.stats()
+    for k in _LEG_COUNTERS:
+        st[k] -= st0[k]
     lat = [done[tick[i]][0] - (start + arrivals_ms[i]) for i in range(nq)]
     bad = sum(1 for i in range(nq) if done[tick[i]][1] != 0)
     makespan = max(done[t][0] for t in tick) - start
@@ -588,26 +597,33 @@ def serve_stream(args, gu, hgs, tune, plain_host=True):
                 b.close()
             unit[name] = row
         print("serve: unit costs %s" % json.dumps(unit), file=sys.stderr, flush=True)
-        # results to the host
-        lat_hs, mk_hs, st_hs = _served(gu, dgs, qs, zero, budget, args.slots, 256, host_budget)
-        sat_h = nq / (mk_hs / 1e3)
-        qps_h = args.qps if args.qps > 0 else 0.9 * sat_h
-        arr_h = arr_of(qps_h)
-        lat_h, mk_h, st_h = _served(gu, dgs, qs, arr_h, budget, args.slots, 256, host_budget)
-        print("serve: host results, saturated %.1f q/s, served at %.1f q/s offered" % (sat_h, qps_h),
-              file=sys.stderr, flush=True)
-        if plain_host:
-            lat_ph, mk_ph = _plain(gu, dgs, qs, arr_h, True)
-            print("serve: plain (host results) done", file=sys.stderr, flush=True)
-        # results left in HBM
-        lat_s, mk_s, st_s = _served(gu, dgs, qs, zero, budget, args.slots, 256)
-        sat_d = nq / (mk_s / 1e3)
-        qps_d = args.qps if args.qps > 0 else 0.9 * sat_d
-        arr_d, arr70 = arr_of(qps_d), arr_of(0.7 * sat_d)
-        lat_70, mk_70, st_70 = _served(gu, dgs, qs, arr70, budget, args.slots, 256)
-        lat_d, mk_d, st_d = _served(gu, dgs, qs, arr_d, budget, args.slots, 256)
-        lat_pd, mk_pd = _plain(gu, dgs, qs, arr_d, False)
-        print("serve: device results, saturated %.1f q/s; plain done" % sat_d, file=sys.stderr, flush=True)
+        # One server for every leg, its graphs registered (each slot's search
+        # workspace reserved) and warmed once.  The plain legs run while it
+        # is alive and idle: freeing its memory just before them slowed
+        # their per-query allocations by seconds at a time.
+        with gu.Server(hbm_budget=budget, slots=args.slots, max_pack=256, host_budget=host_budget) as srv:
+            ids = [srv.add_graph(dg) for dg in dgs]
+            _warm(gu, srv, ids, dgs, args.slots, host_budget > 0)
+            # results to the host
+            lat_hs, mk_hs, st_hs = _served(gu, srv, ids, qs, zero, True)
+            sat_h = nq / (mk_hs / 1e3)
+            qps_h = args.qps if args.qps > 0 else 0.9 * sat_h
+            arr_h = arr_of(qps_h)
+            lat_h, mk_h, st_h = _served(gu, srv, ids, qs, arr_h, True)
+            print("serve: host results, saturated %.1f q/s, served at %.1f q/s offered" % (sat_h, qps_h),
+                  file=sys.stderr, flush=True)
+            if plain_host:
+                lat_ph, mk_ph = _plain(gu, dgs, qs, arr_h, True)
+                print("serve: plain (host results) done", file=sys.stderr, flush=True)
+            # results left in HBM
+            lat_s, mk_s, st_s = _served(gu, srv, ids, qs, zero, False)
+            sat_d = nq / (mk_s / 1e3)
+            qps_d = args.qps if args.qps > 0 else 0.9 * sat_d
+            arr_d, arr70 = arr_of(qps_d), arr_of(0.7 * sat_d)
+            lat_70, mk_70, st_70 = _served(gu, srv, ids, qs, arr70, False)
+            lat_d, mk_d, st_d = _served(gu, srv, ids, qs, arr_d, False)
+            lat_pd, mk_pd = _plain(gu, dgs, qs, arr_d, False)
+            print("serve: device results, saturated %.1f q/s; plain done" % sat_d, file=sys.stderr, flush=True)
     finally:
         for dg in dgs:
             dg.close()

@@ -1912,6 +1912,23 @@ int run_persource(gu_graph *g, const int *sources, int nsrc, const HostRows &hr,
 // Brandes on the per-source engine: forward and backward of each source in
 // one workgroup (k_ps_bc), then the FP64 accumulation into the targets in
 // source order (k_ps_bc_reduce).
+// Brandes scratch rows, kPsCap of each: (1 + dependency) / path count of
+// each vertex (qscratch) and, when the dependencies are not written into the caller's
+// rows directly, the dependencies (xscratch).
+int ensure_bc_scratch(gu_graph *g, bool need_x) {
+  PsWorkspace &p = g->pws;
+  const size_t n = (size_t)g->n;
+  if (!p.qscratch) {
+    GU_HIP(dalloc(&p.qscratch, (size_t)kPsCap * n));
+    p.bytes += (size_t)kPsCap * n * 4;
+  }
+  if (need_x && !p.xscratch) {
+    GU_HIP(dalloc(&p.xscratch, (size_t)kPsCap * n));
+    p.bytes += (size_t)kPsCap * n * 4;
+  }
+  return GU_OK;
+}
+
 int run_persource_bc(gu_graph *g, const int *sources, int nsrc, const HostRows &hr, gu_stats &s,
                      int &dmax, hipStream_t st) {
   int rc = ensure_pws(g, true);
@@ -1919,17 +1936,10 @@ int run_persource_bc(gu_graph *g, const int *sources, int nsrc, const HostRows &
   PsWorkspace &p = g->pws;
   const size_t n = (size_t)g->n;
   if ((rc = ensure_sscratch(g))) return rc;
-  if (!p.qscratch) {
-    GU_HIP(dalloc(&p.qscratch, (size_t)kPsCap * n));
-    p.bytes += (size_t)kPsCap * n * 4;
-  }
   const bool rl = g->LR != nullptr;
   bool need_x = rl || hr.delta == nullptr;
   for (int i = 0; !need_x && i < nsrc; i++) need_x = hr.delta[i] == nullptr;
-  if (need_x && !p.xscratch) {
-    GU_HIP(dalloc(&p.xscratch, (size_t)kPsCap * n));
-    p.bytes += (size_t)kPsCap * n * 4;
-  }
+  if ((rc = ensure_bc_scratch(g, need_x))) return rc;
   u64 *part = g->part_h;
   int *depth = g->misc_h + 64;
   int ovf_any = 0;
@@ -2040,26 +2050,64 @@ int resolve_width(const gu_config &cfg, int nsrc) {
   return nsrc <= 64 ? 64 : (nsrc <= 128 ? 128 : 256);
 }
 
+// The launch shape of a job of nsrc sources: fields left at "auto" take the
+// autotuned shape of the job family, for launches of about the size it was
+// tuned on or larger (small launches keep the degree-profile defaults).
+gu_config job_config(const gu_graph *g, gu_config cfg, Job job, int nsrc) {
+  const gu_graph::Tuned &t = g->tuned[job == JOB_BC ? 1 : 0];
+  if (2 * nsrc < t.sources) return cfg;
+  if (cfg.engine == GU_ENGINE_AUTO) cfg.engine = t.engine;
+  if (cfg.variant == GU_VARIANT_AUTO && t.variant >= 0) cfg.variant = t.variant;
+  if (cfg.wg_size == 0) cfg.wg_size = t.wg;
+  if (cfg.alpha == 0) cfg.alpha = t.alpha;
+  if (cfg.beta == 0) cfg.beta = t.beta;
+  if (cfg.batch_width == 0 && t.width > 0) {
+    const int w = nsrc <= 64 ? 64 : (nsrc <= 128 ? 128 : 256);
+    cfg.batch_width = std::min(w, t.width);
+  }
+  return cfg;
+}
+
+// Allocates every workspace buffer that a launch of 1 to kPsCap sources of
+// each job (search with or without path counts, Brandes, diameter) can need
+// on this handle with this configuration, so that such launches allocate
+// nothing.  The batched Brandes stack is sized for 24n entries, room for a
+// chunk of 8 levels on top of 14n entries already pushed.
+int reserve_launches(gu_graph *g, const gu_config &base) {
+  GU_HIP(hipSetDevice(g->device));
+  const size_t n = (size_t)g->n;
+  const bool rl = g->LR != nullptr;
+  const int4 *LE = rl ? g->LE : nullptr;
+  int rc = GU_OK;
+  // Brandes and the full width first: the batched workspace is sized for the
+  // widest batch, and a narrower one uses its front
+  for (Job job : {JOB_BC, JOB_BFS, JOB_DIAM})
+    for (int nsrc : {kPsCap, 1}) {
+      const gu_config cfg = job_config(g, base, job, nsrc);
+      if (cfg.engine < 0 || cfg.engine > GU_ENGINE_PER_SOURCE) return GU_ERR_INVALID;
+      if (resolve_engine(g, cfg, job) == GU_ENGINE_PER_SOURCE) {
+        const bool sigma = job != JOB_DIAM;  // searches may ask for path counts
+        if ((rc = ensure_pws(g, true))) return rc;
+        if (sigma && (job == JOB_BC || rl) && (rc = ensure_sscratch(g))) return rc;
+        if (job == JOB_BC && (rc = ensure_bc_scratch(g, true))) return rc;
+        if (sigma && lds_ok(g, LE, true) && (rc = ensure_slog(g))) return rc;
+        if (cfg.source_queue == 1 && (rc = ensure_jobs(g, kPsCap))) return rc;
+      } else {
+        const int width = resolve_width(cfg, nsrc);
+        if (width < 0) return GU_ERR_INVALID;
+        const bool stack = job == JOB_BC;
+        if ((rc = ensure_ws(g, width / 64, stack, stack ? 24 * n : 0, nullptr))) return rc;
+      }
+    }
+  return GU_OK;
+}
+
 int run_job(gu_graph *g, const int32_t *sources, int32_t nsrc, Job job, const HostRows &hr,
             int32_t *diam, const gu_config *cfgp, gu_stats *stats) {
   if (!g) return GU_ERR_INVALID;
   gu_config cfg{};
   if (cfgp) cfg = *cfgp;
-  if (2 * nsrc >= g->tuned[job == JOB_BC ? 1 : 0].sources) {
-    // fields left at "auto" take the autotuned shape of this job family,
-    // for launches of about the size it was tuned on or larger (small
-    // launches keep the degree-profile defaults)
-    const gu_graph::Tuned &t = g->tuned[job == JOB_BC ? 1 : 0];
-    if (cfg.engine == GU_ENGINE_AUTO) cfg.engine = t.engine;
-    if (cfg.variant == GU_VARIANT_AUTO && t.variant >= 0) cfg.variant = t.variant;
-    if (cfg.wg_size == 0) cfg.wg_size = t.wg;
-    if (cfg.alpha == 0) cfg.alpha = t.alpha;
-    if (cfg.beta == 0) cfg.beta = t.beta;
-    if (cfg.batch_width == 0 && t.width > 0) {
-      const int w = nsrc <= 64 ? 64 : (nsrc <= 128 ? 128 : 256);
-      cfg.batch_width = std::min(w, t.width);
-    }
-  }
+  cfg = job_config(g, cfg, job, nsrc);
   if (cfg.direction < 0 || cfg.direction > 2) return GU_ERR_INVALID;
   if (cfg.variant < 0 || cfg.variant > GU_VARIANT_AUTO) return GU_ERR_INVALID;
   if (cfg.engine < 0 || cfg.engine > GU_ENGINE_PER_SOURCE) return GU_ERR_INVALID;
@@ -2763,6 +2811,13 @@ int gu_search_rows_wrap(gu_graph_t g, const int32_t *sources, int32_t nsrc, int3
   hr.wrap = wrap;
   const Job job = (!dist_rows && !sigma_rows && ecc) ? JOB_DIAM : JOB_BFS;
   return run_job(g, sources, nsrc, job, hr, nullptr, cfg, stats);
+}
+
+int gu_reserve_launches(gu_graph_t g, const gu_config *cfg) {
+  if (!g) return GU_ERR_INVALID;
+  gu_config c{};
+  if (cfg) c = *cfg;
+  return reserve_launches(g, c);
 }
 
 int gu_bc_rows_wrap(gu_graph_t g, const int32_t *sources, int32_t nsrc, float *const *delta_rows,

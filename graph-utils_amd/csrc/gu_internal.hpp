@@ -15,6 +15,15 @@ int gu_search_rows_wrap(gu_graph_t g, const int32_t *sources, int32_t nsrc, int3
 int gu_bc_rows_wrap(gu_graph_t g, const int32_t *sources, int32_t nsrc, float *const *delta_rows,
                     double *const *bc_targets, int32_t *wrap, const gu_config *cfg, gu_stats *stats);
 
+// Allocates every workspace buffer that a launch of 1 to 256 sources of each
+// job (search with or without path counts, Brandes, diameter) can need on
+// handle g with configuration cfg, so that those launches allocate no device
+// memory.  The query server calls it for each launch handle when a graph is
+// registered: an allocation made while other streams run stalls their
+// launches and copies, and one that has to wait for memory freed moments
+// before took seconds.
+int gu_reserve_launches(gu_graph_t g, const gu_config *cfg);
+
 // A roctx range over a scope (gu_trace_push / gu_trace_pop): the engine's
 // phases (level chunks, write-backs, Brandes backward, LBS) show up as
 // ranges in `rocprofv3 --marker-trace`, with the kernels they launch.

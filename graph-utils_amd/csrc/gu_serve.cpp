@@ -4,8 +4,9 @@
 // launches of different slots overlap on the device while reading one
 // resident copy of the graph.  A server with a host arena also has one
 // delivery stream, on which every device-to-host copy of results runs.  The
-// output arenas are allocated when the server is created; nothing is
-// allocated per query.
+// output arenas are allocated when the server is created and each slot's
+// search workspace when a graph is registered; nothing is allocated per
+// query.
 #include <hip/hip_runtime.h>
 
 #include <new>
@@ -151,6 +152,10 @@ int gu_server_add_graph(gu_server_t s, gu_graph_t g, int32_t *graph_id) {
   const int gid = s->ex.ngraphs;
   for (int i = 0; i < s->ex.slots; i++) {
     rc = gu_graph_share(g, &s->ex.views[gid][i]);
+    if (!rc) {
+      rc = gu_reserve_launches(s->ex.views[gid][i], &s->ex.base);
+      if (rc) gu_graph_destroy(s->ex.views[gid][i]);
+    }
     if (rc) {
       for (int j = 0; j < i; j++) gu_graph_destroy(s->ex.views[gid][j]);
       return rc;

@@ -389,7 +389,9 @@ typedef struct {
  * per slot and the worker threads. */
 int gu_server_create(const gu_server_config *cfg, gu_server_t *out);
 /* Registers a resident graph (the caller keeps ownership and destroys it
- * after the server). */
+ * after the server) and allocates each slot's search workspace on it, so
+ * that queries allocate no device memory; fails (GU_ERR_NOMEM) when the
+ * device cannot hold that workspace. */
 int gu_server_add_graph(gu_server_t s, gu_graph_t g, int32_t *graph_id);
 /* Queues a query; returns at once.  A query whose outputs exceed the whole
  * budget finishes immediately with status GU_ERR_CAPACITY. */

@@ -277,6 +277,37 @@ def test_server_rejects_oversized_query(gu, gpu):
         assert srv.stats()["rejected"] == 1
 
 
+def test_server_queries_allocate_nothing(gu, gpu):
+    """Registering a graph reserves each slot's search workspace: a stream of
+    every query kind at 1, 17 and 256 sources, on a graph of the batched
+    engine and one of the per-source level-set engine, then leaves the
+    device's free memory where it was (within 16 MiB for the runtime)."""
+    announce("server reserves workspace")
+    hgs = [gu.HostGraph.rmat(16, 16, seed=16), gu.HostGraph.grid(512, 0.1, seed=51)]
+    dgs = [gu.DeviceGraph(h) for h in hgs]
+    try:
+        for dg in dgs:  # loads the kernels; sizes the handles' own workspaces
+            dg.diameter(np.arange(4, dtype=np.int32))
+        with gu.Server(hbm_budget=2 << 30, slots=2) as srv:
+            ids = [srv.add_graph(dg) for dg in dgs]
+            free0 = gu.choose_device(0)["free_mem"]
+            ticks = []
+            for g, hg in enumerate(hgs):
+                for k in (256, 17, 1):
+                    src = hg.sample_sources(k, seed=k)
+                    for kind, fl in ((gu.QUERY_BFS, gu.QF_SIGMA), (gu.QUERY_BFS, 0), (gu.QUERY_BC, 0),
+                                     (gu.QUERY_DIAMETER, 0)):
+                        ticks.append(srv.submit(ids[g], kind, src, fl))
+            for t in ticks:
+                assert srv.wait(t, 60000).status == 0
+                srv.release(t)
+            free1 = gu.choose_device(0)["free_mem"]
+    finally:
+        for dg in dgs:
+            dg.close()
+    assert free0 - free1 <= 16 << 20, (free0, free1)
+
+
 def test_autotune_keeps_results_exact(gu, orc, gpu):
     announce("autotune rmat12 / grid32")
     hg = gu.HostGraph.rmat(12, 16, seed=41)
