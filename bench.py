@@ -447,14 +447,13 @@ def _pct(x, p):
 
 def _warm(gu, srv, ids, dgs, slots, host):
     """Every slot sees every kind at full width on every graph, with results
-    in HBM and, when the server has a host arena, in host memory."""
+    where the stream will ask for them."""
     warm = []
     for g, dg in enumerate(dgs):
         src = np.arange(256, dtype=np.int32) % dg.n
         for kind, fl in ((gu.QUERY_BFS, gu.QF_SIGMA), (gu.QUERY_BC, 0), (gu.QUERY_DIAMETER, 0)):
-            for extra in ((0, gu.QF_HOST) if host else (0,)):
-                for _ in range(slots):
-                    warm.append(srv.submit(ids[g], kind, src, fl | extra))
+            for _ in range(slots):
+                warm.append(srv.submit(ids[g], kind, src, fl | (gu.QF_HOST if host else 0)))
     for t in warm:
         srv.wait(t)
         srv.release(t)
@@ -597,14 +596,16 @@ def serve_stream(args, gu, hgs, tune, plain_host=True):
                 b.close()
             unit[name] = row
         print("serve: unit costs %s" % json.dumps(unit), file=sys.stderr, flush=True)
-        # One server for every leg, its graphs registered (each slot's search
-        # workspace reserved) and warmed once.  The plain legs run while it
-        # is alive and idle: freeing its memory just before them slowed
-        # their per-query allocations by seconds at a time.
+        # One server per destination, configured as it would be deployed
+        # (a host arena only for host results), its graphs registered (each
+        # slot's search workspace reserved) and warmed once; every leg of
+        # that destination, plain one included, runs while it is alive.
+        # Freeing a server just before a plain leg slowed the plain path's
+        # per-query allocations by seconds at a time, and the host arena's
+        # delivery stream is a fifth stream on four hardware queues.
         with gu.Server(hbm_budget=budget, slots=args.slots, max_pack=256, host_budget=host_budget) as srv:
             ids = [srv.add_graph(dg) for dg in dgs]
-            _warm(gu, srv, ids, dgs, args.slots, host_budget > 0)
-            # results to the host
+            _warm(gu, srv, ids, dgs, args.slots, True)
             lat_hs, mk_hs, st_hs = _served(gu, srv, ids, qs, zero, True)
             sat_h = nq / (mk_hs / 1e3)
             qps_h = args.qps if args.qps > 0 else 0.9 * sat_h
@@ -615,7 +616,9 @@ def serve_stream(args, gu, hgs, tune, plain_host=True):
             if plain_host:
                 lat_ph, mk_ph = _plain(gu, dgs, qs, arr_h, True)
                 print("serve: plain (host results) done", file=sys.stderr, flush=True)
-            # results left in HBM
+        with gu.Server(hbm_budget=budget, slots=args.slots, max_pack=256) as srv:
+            ids = [srv.add_graph(dg) for dg in dgs]
+            _warm(gu, srv, ids, dgs, args.slots, False)
             lat_s, mk_s, st_s = _served(gu, srv, ids, qs, zero, False)
             sat_d = nq / (mk_s / 1e3)
             qps_d = args.qps if args.qps > 0 else 0.9 * sat_d
