@@ -645,7 +645,11 @@ def serve_stream(args, gu, hgs, tune, plain_host=True):
             "plain": _leg(lat_pd, mk_pd, nq)}
     devr["ratio"] = round(devr["served"]["qps"] / devr["plain"]["qps"], 2)
     devr["p99_ratio"] = round(devr["plain"]["p99_ms"] / devr["served"]["p99_ms"], 2)
-    res = {"queries": nq, "autotuned": tuned, "launch_ms": unit,
+    res = {"queries": nq, "slots": args.slots,
+           # hardware queues of the process: HIP's default is 4, and a server
+           # with a host arena has one stream more than its slots
+           "gpu_max_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "default (4)"),
+           "autotuned": tuned, "launch_ms": unit,
            "qps": hostr["served"]["qps"], "p50_ms": hostr["served"]["p50_ms"],
            "p99_ms": hostr["served"]["p99_ms"],
            "like_for_like_ratio": hostr.get("like_for_like_ratio"),
